@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Throughput of the MLS-MPM substep on one MI355X.
+
+Workload (default): the liquid dam break -- 1,000,000 particles uniform in
+the box [0.05,0.45]x[0.05,0.65]x[0.05,0.95], grid N=128, dt=5e-5, weakly
+compressible liquid K=4e4, rho=1000, gravity -9.8 (seeded synthetic input,
+mpm_amd.scenes.dam_break). A step is one substep of the engine (grid update +
+fused G2P/P2G, with the periodic cell sort) over all particles, with the
+particle state resident in HBM. Prints one JSON line:
+
+  value        particle-substeps/s (P / seconds per substep), whole job
+  roofline     the dominant kernel's algorithmic bytes per launch divided by
+               its mean launch time (HIP events recorded by the engine on the
+               launch stream during the timed region), against 8 TB/s HBM
+  cpu_baseline the FP32 CPU oracle (a C++/OpenMP restatement of the
+               reference step) on a bounded sample of the same workload
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def alg_bytes(phase: str, P: int, N: int) -> int:
+    """Algorithmic bytes of one launch of a kernel phase (dense FP32 grid):
+    g2p2g   G2P reads x,F,Jp,mat (53 B) and writes x,v,C,F,Jp (100 B); the
+            fused P2G uses the state it just computed, whose algorithmic read
+            (101 B) is counted as in the three-phase step: 254 B/particle;
+            grid: gather of the velocities and write of the scatter, 2 x 16 B/node
+    grid_update  read and write of every node: 2 x 16 B/node
+    """
+    if phase == "g2p2g":
+        return 254 * P + 32 * N ** 3
+    if phase == "grid_update":
+        return 32 * N ** 3
+    return 0
+
+
+def load_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of the kernel from the committed rocprofv3 PMC
+    summary (profiles/traffic_<workload>.json), or None."""
+    f = ROOT / "profiles" / f"traffic_{workload}.json"
+    if not f.exists():
+        return None
+    try:
+        d = json.loads(f.read_text())
+        k = d["kernels"].get(kernel)
+        return None if k is None else int(k["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(scene, steps: int, threads: int):
+    import oracle
+    st = scene.state()
+    oracle.time_advance_f32(scene.params, scene.materials, st, 1, threads)  # warm-up substep
+    secs = oracle.time_advance_f32(scene.params, scene.materials, st, steps, threads)
+    return {
+        "value": scene.n * steps / secs,
+        "unit": "particle-substeps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{scene.name}: all {scene.n} particles, N={scene.params.N}, {steps} FP32 "
+                   f"substeps after 1 warm-up, C++/OpenMP oracle (omp atomic scatter), "
+                   f"{threads} threads, {secs:.2f} s"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="dam_break", choices=["dam_break", "snow_spheres", "elastic_sphere"])
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py runs on one GPU only: the engine advances one simulation per GPU and this "
+              "build has no multi-GPU path; run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import torch
+    import mpm_amd
+    from mpm_amd import Engine, scenes
+
+    scene = {"dam_break": scenes.dam_break, "snow_spheres": scenes.snow_spheres,
+             "elastic_sphere": scenes.elastic_sphere}[args.workload]()
+    P, N = scene.n, scene.params.N
+    st = scene.state()
+    eng = Engine(scene.params)
+    eng.set_materials(scene.materials)
+    eng.add_particles(st["x"], st["material"], st["v"], st["C"], st["F"], st["Jp"])
+    eng.upload()
+    eng.advance(args.warmup)
+    torch.cuda.synchronize()
+
+    # timed region: events on the launch stream bracket each kernel phase
+    eng.set_option("timing", 1)
+    eng.timing_reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.advance(args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    eng.set_option("timing", 0)
+    phases = {}
+    for ph in ("g2p2g", "grid_update", "sort", "p2g", "g2p", "clear"):
+        ms, n = eng.timing(ph)
+        if n:
+            phases[ph] = {"mean_us": 1e3 * ms / n, "launches": n}
+    out = eng.download()
+    eng.close()
+    finite = bool(torch.isfinite(torch.from_numpy(out["x"])).all())
+
+    sec_per_step = (t1 - t0) / args.steps
+    dom = max(phases, key=lambda p: phases[p]["mean_us"] * phases[p]["launches"])
+    mean_s = phases[dom]["mean_us"] * 1e-6
+    achieved = alg_bytes(dom, P, N) / mean_s / 1e9
+    kernel_name = {"g2p2g": "k_g2p2g_tiled", "grid_update": "k_grid_update"}.get(dom, dom)
+
+    result = {
+        "metric": "particle-substeps/s at 1M particles N=128",
+        "value": P / sec_per_step,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec_per_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SplitMix64 scene, no checkpoint)",
+        "config": {
+            "workload": scene.name, "particles": P, "grid_N": N, "dt": scene.params.dt,
+            "material": {"dam_break": "weakly compressible liquid K=4e4 rho=1000",
+                         "snow_spheres": "snow E=1.4e5 nu=0.2 rho=400 xi=10",
+                         "elastic_sphere": "elastic (snow model, clamps [0,1e30]) E=1e4 nu=0.3"}[args.workload],
+            "substep_kernels": phases,
+            "algorithmic_bytes_per_substep": scenes.algorithmic_bytes(P, N),
+            "substep_fraction_of_hbm_roofline":
+                scenes.algorithmic_bytes(P, N) / sec_per_step / (HBM_PEAK_GBS * 1e9),
+            "finite": finite,
+            "device": mpm_amd._lib.device_info(),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kernel_name,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": load_traffic(scene.name, kernel_name),
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        result["cpu_baseline"] = cpu_baseline(scene, args.cpu_steps, threads)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,200 @@
+/*
+ * mpm_hip.h -- C-ABI of the MI355X MLS-MPM substep engine (libmpm_hip.so).
+ *
+ * The reference exposes its simulation step only as a C++ class with no C-ABI
+ * (class Simulation, include/mpm.cuh:44-106, src/mpm.cu:180-394) whose physics
+ * is bound at compile time by three plugin aliases (include/mpm.cuh:24-27):
+ * TransferScheme = MLS_APIC_Scheme, MaterialModel = MMSnow, InterpolationKernel
+ * = QuadraticInterpolationKernel. This header is the flat boundary that replaces
+ * that class: every entry point names the reference method it stands in for.
+ * Plain pointers and sizes only; streams are passed as `void*` (a hipStream_t,
+ * NULL = the device's null stream). Every call returns 0 (MPM_OK) or an error
+ * code, and mpm_last_error() holds the message of the last failure on a handle.
+ *
+ * Layout conventions at this boundary:
+ *   - "SoA" host arrays are per-field arrays: x[n][3], v[n][3], C[n][3][3],
+ *     F[n][3][3] (row-major 3x3: element (r,c) at r*3+c), Jp[n], material[n].
+ *   - "AoS104" is the reference's own particle bytes (MLS_APIC_Particle,
+ *     include/types.h:24-35 + include/TransferScheme.h:46-54): u8 material@0,
+ *     float x[3]@4, v[3]@16, F[9]@28 column-major, C[9]@64 column-major,
+ *     Jp@100; 104 bytes, 4-byte aligned.
+ *   - Downloads return particles in the order they were added, whatever order
+ *     the engine keeps them in on the device.
+ *
+ * Handles are not thread-safe across concurrent calls on the same handle;
+ * distinct handles are independent.
+ */
+#ifndef MPM_HIP_H
+#define MPM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_ABI_VERSION 1
+
+/* status codes */
+#define MPM_OK 0
+#define MPM_ERR_INVALID 1   /* bad argument (null pointer, N out of range, ...) */
+#define MPM_ERR_HIP 2       /* a HIP runtime call failed */
+#define MPM_ERR_STATE 3     /* call order violated (e.g. advance before upload) */
+#define MPM_ERR_CAPACITY 4  /* size beyond what the handle can hold */
+#define MPM_ERR_IO 5        /* file could not be read or written */
+#define MPM_ERR_NOGPU 6     /* no HIP device visible */
+
+/* material kinds.  SNOW is the reference's only bound model
+ * (MMSnow, include/MaterialModel.cuh:66-115; "elastic" scenes are SNOW with
+ * hardening 0 and clamps [0, 1e30], scenes/rubber_duck.toml:1-8).
+ * FIXED_COROTATED and JELLY are the reference's other plugin classes
+ * (MaterialModel.cuh:39-64, :118-150).  WC_LIQUID is a weakly compressible
+ * J-based liquid with no reference counterpart: tau = K (J-1) J I, J kept in
+ * the Jp slot, F reset to J^(1/3) I after each substep. */
+#define MPM_MATERIAL_SNOW 0
+#define MPM_MATERIAL_FIXED_COROTATED 1
+#define MPM_MATERIAL_JELLY 2
+#define MPM_MATERIAL_WC_LIQUID 3
+
+/* boundary modes for the grid update (reference: mpm.cu:92-105) */
+#define MPM_BOUNDARY_FRACTION 0 /* walls where node_index/N < b or > 1-b (reference, b=0.05) */
+#define MPM_BOUNDARY_CELLS 1    /* walls on the first/last b node layers of each axis */
+
+/* compatibility with result-changing details of the reference step */
+#define MPM_COMPAT_REFERENCE 0 /* reproduce them (the default for parity) */
+#define MPM_COMPAT_FIXED 1     /* every particle updated, P2G clipped, J = det F */
+
+/* Simulation parameters.  Replaces SimulationParameters
+ * (include/TransferScheme.h:6-29) plus the constants the reference hard-codes
+ * in src/mpm.cu:6 (gravity) and :92 (boundary). dx and 1/dx are derived as
+ * float(1.0/N) and float(1.0/dx), as TransferScheme.h:26-27 does. */
+typedef struct mpm_params {
+  float dt;              /* substep length (options.h --dt, default 1e-4) */
+  uint32_t N;            /* grid nodes per axis, 4 <= N <= 1024 (options.h -N) */
+  float gravity;         /* y acceleration; the reference uses -9.81 */
+  int32_t boundary_mode; /* MPM_BOUNDARY_* */
+  float boundary;        /* fraction (0.05) or node-layer count */
+  int32_t compat;        /* MPM_COMPAT_* */
+} mpm_params;
+
+/* One material.  Replaces the MaterialModel plugin instance
+ * (MaterialModelBase + MMFixedCorotated + MMSnow fields,
+ * include/MaterialModel.cuh:18-35,47-48,70-72). Build it with
+ * mpm_material_init() to get the reference's float arithmetic for the
+ * Lame constants. */
+typedef struct mpm_material {
+  int32_t kind;          /* MPM_MATERIAL_* */
+  float particle_volume; /* MaterialModelBase::particleVolume */
+  float particle_mass;   /* density * volume */
+  float mu0;             /* E / (2 (1 + nu)) */
+  float lambda0;         /* E nu / ((1 + nu)(1 - 2 nu)) */
+  float hardening;       /* xi in exp(xi (1 - Jp)) */
+  float clamp_lo;        /* plasticity clamp on singular values */
+  float clamp_hi;
+  float bulk_modulus;    /* K, WC_LIQUID only */
+} mpm_material;
+
+/* Fill *m the way the reference's constructors do (MaterialModel.cuh:27-29,
+ * :50-54, :75-82; main.cu:36-42 passes the TOML doubles as float). */
+int mpm_material_init(mpm_material* m, int32_t kind, double volume, double density,
+                      double E, double nu, double hardening, double clamp_lo,
+                      double clamp_hi, double bulk_modulus);
+
+/* ---- lifetime (replaces Simulation::Simulation / ~Simulation, mpm.cu:180-195) ---- */
+typedef struct mpm_sim* mpm_handle;
+
+/* Create a handle on the current HIP device. */
+int mpm_create(const mpm_params* params, mpm_handle* out);
+/* Release all device and host memory of the handle. NULL is accepted. */
+int mpm_destroy(mpm_handle h);
+/* Message of the last failed call on h (h == NULL: last failure of
+ * mpm_create or any call given a NULL handle). Never NULL. */
+const char* mpm_last_error(mpm_handle h);
+/* ABI version of the library (MPM_ABI_VERSION). */
+int mpm_abi_version(void);
+
+/* ---- materials (replaces the material vector handed to Simulation,
+ *      main.cu:9,25,45 and its upload in initCuda, mpm.cu:198-201) ---- */
+int mpm_set_materials(mpm_handle h, const mpm_material* materials, int32_t n);
+
+/* ---- particles (replaces Simulation::addObject/addParticles host staging,
+ *      mpm.cu:237-251,347-394, and initCuda's upload, mpm.cu:197-207,278-286) ---- */
+/* Append n particles to the host staging set. x and material are required;
+ * v, C, F, Jp may be NULL (defaults: v = 0, C = 0, F = I, Jp = 1, as
+ * MLS_APIC_Particle's constructor sets them, TransferScheme.h:50-53). */
+int mpm_add_particles_soa(mpm_handle h, int64_t n, const float* x, const float* v,
+                          const float* C, const float* F, const float* Jp,
+                          const uint8_t* material);
+/* Append n particles given in the reference's 104-byte layout. */
+int mpm_add_particles_aos104(mpm_handle h, const void* particles, int64_t n);
+/* Drop the host staging set (device state is kept until the next upload). */
+int mpm_clear_particles(mpm_handle h);
+/* Copy the staging set to the device (allocating as needed) and reset the
+ * simulated time to 0 (Simulation::initCuda, mpm.cu:197-207). */
+int mpm_upload(mpm_handle h, void* stream);
+/* Number of particles on the device (Simulation::getFullParticleCount,
+ * mpm.cu:253-259, for the uploaded set). */
+int mpm_count(mpm_handle h, int64_t* out);
+
+/* ---- the substep (replaces Simulation::advance, mpm.cu:323-329) ---- */
+/* Run `substeps` substeps, enqueued on `stream`; returns without waiting. */
+int mpm_advance(mpm_handle h, int32_t substeps, void* stream);
+/* Simulated time (Simulation::t, accumulated in double as mpm.cu:328 does). */
+int mpm_time(mpm_handle h, double* out);
+
+/* ---- download (replaces Simulation::syncDevice/particlesToHost,
+ *      mpm.cu:209-211,288-306) ---- blocking; NULL outputs are skipped. */
+int mpm_download_soa(mpm_handle h, float* x, float* v, float* C, float* F, float* Jp,
+                     uint8_t* material, void* stream);
+int mpm_download_aos104(mpm_handle h, void* particles, void* stream);
+/* Grid after the last grid update: N^3 nodes x (vx, vy, vz, m) float, node
+ * index N*N*i + N*j + k (reference layout, mpm.cu:49,55,66). After a grid
+ * update the reference leaves m = 1 on nodes that had mass (mpm.cu:86-88). */
+int mpm_download_grid(mpm_handle h, float* grid, void* stream);
+
+/* ---- full-state checkpoint (no reference counterpart: the reference writes
+ *      only x and v, particle_writer.h:32-37). Binary, little-endian. ---- */
+int mpm_save_state(mpm_handle h, const char* path);
+int mpm_load_state(mpm_handle h, const char* path);
+
+/* ---- tuning knobs (integer options; see the library source for names) ---- */
+int mpm_set_option(mpm_handle h, const char* name, int64_t value);
+int mpm_get_option(mpm_handle h, const char* name, int64_t* value);
+
+/* ---- per-kernel timing (no reference counterpart) ----
+ * With option "timing" = 1 the engine records a HIP event pair on the launch
+ * stream around each of its kernel phases. mpm_timing_read waits for the
+ * recorded events and returns the accumulated milliseconds and launch count
+ * of one phase: "grid_update", "g2p2g", "p2g", "g2p", "sort" or "clear". */
+int mpm_timing_read(mpm_handle h, const char* phase, double* total_ms, int64_t* launches);
+int mpm_timing_reset(mpm_handle h);
+
+/* ---- kernel-level entry points used by the parity tests ---- */
+/* Run only the particle-to-grid scatter of the current state into a cleared
+ * grid and copy the raw grid (momentum, mass: no normalisation) to host
+ * (the work of particleToGrid, mpm.cu:14-74). Does not change particles. */
+int mpm_debug_p2g(mpm_handle h, float* grid, void* stream);
+/* Stable cell-key sort of the current particle positions: writes the keys
+ * and the permutation (original indices in sorted order) that the engine
+ * uses to reorder its particles. keys/perm hold n entries. */
+int mpm_debug_sort(mpm_handle h, uint32_t* keys, uint32_t* perm, void* stream);
+/* 3x3 SVD of n row-major matrices on the device (the McAdams routine the
+ * reference vendors, include/svd3_cuda.h:35-1043): U, V rotations, sigma
+ * sorted by magnitude with the sign of det A on sigma[2]. Host pointers. */
+int mpm_debug_svd(int64_t n, const float* A, float* U, float* sigma, float* V);
+
+/* Runtime and device description for logs: HIP runtime and driver versions,
+ * device name, architecture and CU count, and whether this library's device
+ * code object loads on the device (no kernel is launched). No reference
+ * counterpart. Returns MPM_ERR_NOGPU when no device is visible. */
+int mpm_device_info(char* buf, int32_t len);
+/* Runs a fixed sequence of small device operations (copies both ways, a fill
+ * kernel, the SVD kernel on three 3x3 matrices, a pinned copy) and writes
+ * the status of each into buf. No reference counterpart; for bring-up logs. */
+int mpm_debug_selftest(char* buf, int32_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPM_HIP_H */

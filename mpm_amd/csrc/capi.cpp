@@ -1,0 +1,335 @@
+// capi.cpp -- the C-ABI of libmpm_hip.so (declared in include/mpm_hip.h).
+//
+// Host side of the engine: owns the device buffers of one simulation, stages
+// particles on the host, and enqueues the substep kernels. It stands in for
+// the reference's class Simulation (include/mpm.cuh:44-106,
+// src/mpm.cu:180-394); each entry point names the method it replaces in the
+// header.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mpm_hip.h"
+#include "engine.hpp"
+#include "kernels.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;  // for calls without a usable handle
+
+void set_global_error(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+}  // namespace
+
+using mpm_engine::Sim;
+
+// ---------------------------------------------------------------------------
+// material construction (MaterialModel.cuh:27-29,50-54,75-82; the TOML values
+// are doubles converted to the constructors' float parameters, main.cu:36-42)
+// ---------------------------------------------------------------------------
+extern "C" int mpm_material_init(mpm_material* m, int32_t kind, double volume, double density,
+                                 double E, double nu, double hardening, double clamp_lo,
+                                 double clamp_hi, double bulk_modulus) {
+  if (!m) {
+    set_global_error("mpm_material_init: null material");
+    return MPM_ERR_INVALID;
+  }
+  if (kind < MPM_MATERIAL_SNOW || kind > MPM_MATERIAL_WC_LIQUID) {
+    set_global_error("mpm_material_init: unknown material kind %d", kind);
+    return MPM_ERR_INVALID;
+  }
+  const float fv = (float)volume, fd = (float)density, fE = (float)E, fnu = (float)nu;
+  std::memset(m, 0, sizeof(*m));
+  m->kind = kind;
+  m->particle_volume = fv;
+  m->particle_mass = fd * fv;
+  m->mu0 = fE / (2.0f * (1.0f + fnu));
+  m->lambda0 = fE * fnu / ((1.0f + fnu) * (1.0f - 2.0f * fnu));
+  m->hardening = (float)hardening;
+  m->clamp_lo = (float)clamp_lo;
+  m->clamp_hi = (float)clamp_hi;
+  m->bulk_modulus = (float)bulk_modulus;
+  return MPM_OK;
+}
+
+extern "C" int mpm_abi_version(void) { return MPM_ABI_VERSION; }
+
+extern "C" const char* mpm_last_error(mpm_handle h) {
+  if (!h) return g_last_error.c_str();
+  return reinterpret_cast<Sim*>(h)->last_error().c_str();
+}
+
+extern "C" int mpm_create(const mpm_params* params, mpm_handle* out) {
+  if (!params || !out) {
+    set_global_error("mpm_create: null argument");
+    return MPM_ERR_INVALID;
+  }
+  *out = nullptr;
+  std::string why;
+  if (!mpm_engine::validate_params(*params, &why)) {
+    set_global_error("mpm_create: %s", why.c_str());
+    return MPM_ERR_INVALID;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    set_global_error("mpm_create: no HIP device visible");
+    return MPM_ERR_NOGPU;
+  }
+  Sim* s = new Sim(*params);
+  const int rc = s->init_device();
+  if (rc != MPM_OK) {
+    set_global_error("mpm_create: %s", s->last_error().c_str());
+    delete s;
+    return rc;
+  }
+  *out = reinterpret_cast<mpm_handle>(s);
+  return MPM_OK;
+}
+
+extern "C" int mpm_destroy(mpm_handle h) {
+  delete reinterpret_cast<Sim*>(h);
+  return MPM_OK;
+}
+
+#define MPM_GET(h)                                             \
+  if (!h) {                                                    \
+    set_global_error("%s: null handle", __func__);             \
+    return MPM_ERR_INVALID;                                    \
+  }                                                            \
+  Sim* s = reinterpret_cast<Sim*>(h);                          \
+  std::lock_guard<std::mutex> lock_(s->mutex());
+
+extern "C" int mpm_set_materials(mpm_handle h, const mpm_material* mats, int32_t n) {
+  MPM_GET(h);
+  return s->set_materials(mats, n);
+}
+
+extern "C" int mpm_add_particles_soa(mpm_handle h, int64_t n, const float* x, const float* v,
+                                     const float* C, const float* F, const float* Jp,
+                                     const uint8_t* material) {
+  MPM_GET(h);
+  return s->add_particles_soa(n, x, v, C, F, Jp, material);
+}
+
+extern "C" int mpm_add_particles_aos104(mpm_handle h, const void* particles, int64_t n) {
+  MPM_GET(h);
+  return s->add_particles_aos104(particles, n);
+}
+
+extern "C" int mpm_clear_particles(mpm_handle h) {
+  MPM_GET(h);
+  return s->clear_staging();
+}
+
+extern "C" int mpm_upload(mpm_handle h, void* stream) {
+  MPM_GET(h);
+  return s->upload((hipStream_t)stream);
+}
+
+extern "C" int mpm_count(mpm_handle h, int64_t* out) {
+  MPM_GET(h);
+  if (!out) return s->fail(MPM_ERR_INVALID, "mpm_count: null output");
+  *out = s->count();
+  return MPM_OK;
+}
+
+extern "C" int mpm_advance(mpm_handle h, int32_t substeps, void* stream) {
+  MPM_GET(h);
+  return s->advance(substeps, (hipStream_t)stream);
+}
+
+extern "C" int mpm_time(mpm_handle h, double* out) {
+  MPM_GET(h);
+  if (!out) return s->fail(MPM_ERR_INVALID, "mpm_time: null output");
+  *out = s->time();
+  return MPM_OK;
+}
+
+extern "C" int mpm_download_soa(mpm_handle h, float* x, float* v, float* C, float* F, float* Jp,
+                                uint8_t* material, void* stream) {
+  MPM_GET(h);
+  return s->download_soa(x, v, C, F, Jp, material, (hipStream_t)stream);
+}
+
+extern "C" int mpm_download_aos104(mpm_handle h, void* particles, void* stream) {
+  MPM_GET(h);
+  return s->download_aos104(particles, (hipStream_t)stream);
+}
+
+extern "C" int mpm_download_grid(mpm_handle h, float* grid, void* stream) {
+  MPM_GET(h);
+  return s->download_grid(grid, (hipStream_t)stream);
+}
+
+extern "C" int mpm_save_state(mpm_handle h, const char* path) {
+  MPM_GET(h);
+  return s->save_state(path);
+}
+
+extern "C" int mpm_load_state(mpm_handle h, const char* path) {
+  MPM_GET(h);
+  return s->load_state(path);
+}
+
+extern "C" int mpm_set_option(mpm_handle h, const char* name, int64_t value) {
+  MPM_GET(h);
+  return s->set_option(name, value);
+}
+
+extern "C" int mpm_get_option(mpm_handle h, const char* name, int64_t* value) {
+  MPM_GET(h);
+  return s->get_option(name, value);
+}
+
+extern "C" int mpm_timing_read(mpm_handle h, const char* phase, double* total_ms, int64_t* launches) {
+  MPM_GET(h);
+  return s->timing_read(phase, total_ms, launches);
+}
+
+extern "C" int mpm_timing_reset(mpm_handle h) {
+  MPM_GET(h);
+  return s->timing_reset();
+}
+
+extern "C" int mpm_debug_p2g(mpm_handle h, float* grid, void* stream) {
+  MPM_GET(h);
+  return s->debug_p2g(grid, (hipStream_t)stream);
+}
+
+extern "C" int mpm_debug_sort(mpm_handle h, uint32_t* keys, uint32_t* perm, void* stream) {
+  MPM_GET(h);
+  return s->debug_sort(keys, perm, (hipStream_t)stream);
+}
+
+extern "C" int mpm_debug_svd(int64_t n, const float* A, float* U, float* sigma, float* V) {
+  if (n < 0 || (n > 0 && (!A || !U || !sigma || !V))) {
+    set_global_error("mpm_debug_svd: bad arguments");
+    return MPM_ERR_INVALID;
+  }
+  if (n == 0) return MPM_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    set_global_error("mpm_debug_svd: no HIP device visible");
+    return MPM_ERR_NOGPU;
+  }
+  // device layout: A (9n) | U (9n) | sigma (3n) | V (9n)
+  float* d = nullptr;
+  const size_t bytes = (size_t)n * 30 * sizeof(float);
+  hipError_t e = hipMalloc(&d, bytes);
+  if (e != hipSuccess) {
+    set_global_error("mpm_debug_svd: hipMalloc: %s", hipGetErrorString(e));
+    return MPM_ERR_HIP;
+  }
+  float *dA = d, *dU = d + 9 * n, *dS = d + 18 * n, *dV = d + 21 * n;
+  const char* step = "hipMemcpy(host to device)";
+  e = hipMemcpy(dA, A, (size_t)n * 9 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) { step = "kernel launch"; e = mpmk::svd_batch(dA, dU, dS, dV, n, nullptr); }
+  if (e == hipSuccess) { step = "hipMemcpy(device to host)"; e = hipMemcpy(U, dU, (size_t)n * 9 * sizeof(float), hipMemcpyDeviceToHost); }
+  if (e == hipSuccess) e = hipMemcpy(sigma, dS, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(V, dV, (size_t)n * 9 * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) {
+    set_global_error("mpm_debug_svd: %s: %s", step, hipGetErrorString(e));
+    return MPM_ERR_HIP;
+  }
+  return MPM_OK;
+}
+
+extern "C" int mpm_device_info(char* buf, int32_t len) {
+  if (!buf || len <= 0) {
+    set_global_error("mpm_device_info: null buffer");
+    return MPM_ERR_INVALID;
+  }
+  int rt = 0, drv = 0, ndev = 0, dev = 0;
+  (void)hipRuntimeGetVersion(&rt);
+  (void)hipDriverGetVersion(&drv);
+  const hipError_t ce = hipGetDeviceCount(&ndev);
+  std::string out = "hip_runtime=" + std::to_string(rt) + " hip_driver=" + std::to_string(drv) +
+                    " devices=" + std::to_string(ndev) + " count_status=" + hipGetErrorString(ce);
+  if (ce == hipSuccess && ndev > 0 && hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) == hipSuccess) {
+      out += std::string(" name=") + p.name + " arch=" + p.gcnArchName +
+             " cus=" + std::to_string(p.multiProcessorCount) +
+             " lds_per_block=" + std::to_string(p.sharedMemPerBlock);
+    }
+    int nregs = -1;
+    const hipError_t fe = mpmk::probe_kernel_image(&nregs);
+    out += std::string(" kernel_image=") + hipGetErrorString(fe) + " svd_kernel_vgprs=" + std::to_string(nregs);
+  }
+  (void)hipGetLastError();
+  std::snprintf(buf, (size_t)len, "%s", out.c_str());
+  return (ce == hipSuccess && ndev > 0) ? MPM_OK : MPM_ERR_NOGPU;
+}
+
+extern "C" int mpm_debug_selftest(char* buf, int32_t len) {
+  if (!buf || len <= 0) {
+    set_global_error("mpm_debug_selftest: null buffer");
+    return MPM_ERR_INVALID;
+  }
+  std::string out;
+  auto rec = [&](const char* what, hipError_t e) {
+    out += std::string(what) + "=" + hipGetErrorName(e) + "; ";
+    return e == hipSuccess;
+  };
+  int ndev = 0;
+  if (!rec("count", hipGetDeviceCount(&ndev)) || ndev <= 0) {
+    std::snprintf(buf, (size_t)len, "%s", out.c_str());
+    return MPM_ERR_NOGPU;
+  }
+  float host[256], back[256];
+  for (int i = 0; i < 256; ++i) host[i] = (float)i + 0.5f;
+  float* d = nullptr;
+  bool ok = rec("malloc", hipMalloc(&d, 4096));
+  if (ok) ok = rec("h2d", hipMemcpy(d, host, sizeof(host), hipMemcpyHostToDevice));
+  if (ok) {
+    ok = rec("d2h", hipMemcpy(back, d, sizeof(back), hipMemcpyDeviceToHost));
+    out += std::string("d2h_match=") + (std::memcmp(host, back, sizeof(host)) == 0 ? "yes" : "no") + "; ";
+  }
+  if (ok) {
+    rec("fill_launch", mpmk::fill_zero((float4*)d, 64, nullptr));
+    rec("fill_sync", hipDeviceSynchronize());
+    ok = rec("fill_d2h", hipMemcpy(back, d, sizeof(back), hipMemcpyDeviceToHost));
+    bool zero = true;
+    for (int i = 0; i < 256; ++i) zero = zero && back[i] == 0.0f;
+    out += std::string("fill_zero=") + (zero ? "yes" : "no") + "; ";
+  }
+  if (ok) {
+    float A[27] = {1, 2, 1, 1, 3, 1, 1, 8, 1, 0, 1, 0, -1, 2, -1, -1, 0.001f, -1, 1, 0, 0, 0, 1, 0, 0, 0, 1};
+    ok = rec("svd_h2d", hipMemcpy(d, A, sizeof(A), hipMemcpyHostToDevice));
+    if (ok) rec("svd_launch", mpmk::svd_batch(d, d + 27, d + 54, d + 63, 3, nullptr));
+    rec("svd_sync", hipDeviceSynchronize());
+    rec("svd_lasterror", hipGetLastError());
+    ok = rec("svd_d2h", hipMemcpy(back, d, 90 * sizeof(float), hipMemcpyDeviceToHost));
+    if (ok) {
+      char tmp[128];
+      std::snprintf(tmp, sizeof(tmp), "sigma0=%g,%g,%g; ", back[54], back[55], back[56]);
+      out += tmp;
+    }
+  }
+  float* pinned = nullptr;
+  if (rec("hostmalloc", hipHostMalloc((void**)&pinned, 1024, 0))) {
+    rec("d2h_pinned", hipMemcpy(pinned, d, 1024, hipMemcpyDeviceToHost));
+    (void)hipHostFree(pinned);
+  }
+  if (d) rec("free", hipFree(d));
+  (void)hipGetLastError();
+  std::snprintf(buf, (size_t)len, "%s", out.c_str());
+  return MPM_OK;
+}

@@ -1,0 +1,658 @@
+// engine.cpp -- host orchestration of one simulation handle.
+//
+// Reference counterparts (kekeblom/mpm):
+//   Sim::upload          Simulation::initCuda / particlesToDevice, src/mpm.cu:197-207,278-286
+//   Sim::advance         Simulation::advance, src/mpm.cu:323-329 (t += dt in double, :328)
+//   Sim::download_*      Simulation::syncDevice / particlesToHost, src/mpm.cu:209-211,288-306
+//   make_consts          SimulationParameters, include/TransferScheme.h:22-28
+//   make_material_dev    MaterialModel fields + p2g_prepare_particle's stress factor,
+//                        include/TransferScheme.h:83
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+#include "kernels.hpp"
+
+#define HIPCHK(expr, what)                 \
+  do {                                     \
+    hipError_t e_ = (expr);                \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+namespace mpm_engine {
+
+bool validate_params(const mpm_params& p, std::string* why) {
+  if (!(p.dt > 0.0f) || !std::isfinite(p.dt)) { *why = "dt must be positive and finite"; return false; }
+  if (p.N < 4 || p.N > 1024) { *why = "N must be in [4, 1024]"; return false; }
+  if (!std::isfinite(p.gravity)) { *why = "gravity must be finite"; return false; }
+  if (p.boundary_mode != MPM_BOUNDARY_FRACTION && p.boundary_mode != MPM_BOUNDARY_CELLS) {
+    *why = "unknown boundary mode";
+    return false;
+  }
+  if (!std::isfinite(p.boundary) || p.boundary < 0.0f) { *why = "boundary must be >= 0"; return false; }
+  if (p.boundary_mode == MPM_BOUNDARY_CELLS && p.boundary != std::floor(p.boundary)) {
+    *why = "boundary must be a whole number of node layers in MPM_BOUNDARY_CELLS mode";
+    return false;
+  }
+  if (p.compat != MPM_COMPAT_REFERENCE && p.compat != MPM_COMPAT_FIXED) { *why = "unknown compat mode"; return false; }
+  return true;
+}
+
+StepConsts make_consts(const mpm_params& p) {
+  StepConsts k{};
+  k.N = (int32_t)p.N;
+  k.dt = p.dt;
+  k.dx = (float)(1.0 / (double)p.N);          // TransferScheme.h:26
+  k.dx_inv = (float)(1.0 / (double)k.dx);     // TransferScheme.h:27
+  k.dinv = (4.0f * k.dx_inv) * k.dx_inv;      // InterpolationKernel.cuh:72
+  k.gravity = p.gravity;
+  k.boundary_mode = p.boundary_mode;
+  k.bfrac = p.boundary;
+  k.bcells = (int32_t)p.boundary;
+  k.compat = p.compat;
+  const uint64_t n3 = (uint64_t)p.N * p.N * p.N;
+  k.guard = (p.compat == MPM_COMPAT_REFERENCE) ? (uint32_t)n3 : 0xffffffffu;  // mpm.cu:114
+  return k;
+}
+
+MaterialDev make_material_dev(const mpm_material& m, const StepConsts& k) {
+  MaterialDev d{};
+  d.kind = m.kind;
+  d.volume = m.particle_volume;
+  d.mass = m.particle_mass;
+  d.mu0 = m.mu0;
+  d.lambda0 = m.lambda0;
+  d.hardening = m.hardening;
+  d.clamp_lo = m.clamp_lo;
+  d.clamp_hi = m.clamp_hi;
+  d.bulk = m.bulk_modulus;
+  d.stress_scale = ((-k.dinv) * k.dt) * m.particle_volume;  // TransferScheme.h:83
+  return d;
+}
+
+Sim::Sim(const mpm_params& p) : prm_(p), k_(make_consts(p)) {}
+
+Sim::~Sim() {
+  for (int b = 0; b < 2; ++b) {
+    if (d_f_[b]) (void)hipFree(d_f_[b]);
+    if (d_mat_[b]) (void)hipFree(d_mat_[b]);
+    if (d_id_[b]) (void)hipFree(d_id_[b]);
+    if (d_grid_[b]) (void)hipFree(d_grid_[b]);
+    if (d_keys_[b]) (void)hipFree(d_keys_[b]);
+    if (d_vals_[b]) (void)hipFree(d_vals_[b]);
+  }
+  if (d_hist_) (void)hipFree(d_hist_);
+  if (d_mats_) (void)hipFree(d_mats_);
+  for (auto& p : ev_) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+  for (auto ev : ev_pool_) (void)hipEventDestroy(ev);
+  for (auto ev : open_) if (ev) (void)hipEventDestroy(ev);
+}
+
+const char* phase_name(int p) {
+  static const char* names[NUM_PHASES] = {"grid_update", "g2p2g", "p2g", "g2p", "sort", "clear"};
+  return (p >= 0 && p < NUM_PHASES) ? names[p] : "?";
+}
+
+int Sim::phase_begin(int ph, hipStream_t s) {
+  if (!opt_.timing) return MPM_OK;
+  hipEvent_t ev;
+  if (!ev_pool_.empty()) { ev = ev_pool_.back(); ev_pool_.pop_back(); }
+  else HIPCHK(hipEventCreate(&ev), "hipEventCreate");
+  HIPCHK(hipEventRecord(ev, s), "hipEventRecord");
+  open_[ph] = ev;
+  return MPM_OK;
+}
+
+int Sim::phase_end(int ph, hipStream_t s) {
+  if (!opt_.timing || !open_[ph]) return MPM_OK;
+  hipEvent_t ev;
+  if (!ev_pool_.empty()) { ev = ev_pool_.back(); ev_pool_.pop_back(); }
+  else HIPCHK(hipEventCreate(&ev), "hipEventCreate");
+  HIPCHK(hipEventRecord(ev, s), "hipEventRecord");
+  ev_.push_back({ph, open_[ph], ev});
+  open_[ph] = nullptr;
+  return MPM_OK;
+}
+
+int Sim::timing_read(const char* phase, double* total_ms, int64_t* launches) {
+  if (!phase || !total_ms || !launches) return fail(MPM_ERR_INVALID, "mpm_timing_read: null argument");
+  for (auto& p : ev_) {
+    HIPCHK(hipEventSynchronize(p.b), "hipEventSynchronize");
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+    phase_ms_[p.phase] += ms;
+    phase_n_[p.phase] += 1;
+    ev_pool_.push_back(p.a);
+    ev_pool_.push_back(p.b);
+  }
+  ev_.clear();
+  for (int ph = 0; ph < NUM_PHASES; ++ph)
+    if (std::string(phase) == phase_name(ph)) {
+      *total_ms = phase_ms_[ph];
+      *launches = phase_n_[ph];
+      return MPM_OK;
+    }
+  return fail(MPM_ERR_INVALID, "mpm_timing_read: unknown phase '%s'", phase);
+}
+
+int Sim::timing_reset() {
+  double d;
+  int64_t n;
+  int rc = timing_read("g2p2g", &d, &n);  // drain pending events
+  if (rc) return rc;
+  for (int ph = 0; ph < NUM_PHASES; ++ph) { phase_ms_[ph] = 0.0; phase_n_[ph] = 0; }
+  return MPM_OK;
+}
+
+int Sim::fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  err_ = buf;
+  return code;
+}
+
+int Sim::hip_fail(hipError_t e, const char* what) {
+  return fail(MPM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+
+int Sim::init_device() {
+  HIPCHK(hipGetDevice(&device_), "hipGetDevice");
+  HIPCHK(hipMalloc(&d_mats_, sizeof(MaterialDev) * MPM_MAX_MATERIALS), "hipMalloc(materials)");
+  return ensure_grids();
+}
+
+int Sim::ensure_grids() {
+  const size_t nodes = (size_t)prm_.N * prm_.N * prm_.N;
+  if (d_grid_[0] && grid_nodes_ == nodes) return MPM_OK;
+  for (int b = 0; b < 2; ++b) {
+    if (d_grid_[b]) (void)hipFree(d_grid_[b]);
+    d_grid_[b] = nullptr;
+    HIPCHK(hipMalloc(&d_grid_[b], nodes * sizeof(float4)), "hipMalloc(grid)");
+    HIPCHK(hipMemset(d_grid_[b], 0, nodes * sizeof(float4)), "hipMemset(grid)");
+  }
+  grid_nodes_ = nodes;
+  return MPM_OK;
+}
+
+int Sim::set_materials(const mpm_material* m, int32_t n) {
+  if (n < 0 || n > MPM_MAX_MATERIALS || (n > 0 && !m))
+    return fail(MPM_ERR_INVALID, "mpm_set_materials: need 0..%d materials", MPM_MAX_MATERIALS);
+  for (int32_t i = 0; i < n; ++i) {
+    if (m[i].kind < MPM_MATERIAL_SNOW || m[i].kind > MPM_MATERIAL_WC_LIQUID)
+      return fail(MPM_ERR_INVALID, "mpm_set_materials: material %d has unknown kind %d", i, m[i].kind);
+    if (!(m[i].particle_mass > 0.0f) || !(m[i].particle_volume > 0.0f))
+      return fail(MPM_ERR_INVALID, "mpm_set_materials: material %d needs positive volume and mass", i);
+  }
+  mats_.assign(m, m + n);
+  mats_dirty_ = true;
+  primed_ = false;
+  return MPM_OK;
+}
+
+int Sim::upload_materials() {
+  if (!mats_dirty_) return MPM_OK;
+  std::vector<MaterialDev> d(MPM_MAX_MATERIALS);
+  for (size_t i = 0; i < mats_.size(); ++i) d[i] = make_material_dev(mats_[i], k_);
+  HIPCHK(hipMemcpy(d_mats_, d.data(), sizeof(MaterialDev) * d.size(), hipMemcpyHostToDevice),
+         "upload materials");
+  mats_dirty_ = false;
+  return MPM_OK;
+}
+
+int Sim::add_particles_soa(int64_t n, const float* x, const float* v, const float* C,
+                           const float* F, const float* Jp, const uint8_t* mat) {
+  if (n < 0) return fail(MPM_ERR_INVALID, "mpm_add_particles_soa: negative count");
+  if (n == 0) return MPM_OK;
+  if (!x || !mat) return fail(MPM_ERR_INVALID, "mpm_add_particles_soa: x and material are required");
+  if ((int64_t)hJp_.size() + n > (int64_t)0x7fffffff - 64)
+    return fail(MPM_ERR_CAPACITY, "mpm_add_particles_soa: more than 2^31 particles");
+  const size_t o = hJp_.size();
+  hx_.resize(3 * (o + n));
+  hv_.resize(3 * (o + n));
+  hC_.resize(9 * (o + n));
+  hF_.resize(9 * (o + n));
+  hJp_.resize(o + n);
+  hmat_.resize(o + n);
+  std::memcpy(&hx_[3 * o], x, sizeof(float) * 3 * n);
+  if (v) std::memcpy(&hv_[3 * o], v, sizeof(float) * 3 * n);
+  else std::fill(hv_.begin() + 3 * o, hv_.end(), 0.0f);
+  if (C) std::memcpy(&hC_[9 * o], C, sizeof(float) * 9 * n);
+  else std::fill(hC_.begin() + 9 * o, hC_.end(), 0.0f);
+  if (F) {
+    std::memcpy(&hF_[9 * o], F, sizeof(float) * 9 * n);
+  } else {
+    for (int64_t i = 0; i < n; ++i)
+      for (int a = 0; a < 9; ++a) hF_[9 * (o + i) + a] = (a % 4 == 0) ? 1.0f : 0.0f;
+  }
+  if (Jp) std::memcpy(&hJp_[o], Jp, sizeof(float) * n);
+  else std::fill(hJp_.begin() + o, hJp_.end(), 1.0f);
+  std::memcpy(&hmat_[o], mat, n);
+  return MPM_OK;
+}
+
+// Reference particle bytes: include/types.h:24-35, include/TransferScheme.h:46-54
+// (u8 material@0, x@4, v@16, F@28 column-major, C@64 column-major, Jp@100).
+int Sim::add_particles_aos104(const void* p, int64_t n) {
+  if (n < 0 || (n > 0 && !p)) return fail(MPM_ERR_INVALID, "mpm_add_particles_aos104: bad arguments");
+  const uint8_t* b = (const uint8_t*)p;
+  std::vector<float> x(3 * n), v(3 * n), C(9 * n), F(9 * n), Jp(n);
+  std::vector<uint8_t> mat(n);
+  for (int64_t i = 0; i < n; ++i) {
+    const uint8_t* r = b + 104 * i;
+    mat[i] = r[0];
+    std::memcpy(&x[3 * i], r + 4, 12);
+    std::memcpy(&v[3 * i], r + 16, 12);
+    float fc[9], cc[9];
+    std::memcpy(fc, r + 28, 36);
+    std::memcpy(cc, r + 64, 36);
+    for (int row = 0; row < 3; ++row)
+      for (int col = 0; col < 3; ++col) {
+        F[9 * i + 3 * row + col] = fc[3 * col + row];
+        C[9 * i + 3 * row + col] = cc[3 * col + row];
+      }
+    std::memcpy(&Jp[i], r + 100, 4);
+  }
+  return add_particles_soa(n, x.data(), v.data(), C.data(), F.data(), Jp.data(), mat.data());
+}
+
+int Sim::clear_staging() {
+  hx_.clear(); hv_.clear(); hC_.clear(); hF_.clear(); hJp_.clear(); hmat_.clear();
+  return MPM_OK;
+}
+
+int Sim::ensure_capacity(int64_t n) {
+  if (n <= cap_ && d_f_[0]) return MPM_OK;
+  const int64_t cap = std::max<int64_t>(64, ((n + 63) / 64) * 64);
+  for (int b = 0; b < 2; ++b) {
+    if (d_f_[b]) (void)hipFree(d_f_[b]);
+    if (d_mat_[b]) (void)hipFree(d_mat_[b]);
+    if (d_id_[b]) (void)hipFree(d_id_[b]);
+    if (d_keys_[b]) (void)hipFree(d_keys_[b]);
+    if (d_vals_[b]) (void)hipFree(d_vals_[b]);
+    d_f_[b] = nullptr; d_mat_[b] = nullptr; d_id_[b] = nullptr; d_keys_[b] = nullptr; d_vals_[b] = nullptr;
+  }
+  cap_ = 0;
+  for (int b = 0; b < 2; ++b) {
+    HIPCHK(hipMalloc(&d_f_[b], sizeof(float) * NUM_PLANES * cap), "hipMalloc(particles)");
+    HIPCHK(hipMalloc(&d_mat_[b], cap), "hipMalloc(material ids)");
+    HIPCHK(hipMalloc(&d_id_[b], sizeof(uint32_t) * cap), "hipMalloc(particle ids)");
+    HIPCHK(hipMalloc(&d_keys_[b], sizeof(uint32_t) * cap), "hipMalloc(sort keys)");
+    HIPCHK(hipMalloc(&d_vals_[b], sizeof(uint32_t) * cap), "hipMalloc(sort values)");
+  }
+  const size_t words = mpmk::radix_scratch_words(cap, 8);
+  if (words > hist_cap_) {
+    if (d_hist_) (void)hipFree(d_hist_);
+    d_hist_ = nullptr;
+    HIPCHK(hipMalloc(&d_hist_, words * sizeof(uint32_t)), "hipMalloc(sort scratch)");
+    hist_cap_ = words;
+  }
+  cap_ = cap;
+  stride_ = (size_t)cap;
+  return MPM_OK;
+}
+
+ParticlesDev Sim::particles() const {
+  ParticlesDev P;
+  P.f = d_f_[cur_];
+  P.mat = d_mat_[cur_];
+  P.id = d_id_[cur_];
+  P.stride = stride_;
+  P.n = (int32_t)P_;
+  return P;
+}
+
+int Sim::upload(hipStream_t s) {
+  const int64_t n = (int64_t)hJp_.size();
+  for (int64_t i = 0; i < n; ++i)
+    if (hmat_[i] >= mats_.size())
+      return fail(MPM_ERR_INVALID, "mpm_upload: particle %lld uses material %d but only %zu are set",
+                  (long long)i, (int)hmat_[i], mats_.size());
+  int rc = ensure_capacity(n);
+  if (rc) return rc;
+  rc = upload_materials();
+  if (rc) return rc;
+  // transpose the staging set into planes
+  std::vector<float> planes((size_t)NUM_PLANES * stride_, 0.0f);
+  for (int64_t i = 0; i < n; ++i) {
+    for (int a = 0; a < 3; ++a) {
+      planes[(PL_X + a) * stride_ + i] = hx_[3 * i + a];
+      planes[(PL_V + a) * stride_ + i] = hv_[3 * i + a];
+    }
+    for (int a = 0; a < 9; ++a) {
+      planes[(PL_C + a) * stride_ + i] = hC_[9 * i + a];
+      planes[(PL_F + a) * stride_ + i] = hF_[9 * i + a];
+    }
+    planes[PL_JP * stride_ + i] = hJp_[i];
+  }
+  std::vector<uint32_t> ids(n);
+  std::iota(ids.begin(), ids.end(), 0u);
+  max_mat_ = 0;
+  for (int64_t i = 0; i < n; ++i) max_mat_ = std::max<uint8_t>(max_mat_, hmat_[i]);
+  cur_ = 0;
+  HIPCHK(hipMemcpyAsync(d_f_[0], planes.data(), sizeof(float) * planes.size(), hipMemcpyHostToDevice, s), "upload particles");
+  if (n > 0) {
+    HIPCHK(hipMemcpyAsync(d_mat_[0], hmat_.data(), n, hipMemcpyHostToDevice, s), "upload materials ids");
+    HIPCHK(hipMemcpyAsync(d_id_[0], ids.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s), "upload ids");
+  }
+  HIPCHK(hipStreamSynchronize(s), "upload sync");
+  P_ = n;
+  uploaded_ = true;
+  primed_ = false;
+  have_vel_ = false;
+  steps_since_sort_ = 1 << 30;
+  t_ = 0.0;
+  return MPM_OK;
+}
+
+#define PHASE(ph, expr)                 \
+  do {                                  \
+    int rc_ = phase_begin(ph, s);       \
+    if (rc_) return rc_;                \
+    expr;                               \
+    rc_ = phase_end(ph, s);             \
+    if (rc_) return rc_;                \
+  } while (0)
+
+int Sim::sort_particles(hipStream_t s) {
+  if (P_ <= 1) return MPM_OK;
+  const ParticlesDev src = particles();
+  int out = 0;
+  ParticlesDev dst = src;
+  dst.f = d_f_[cur_ ^ 1];
+  dst.mat = d_mat_[cur_ ^ 1];
+  dst.id = d_id_[cur_ ^ 1];
+  PHASE(PH_SORT, {
+    HIPCHK(mpmk::cell_keys(src, k_, d_keys_[0], d_vals_[0], s), "cell keys");
+    HIPCHK(mpmk::radix_sort(d_keys_, d_vals_, P_, mpmk::cell_key_bits(k_.N), (int)opt_.radix_bits,
+                            d_hist_, &out, s),
+           "radix sort");
+    HIPCHK(mpmk::reorder(src, dst, d_vals_[out], s), "reorder");
+  });
+  cur_ ^= 1;
+  steps_since_sort_ = 0;
+  return MPM_OK;
+}
+
+int Sim::substep(hipStream_t s) {
+  float4* gA = d_grid_[gcur_];
+  float4* gB = d_grid_[gcur_ ^ 1];
+  const bool tiled = opt_.p2g_mode == 1;
+  if (tiled && opt_.sort_every > 0 && steps_since_sort_ >= opt_.sort_every) {
+    int rc = sort_particles(s);
+    if (rc) return rc;
+  }
+  const ParticlesDev P = particles();
+  const int nm = (int)mats_.size();
+  const int chunk = (int)opt_.chunk, seg = (int)opt_.seg_reduce;
+  if (!opt_.fused) {
+    // reference order: clear, P2G, grid update, G2P
+    PHASE(PH_CLEAR, HIPCHK(mpmk::fill_zero(gA, grid_nodes_, s), "clear grid"));
+    if (tiled) PHASE(PH_P2G, HIPCHK(mpmk::g2p2g_tiled(P, d_mats_, nm, k_, nullptr, gA, 0, chunk, seg, s), "p2g"));
+    else PHASE(PH_P2G, HIPCHK(mpmk::p2g_direct(P, d_mats_, k_, gA, s), "p2g"));
+    PHASE(PH_GRID_UPDATE, HIPCHK(mpmk::grid_update(gA, nullptr, k_, s), "grid update"));
+    PHASE(PH_G2P, HIPCHK(mpmk::g2p(P, d_mats_, k_, gA, s), "g2p"));
+    gcur_ ^= 1;  // gA now holds the velocities: keep it as "last update"
+    have_vel_ = true;
+    primed_ = false;
+    ++steps_since_sort_;
+    return MPM_OK;
+  }
+  if (!primed_) {
+    PHASE(PH_CLEAR, HIPCHK(mpmk::fill_zero(gA, grid_nodes_, s), "clear grid"));
+    if (tiled) PHASE(PH_P2G, HIPCHK(mpmk::g2p2g_tiled(P, d_mats_, nm, k_, nullptr, gA, 0, chunk, seg, s), "p2g"));
+    else PHASE(PH_P2G, HIPCHK(mpmk::p2g_direct(P, d_mats_, k_, gA, s), "p2g"));
+    primed_ = true;
+  }
+  // gA: P2G of this substep -> velocities; gB cleared in the same pass
+  PHASE(PH_GRID_UPDATE, HIPCHK(mpmk::grid_update(gA, gB, k_, s), "grid update"));
+  if (tiled) PHASE(PH_G2P2G, HIPCHK(mpmk::g2p2g_tiled(P, d_mats_, nm, k_, gA, gB, 1, chunk, seg, s), "g2p2g"));
+  else PHASE(PH_G2P2G, HIPCHK(mpmk::g2p2g_direct(P, d_mats_, k_, gA, gB, s), "g2p2g"));
+  gcur_ ^= 1;  // gB (primed P2G of the next substep) becomes current
+  have_vel_ = true;
+  ++steps_since_sort_;
+  return MPM_OK;
+}
+
+int Sim::check_materials_cover_particles(const char* who) {
+  if (P_ > 0 && (int)max_mat_ >= (int)mats_.size())
+    return fail(MPM_ERR_STATE, "%s: particles use material %d but only %zu materials are set", who,
+                (int)max_mat_, mats_.size());
+  return MPM_OK;
+}
+
+int Sim::advance(int32_t substeps, hipStream_t s) {
+  if (!uploaded_) return fail(MPM_ERR_STATE, "mpm_advance: call mpm_upload first");
+  if (substeps < 0) return fail(MPM_ERR_INVALID, "mpm_advance: negative substep count");
+  int rc0 = check_materials_cover_particles("mpm_advance");
+  if (rc0) return rc0;
+  int rc = upload_materials();
+  if (rc) return rc;
+  for (int32_t i = 0; i < substeps; ++i) {
+    rc = substep(s);
+    if (rc) return rc;
+    t_ += (double)prm_.dt;  // mpm.cu:328
+  }
+  return MPM_OK;
+}
+
+int Sim::fetch_state(std::vector<float>& planes, std::vector<uint8_t>& mat,
+                     std::vector<uint32_t>& id, hipStream_t s) {
+  planes.resize((size_t)NUM_PLANES * stride_);
+  mat.resize(P_);
+  id.resize(P_);
+  if (P_ == 0) return MPM_OK;
+  HIPCHK(hipMemcpyAsync(planes.data(), d_f_[cur_], sizeof(float) * planes.size(), hipMemcpyDeviceToHost, s), "download particles");
+  HIPCHK(hipMemcpyAsync(mat.data(), d_mat_[cur_], P_, hipMemcpyDeviceToHost, s), "download material ids");
+  HIPCHK(hipMemcpyAsync(id.data(), d_id_[cur_], sizeof(uint32_t) * P_, hipMemcpyDeviceToHost, s), "download ids");
+  HIPCHK(hipStreamSynchronize(s), "download sync");
+  return MPM_OK;
+}
+
+int Sim::download_soa(float* x, float* v, float* C, float* F, float* Jp, uint8_t* mat,
+                      hipStream_t s) {
+  if (!uploaded_) return fail(MPM_ERR_STATE, "mpm_download_soa: nothing uploaded");
+  std::vector<float> pl;
+  std::vector<uint8_t> m;
+  std::vector<uint32_t> id;
+  int rc = fetch_state(pl, m, id, s);
+  if (rc) return rc;
+  for (int64_t i = 0; i < P_; ++i) {
+    const size_t o = id[i];
+    if ((int64_t)o >= P_) return fail(MPM_ERR_STATE, "mpm_download_soa: corrupt particle id");
+    for (int a = 0; a < 3; ++a) {
+      if (x) x[3 * o + a] = pl[(PL_X + a) * stride_ + i];
+      if (v) v[3 * o + a] = pl[(PL_V + a) * stride_ + i];
+    }
+    for (int a = 0; a < 9; ++a) {
+      if (C) C[9 * o + a] = pl[(PL_C + a) * stride_ + i];
+      if (F) F[9 * o + a] = pl[(PL_F + a) * stride_ + i];
+    }
+    if (Jp) Jp[o] = pl[PL_JP * stride_ + i];
+    if (mat) mat[o] = m[i];
+  }
+  return MPM_OK;
+}
+
+int Sim::download_aos104(void* out, hipStream_t s) {
+  if (!out && P_ > 0) return fail(MPM_ERR_INVALID, "mpm_download_aos104: null output");
+  std::vector<float> x(3 * P_), v(3 * P_), C(9 * P_), F(9 * P_), Jp(P_);
+  std::vector<uint8_t> mat(P_);
+  int rc = download_soa(x.data(), v.data(), C.data(), F.data(), Jp.data(), mat.data(), s);
+  if (rc) return rc;
+  uint8_t* b = (uint8_t*)out;
+  for (int64_t i = 0; i < P_; ++i) {
+    uint8_t* r = b + 104 * i;
+    std::memset(r, 0, 104);
+    r[0] = mat[i];
+    std::memcpy(r + 4, &x[3 * i], 12);
+    std::memcpy(r + 16, &v[3 * i], 12);
+    float fc[9], cc[9];
+    for (int row = 0; row < 3; ++row)
+      for (int col = 0; col < 3; ++col) {
+        fc[3 * col + row] = F[9 * i + 3 * row + col];
+        cc[3 * col + row] = C[9 * i + 3 * row + col];
+      }
+    std::memcpy(r + 28, fc, 36);
+    std::memcpy(r + 64, cc, 36);
+    std::memcpy(r + 100, &Jp[i], 4);
+  }
+  return MPM_OK;
+}
+
+int Sim::download_grid(float* grid, hipStream_t s) {
+  if (!grid) return fail(MPM_ERR_INVALID, "mpm_download_grid: null output");
+  if (!have_vel_) return fail(MPM_ERR_STATE, "mpm_download_grid: no grid update has run yet");
+  // after a fused or unfused substep the velocities of the last update are in
+  // the grid that is not current
+  HIPCHK(hipMemcpyAsync(grid, d_grid_[gcur_ ^ 1], grid_nodes_ * sizeof(float4), hipMemcpyDeviceToHost, s), "download grid");
+  HIPCHK(hipStreamSynchronize(s), "download grid sync");
+  return MPM_OK;
+}
+
+int Sim::debug_p2g(float* grid, hipStream_t s) {
+  if (!uploaded_) return fail(MPM_ERR_STATE, "mpm_debug_p2g: nothing uploaded");
+  if (!grid) return fail(MPM_ERR_INVALID, "mpm_debug_p2g: null output");
+  int rc = check_materials_cover_particles("mpm_debug_p2g");
+  if (rc) return rc;
+  rc = upload_materials();
+  if (rc) return rc;
+  // scatter into the non-current grid so a primed grid survives
+  float4* g = d_grid_[gcur_ ^ 1];
+  HIPCHK(mpmk::fill_zero(g, grid_nodes_, s), "clear grid");
+  if (opt_.p2g_mode == 1) {
+    rc = sort_particles(s);
+    if (rc) return rc;
+    HIPCHK(mpmk::g2p2g_tiled(particles(), d_mats_, (int)mats_.size(), k_, nullptr, g, 0, (int)opt_.chunk, (int)opt_.seg_reduce, s), "p2g");
+  } else {
+    HIPCHK(mpmk::p2g_direct(particles(), d_mats_, k_, g, s), "p2g");
+  }
+  HIPCHK(hipMemcpyAsync(grid, g, grid_nodes_ * sizeof(float4), hipMemcpyDeviceToHost, s), "download grid");
+  HIPCHK(hipStreamSynchronize(s), "p2g sync");
+  have_vel_ = false;
+  return MPM_OK;
+}
+
+int Sim::debug_sort(uint32_t* keys, uint32_t* perm, hipStream_t s) {
+  if (!uploaded_) return fail(MPM_ERR_STATE, "mpm_debug_sort: nothing uploaded");
+  if (P_ == 0) return MPM_OK;
+  if (!keys || !perm) return fail(MPM_ERR_INVALID, "mpm_debug_sort: null output");
+  const ParticlesDev P = particles();
+  HIPCHK(mpmk::cell_keys(P, k_, d_keys_[0], d_vals_[0], s), "cell keys");
+  int out = 0;
+  HIPCHK(mpmk::radix_sort(d_keys_, d_vals_, P_, mpmk::cell_key_bits(k_.N), (int)opt_.radix_bits, d_hist_, &out, s), "radix sort");
+  std::vector<uint32_t> vals(P_), ids(P_);
+  HIPCHK(hipMemcpyAsync(keys, d_keys_[out], sizeof(uint32_t) * P_, hipMemcpyDeviceToHost, s), "download keys");
+  HIPCHK(hipMemcpyAsync(vals.data(), d_vals_[out], sizeof(uint32_t) * P_, hipMemcpyDeviceToHost, s), "download perm");
+  HIPCHK(hipMemcpyAsync(ids.data(), d_id_[cur_], sizeof(uint32_t) * P_, hipMemcpyDeviceToHost, s), "download ids");
+  HIPCHK(hipStreamSynchronize(s), "sort sync");
+  for (int64_t i = 0; i < P_; ++i) perm[i] = ids[vals[i]];
+  return MPM_OK;
+}
+
+// ---- checkpoint: magic, version, params, materials, count, time, SoA arrays
+namespace {
+const char kMagic[8] = {'M', 'P', 'M', 'S', 'T', 'A', 'T', '1'};
+}
+
+int Sim::save_state(const char* path) {
+  if (!path) return fail(MPM_ERR_INVALID, "mpm_save_state: null path");
+  if (!uploaded_) return fail(MPM_ERR_STATE, "mpm_save_state: nothing uploaded");
+  std::vector<float> x(3 * P_), v(3 * P_), C(9 * P_), F(9 * P_), Jp(P_);
+  std::vector<uint8_t> mat(P_);
+  int rc = download_soa(x.data(), v.data(), C.data(), F.data(), Jp.data(), mat.data(), nullptr);
+  if (rc) return rc;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(MPM_ERR_IO, "mpm_save_state: cannot open %s", path);
+  const int32_t ver = 1, nm = (int32_t)mats_.size();
+  const int64_t n = P_;
+  bool ok = std::fwrite(kMagic, 8, 1, f) == 1 && std::fwrite(&ver, 4, 1, f) == 1 &&
+            std::fwrite(&prm_, sizeof(prm_), 1, f) == 1 && std::fwrite(&nm, 4, 1, f) == 1 &&
+            (nm == 0 || std::fwrite(mats_.data(), sizeof(mpm_material), nm, f) == (size_t)nm) &&
+            std::fwrite(&n, 8, 1, f) == 1 && std::fwrite(&t_, 8, 1, f) == 1;
+  auto wr = [&](const void* p, size_t bytes) { if (ok && bytes) ok = std::fwrite(p, bytes, 1, f) == 1; };
+  wr(x.data(), 4 * x.size()); wr(v.data(), 4 * v.size()); wr(C.data(), 4 * C.size());
+  wr(F.data(), 4 * F.size()); wr(Jp.data(), 4 * Jp.size()); wr(mat.data(), mat.size());
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) return fail(MPM_ERR_IO, "mpm_save_state: write to %s failed", path);
+  return MPM_OK;
+}
+
+int Sim::load_state(const char* path) {
+  if (!path) return fail(MPM_ERR_INVALID, "mpm_load_state: null path");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(MPM_ERR_IO, "mpm_load_state: cannot open %s", path);
+  char magic[8];
+  int32_t ver = 0, nm = 0;
+  mpm_params p{};
+  int64_t n = 0;
+  double t = 0;
+  bool ok = std::fread(magic, 8, 1, f) == 1 && std::memcmp(magic, kMagic, 8) == 0 &&
+            std::fread(&ver, 4, 1, f) == 1 && ver == 1 && std::fread(&p, sizeof(p), 1, f) == 1 &&
+            std::fread(&nm, 4, 1, f) == 1 && nm >= 0 && nm <= MPM_MAX_MATERIALS;
+  std::vector<mpm_material> mats(ok ? nm : 0);
+  if (ok && nm) ok = std::fread(mats.data(), sizeof(mpm_material), nm, f) == (size_t)nm;
+  ok = ok && std::fread(&n, 8, 1, f) == 1 && n >= 0 && n < (int64_t)0x7fffff00 && std::fread(&t, 8, 1, f) == 1;
+  std::vector<float> x, v, C, F, Jp;
+  std::vector<uint8_t> mat;
+  if (ok) {
+    x.resize(3 * n); v.resize(3 * n); C.resize(9 * n); F.resize(9 * n); Jp.resize(n); mat.resize(n);
+  }
+  auto rd = [&](void* q, size_t bytes) { if (ok && bytes) ok = std::fread(q, bytes, 1, f) == 1; };
+  rd(x.data(), 4 * x.size()); rd(v.data(), 4 * v.size()); rd(C.data(), 4 * C.size());
+  rd(F.data(), 4 * F.size()); rd(Jp.data(), 4 * Jp.size()); rd(mat.data(), mat.size());
+  std::fclose(f);
+  if (!ok) return fail(MPM_ERR_IO, "mpm_load_state: %s is not a valid state file", path);
+  std::string why;
+  if (!validate_params(p, &why)) return fail(MPM_ERR_IO, "mpm_load_state: bad parameters: %s", why.c_str());
+  prm_ = p;
+  k_ = make_consts(p);
+  int rc = ensure_grids();
+  if (rc) return rc;
+  rc = set_materials(mats.data(), nm);
+  if (rc) return rc;
+  clear_staging();
+  rc = add_particles_soa(n, x.data(), v.data(), C.data(), F.data(), Jp.data(), mat.data());
+  if (rc) return rc;
+  rc = upload(nullptr);
+  if (rc) return rc;
+  t_ = t;
+  return MPM_OK;
+}
+
+int Sim::set_option(const char* name, int64_t v) {
+  if (!name) return fail(MPM_ERR_INVALID, "mpm_set_option: null name");
+  const std::string n = name;
+  if (n == "fused") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "fused must be 0 or 1"); opt_.fused = v; primed_ = false; return MPM_OK; }
+  if (n == "p2g_mode") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "p2g_mode must be 0 or 1"); opt_.p2g_mode = v; return MPM_OK; }
+  if (n == "sort_every") { if (v < 0) return fail(MPM_ERR_INVALID, "sort_every must be >= 0"); opt_.sort_every = v; return MPM_OK; }
+  if (n == "radix_bits") { if (v != 4 && v != 6 && v != 8) return fail(MPM_ERR_INVALID, "radix_bits must be 4, 6 or 8"); opt_.radix_bits = v; return MPM_OK; }
+  if (n == "chunk") { if (v != 256 && v != 512 && v != 1024) return fail(MPM_ERR_INVALID, "chunk must be 256, 512 or 1024"); opt_.chunk = v; return MPM_OK; }
+  if (n == "seg_reduce") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "seg_reduce must be 0 or 1"); opt_.seg_reduce = v; return MPM_OK; }
+  if (n == "timing") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "timing must be 0 or 1"); opt_.timing = v; return MPM_OK; }
+  return fail(MPM_ERR_INVALID, "mpm_set_option: unknown option '%s'", name);
+}
+
+int Sim::get_option(const char* name, int64_t* v) {
+  if (!name || !v) return fail(MPM_ERR_INVALID, "mpm_get_option: null argument");
+  const std::string n = name;
+  if (n == "fused") *v = opt_.fused;
+  else if (n == "p2g_mode") *v = opt_.p2g_mode;
+  else if (n == "sort_every") *v = opt_.sort_every;
+  else if (n == "radix_bits") *v = opt_.radix_bits;
+  else if (n == "chunk") *v = opt_.chunk;
+  else if (n == "seg_reduce") *v = opt_.seg_reduce;
+  else if (n == "timing") *v = opt_.timing;
+  else return fail(MPM_ERR_INVALID, "mpm_get_option: unknown option '%s'", name);
+  return MPM_OK;
+}
+
+}  // namespace mpm_engine

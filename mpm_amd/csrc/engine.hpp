@@ -1,0 +1,134 @@
+// engine.hpp -- host-side state of one simulation handle (the object behind
+// mpm_handle). Stands in for the reference's class Simulation
+// (include/mpm.cuh:44-106): it owns the device particle state, the two grids,
+// the material table and the sort scratch, and enqueues the substep.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mpm_hip.h"
+#include "engine_types.hpp"
+
+namespace mpm_engine {
+
+bool validate_params(const mpm_params& p, std::string* why);
+StepConsts make_consts(const mpm_params& p);
+MaterialDev make_material_dev(const mpm_material& m, const StepConsts& k);
+
+// Tunables (mpm_set_option names in quotes).
+struct Options {
+  int64_t fused = 1;        // "fused": G2P of step n fused with P2G of step n+1
+  int64_t p2g_mode = 1;     // "p2g_mode": 0 direct atomics, 1 sorted + LDS tile
+  int64_t sort_every = 8;   // "sort_every": re-sort particles every k substeps (0: never)
+  int64_t radix_bits = 8;   // "radix_bits": digit width of the radix sort (4, 6 or 8)
+  int64_t chunk = 512;      // "chunk": particles per workgroup in the tiled P2G
+  int64_t seg_reduce = 1;   // "seg_reduce": wavefront segmented reduction before LDS adds
+  int64_t timing = 0;       // "timing": bracket every kernel with HIP events (per-phase time)
+};
+
+// Phases whose kernels the timing option brackets with HIP events.
+enum Phase : int { PH_GRID_UPDATE = 0, PH_G2P2G, PH_P2G, PH_G2P, PH_SORT, PH_CLEAR, NUM_PHASES };
+const char* phase_name(int p);
+
+class Sim {
+ public:
+  explicit Sim(const mpm_params& p);
+  ~Sim();
+
+  std::mutex& mutex() { return mu_; }
+  const std::string& last_error() const { return err_; }
+  int fail(int code, const char* fmt, ...);
+
+  int init_device();
+  int set_materials(const mpm_material* m, int32_t n);
+  int add_particles_soa(int64_t n, const float* x, const float* v, const float* C, const float* F,
+                        const float* Jp, const uint8_t* mat);
+  int add_particles_aos104(const void* p, int64_t n);
+  int clear_staging();
+  int upload(hipStream_t s);
+  int64_t count() const { return P_; }
+  double time() const { return t_; }
+  int advance(int32_t substeps, hipStream_t s);
+  int download_soa(float* x, float* v, float* C, float* F, float* Jp, uint8_t* mat, hipStream_t s);
+  int download_aos104(void* out, hipStream_t s);
+  int download_grid(float* grid, hipStream_t s);
+  int save_state(const char* path);
+  int load_state(const char* path);
+  int set_option(const char* name, int64_t v);
+  int get_option(const char* name, int64_t* v);
+  int debug_p2g(float* grid, hipStream_t s);
+  int debug_sort(uint32_t* keys, uint32_t* perm, hipStream_t s);
+  int timing_read(const char* phase, double* total_ms, int64_t* launches);
+  int timing_reset();
+
+  // device views for other entry points of the library
+  ParticlesDev particles() const;
+  const StepConsts& consts() const { return k_; }
+  const MaterialDev* device_materials() const { return d_mats_; }
+
+ private:
+  int ensure_capacity(int64_t n);
+  int ensure_grids();
+  int upload_materials();
+  int sort_particles(hipStream_t s);
+  int substep(hipStream_t s);
+  int fetch_state(std::vector<float>& planes, std::vector<uint8_t>& mat,
+                  std::vector<uint32_t>& id, hipStream_t s);
+  int hip_fail(hipError_t e, const char* what);
+  int check_materials_cover_particles(const char* who);
+  // timing: events recorded around a phase when opt_.timing is set
+  int phase_begin(int ph, hipStream_t s);
+  int phase_end(int ph, hipStream_t s);
+
+  std::mutex mu_;
+  std::string err_;
+  int device_ = 0;
+  mpm_params prm_;
+  StepConsts k_;
+  Options opt_;
+  std::vector<mpm_material> mats_;
+  MaterialDev* d_mats_ = nullptr;
+  bool mats_dirty_ = true;
+
+  // host staging (SoA, upload order)
+  std::vector<float> hx_, hv_, hC_, hF_, hJp_;
+  std::vector<uint8_t> hmat_;
+
+  // device particles: two buffers (reorder ping-pong)
+  int64_t P_ = 0, cap_ = 0;
+  size_t stride_ = 0;
+  float* d_f_[2] = {nullptr, nullptr};
+  uint8_t* d_mat_[2] = {nullptr, nullptr};
+  uint32_t* d_id_[2] = {nullptr, nullptr};
+  int cur_ = 0;
+  bool uploaded_ = false;
+  uint8_t max_mat_ = 0;  // largest material index among uploaded particles
+
+  // grids: g_[0], g_[1]; `gcur_` holds the primed P2G of the next substep
+  float4* d_grid_[2] = {nullptr, nullptr};
+  size_t grid_nodes_ = 0;
+  int gcur_ = 0;
+  bool primed_ = false;
+  bool have_vel_ = false;  // d_grid_[gcur_ ^ 1] holds velocities of the last update
+
+  // sort scratch
+  uint32_t* d_keys_[2] = {nullptr, nullptr};
+  uint32_t* d_vals_[2] = {nullptr, nullptr};
+  uint32_t* d_hist_ = nullptr;
+  size_t hist_cap_ = 0;
+  int64_t steps_since_sort_ = 1 << 30;
+
+  double t_ = 0.0;
+
+  struct EventPair { int phase; hipEvent_t a, b; };
+  std::vector<EventPair> ev_;      // recorded, not yet read
+  std::vector<hipEvent_t> ev_pool_;
+  double phase_ms_[NUM_PHASES] = {0};
+  int64_t phase_n_[NUM_PHASES] = {0};
+  hipEvent_t open_[NUM_PHASES] = {nullptr};
+};
+
+}  // namespace mpm_engine

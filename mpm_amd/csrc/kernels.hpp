@@ -1,0 +1,39 @@
+// kernels.hpp -- host-side launchers of the HIP kernels (defined in the .hip
+// files of this directory). All launches are asynchronous on the given stream
+// and return the launch status.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "engine_types.hpp"
+
+namespace mpmk {
+
+hipError_t fill_zero(float4* g, size_t n, hipStream_t s);
+hipError_t p2g_direct(const ParticlesDev& P, const MaterialDev* mats, const StepConsts& k,
+                      float4* grid, hipStream_t s);
+hipError_t grid_update(float4* g, float4* clear, const StepConsts& k, hipStream_t s);
+hipError_t g2p(const ParticlesDev& P, const MaterialDev* mats, const StepConsts& k,
+               const float4* grid, hipStream_t s);
+hipError_t g2p2g_direct(const ParticlesDev& P, const MaterialDev* mats, const StepConsts& k,
+                        const float4* grid_in, float4* grid_out, hipStream_t s);
+hipError_t svd_batch(const float* A, float* U, float* S, float* V, int64_t n, hipStream_t s);
+hipError_t probe_kernel_image(int* nregs);
+
+// sort.hip
+int cell_key_bits(int N);
+size_t radix_scratch_words(int64_t n, int radix_bits);
+hipError_t cell_keys(const ParticlesDev& P, const StepConsts& k, uint32_t* keys, uint32_t* vals,
+                     hipStream_t s);
+// Stable LSD radix sort of (keys[0], vals[0]); the result lands in
+// keys[*out_idx], vals[*out_idx]. scratch holds radix_scratch_words() words.
+hipError_t radix_sort(uint32_t* keys[2], uint32_t* vals[2], int64_t n, int key_bits,
+                      int radix_bits, uint32_t* scratch, int* out_idx, hipStream_t s);
+hipError_t reorder(const ParticlesDev& src, const ParticlesDev& dst, const uint32_t* perm,
+                   hipStream_t s);
+
+// p2g_tiled.hip: fused G2P + tiled P2G over cell-sorted particles
+hipError_t g2p2g_tiled(const ParticlesDev& P, const MaterialDev* mats, int nmats,
+                       const StepConsts& k, const float4* grid_in, float4* grid_out, int do_g2p,
+                       int chunk, int seg_reduce, hipStream_t s);
+
+}  // namespace mpmk

@@ -1,0 +1,257 @@
+// sort.hip -- cell-key sort of particles for gfx950.
+//
+// The reference processes particles in their storage order (mpm.cu:17,112).
+// The engine keeps them ordered by grid cell so that a workgroup's particles
+// cover a compact block of the grid (P2G LDS tile) and neighbouring lanes
+// share cells (segmented reduction). The order is an implementation detail:
+// downloads restore the reference's particle order.
+//
+// Key: blocked Morton order. Cells are grouped in 4x4x4 blocks; the key is
+// morton(block) << 6 | (cell within block). Particles outside the grid are
+// clamped to the nearest cell.
+//
+// Sort: least-significant-digit radix sort of (key, index) pairs, stable, so
+// the permutation equals a CPU stable sort by key. Per digit pass:
+//   k_radix_hist     per-tile digit histogram (LDS), digit-major in HBM
+//   k_radix_scan     one workgroup per digit: exclusive scan over tiles
+//   k_radix_scatter  stable in-tile ranking (wavefront ballots for equal
+//                    digits, per-wave counts in LDS), then scatter
+#include <hip/hip_runtime.h>
+
+#include "device_math.hpp"
+#include "engine_types.hpp"
+#include "kernels.hpp"
+
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kTile = kSortThreads * kSortItems;
+constexpr int kWaves = kSortThreads / 64;
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  // interleave the low 10 bits of v with two zero bits between each
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t cell_key_of(const float x[3], const StepConsts& k) {
+  int c[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    int b = (int)(x[a] * k.dx_inv - 0.5f);
+    b = b < 0 ? 0 : (b >= k.N ? k.N - 1 : b);
+    c[a] = b;
+  }
+  const uint32_t m = (spread3((uint32_t)c[0] >> 2) << 2) | (spread3((uint32_t)c[1] >> 2) << 1) |
+                     spread3((uint32_t)c[2] >> 2);
+  const uint32_t local = (uint32_t)(((c[0] & 3) << 4) | ((c[1] & 3) << 2) | (c[2] & 3));
+  return (m << 6) | local;
+}
+
+__global__ void __launch_bounds__(256)
+    k_cell_keys(ParticlesDev P, StepConsts k, uint32_t* __restrict__ keys,
+                uint32_t* __restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  float x[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) x[a] = P.f[(PL_X + a) * P.stride + i];
+  keys[i] = cell_key_of(x, k);
+  vals[i] = (uint32_t)i;
+}
+
+template <int BITS>
+__global__ void __launch_bounds__(kSortThreads)
+    k_radix_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                 uint32_t* __restrict__ hist, int ntiles) {
+  constexpr int R = 1 << BITS;
+  __shared__ uint32_t cnt[R];
+  for (int d = threadIdx.x; d < R; d += kSortThreads) cnt[d] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll
+  for (int it = 0; it < kSortItems; ++it) {
+    const int64_t i = base + it * kSortThreads + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & (R - 1)], 1u);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < R; d += kSortThreads) hist[(size_t)d * ntiles + blockIdx.x] = cnt[d];
+}
+
+// One workgroup per digit: exclusive scan of hist[d][0..ntiles) in place;
+// total[d] receives the digit's count.
+__global__ void __launch_bounds__(256)
+    k_radix_scan(uint32_t* __restrict__ hist, int ntiles, uint32_t* __restrict__ total) {
+  __shared__ uint32_t part[256];
+  uint32_t* row = hist + (size_t)blockIdx.x * ntiles;
+  uint32_t carry = 0;
+  for (int c0 = 0; c0 < ntiles; c0 += 256) {
+    const int i = c0 + threadIdx.x;
+    const uint32_t v = (i < ntiles) ? row[i] : 0u;
+    part[threadIdx.x] = v;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 256 entries
+    for (int off = 1; off < 256; off <<= 1) {
+      const uint32_t add = (threadIdx.x >= (unsigned)off) ? part[threadIdx.x - off] : 0u;
+      __syncthreads();
+      part[threadIdx.x] += add;
+      __syncthreads();
+    }
+    if (i < ntiles) row[i] = carry + part[threadIdx.x] - v;
+    carry += part[255];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) total[blockIdx.x] = carry;
+}
+
+template <int BITS>
+__global__ void __launch_bounds__(kSortThreads)
+    k_radix_scatter(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+                    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n,
+                    int shift, const uint32_t* __restrict__ hist, int ntiles,
+                    const uint32_t* __restrict__ total) {
+  constexpr int R = 1 << BITS;
+  __shared__ uint32_t dbase[R];             // global start of each digit for this tile
+  __shared__ uint32_t wcnt[kWaves][R];      // per-wave digit counts -> offsets
+  __shared__ uint32_t run[R];               // digits already placed by earlier items
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  // digit bases: exclusive scan of the digit totals (R <= 256 entries), done
+  // redundantly by every workgroup
+  if (tid < R) run[tid] = total[tid];
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t acc = 0;
+    for (int d = 0; d < R; ++d) {
+      const uint32_t t = run[d];
+      run[d] = acc;
+      acc += t;
+    }
+  }
+  __syncthreads();
+  for (int d = tid; d < R; d += kSortThreads) {
+    dbase[d] = run[d] + hist[(size_t)d * ntiles + blockIdx.x];
+  }
+  __syncthreads();
+  for (int d = tid; d < R; d += kSortThreads) run[d] = 0;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  for (int it = 0; it < kSortItems; ++it) {
+    for (int d = tid; d < kWaves * R; d += kSortThreads) (&wcnt[0][0])[d] = 0;
+    __syncthreads();
+    const int64_t i = base + it * kSortThreads + tid;
+    const bool valid = i < n;
+    const uint32_t key = valid ? keys_in[i] : 0u;
+    const uint32_t val = valid ? vals_in[i] : 0u;
+    const uint32_t dig = (key >> shift) & (R - 1);
+    // lanes of this wavefront holding the same digit
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < BITS; ++b) {
+      const uint64_t m = __ballot((dig >> b) & 1u);
+      peers &= ((dig >> b) & 1u) ? m : ~m;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+    const bool leader = valid && (peers & lt_mask) == 0ull;
+    if (leader) wcnt[wave][dig] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    // per digit: exclusive offsets over waves, continuing the running count
+    for (int d = tid; d < R; d += kSortThreads) {
+      uint32_t r = run[d];
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) {
+        const uint32_t c = wcnt[w][d];
+        wcnt[w][d] = r;
+        r += c;
+      }
+      run[d] = r;
+    }
+    __syncthreads();
+    if (valid) {
+      const uint32_t pos = dbase[dig] + wcnt[wave][dig] + rank;
+      keys_out[pos] = key;
+      vals_out[pos] = val;
+    }
+    __syncthreads();
+  }
+}
+
+// dst[i] = src[perm[i]] for every plane, material and id
+__global__ void __launch_bounds__(256)
+    k_reorder(ParticlesDev src, ParticlesDev dst, const uint32_t* __restrict__ perm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= src.n) return;
+  const uint32_t j = perm[i];
+#pragma unroll
+  for (int p = 0; p < NUM_PLANES; ++p) dst.f[p * dst.stride + i] = src.f[p * src.stride + j];
+  dst.mat[i] = src.mat[j];
+  dst.id[i] = src.id[j];
+}
+
+template <int BITS>
+hipError_t radix_sort_t(uint32_t* keys[2], uint32_t* vals[2], int64_t n, int key_bits,
+                        uint32_t* scratch, int* out_idx, hipStream_t s) {
+  constexpr int R = 1 << BITS;
+  const int ntiles = (int)((n + kTile - 1) / kTile);
+  uint32_t* hist = scratch;
+  uint32_t* total = scratch + (size_t)R * ntiles;
+  int cur = 0;
+  for (int shift = 0; shift < key_bits; shift += BITS) {
+    hipLaunchKernelGGL(k_radix_hist<BITS>, dim3(ntiles), dim3(kSortThreads), 0, s, keys[cur], n,
+                       shift, hist, ntiles);
+    hipLaunchKernelGGL(k_radix_scan, dim3(R), dim3(256), 0, s, hist, ntiles, total);
+    hipLaunchKernelGGL(k_radix_scatter<BITS>, dim3(ntiles), dim3(kSortThreads), 0, s, keys[cur],
+                       vals[cur], keys[cur ^ 1], vals[cur ^ 1], n, shift, hist, ntiles, total);
+    cur ^= 1;
+  }
+  *out_idx = cur;
+  return hipGetLastError();
+}
+
+}  // namespace
+
+namespace mpmk {
+
+int cell_key_bits(int N) {
+  int nb = (N + 3) / 4, b = 0;
+  while ((1 << b) < nb) ++b;
+  return 3 * b + 6;
+}
+
+size_t radix_scratch_words(int64_t n, int radix_bits) {
+  const int64_t ntiles = (n + kTile - 1) / kTile;
+  return (size_t)(1u << radix_bits) * (size_t)(ntiles > 0 ? ntiles : 1) + 256;
+}
+
+hipError_t cell_keys(const ParticlesDev& P, const StepConsts& k, uint32_t* keys, uint32_t* vals,
+                     hipStream_t s) {
+  if (P.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cell_keys, dim3((P.n + 255) / 256), dim3(256), 0, s, P, k, keys, vals);
+  return hipGetLastError();
+}
+
+hipError_t radix_sort(uint32_t* keys[2], uint32_t* vals[2], int64_t n, int key_bits,
+                      int radix_bits, uint32_t* scratch, int* out_idx, hipStream_t s) {
+  *out_idx = 0;
+  if (n <= 1 || key_bits <= 0) return hipSuccess;
+  switch (radix_bits) {
+    case 4: return radix_sort_t<4>(keys, vals, n, key_bits, scratch, out_idx, s);
+    case 6: return radix_sort_t<6>(keys, vals, n, key_bits, scratch, out_idx, s);
+    case 8: return radix_sort_t<8>(keys, vals, n, key_bits, scratch, out_idx, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t reorder(const ParticlesDev& src, const ParticlesDev& dst, const uint32_t* perm,
+                   hipStream_t s) {
+  if (src.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_reorder, dim3((src.n + 255) / 256), dim3(256), 0, s, src, dst, perm);
+  return hipGetLastError();
+}
+
+}  // namespace mpmk

@@ -1,0 +1,236 @@
+"""Python handle on one engine instance (wraps the C-ABI of libmpm_hip.so).
+
+Mirrors the reference's class Simulation (include/mpm.cuh:44-106) for Python
+callers: materials and particles are staged, `upload()` stands in for
+`initCuda()`, `advance(k)` for k calls of `advance()`, `download()` for
+`syncDevice()` + `getFullParticleList()`.
+
+Particle arrays use the SoA layout of the C-ABI: x (n,3), v (n,3), C (n,3,3),
+F (n,3,3) row-major, Jp (n,), material (n,) uint8, all float32.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib, mpm_material, mpm_params
+
+
+def _fp(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _stream(stream) -> Optional[int]:
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream or None
+    # torch.cuda.Stream (HIP stream on ROCm)
+    return int(stream.cuda_stream) or None
+
+
+@dataclass
+class Params:
+    """Simulation parameters (SimulationParameters, include/TransferScheme.h:6-29,
+    plus the constants src/mpm.cu:6,92 hard-codes)."""
+    dt: float = 1e-4
+    N: int = 60
+    gravity: float = -9.81
+    boundary_mode: int = _lib.BOUNDARY_FRACTION
+    boundary: float = 0.05
+    compat: int = _lib.COMPAT_REFERENCE
+
+    def to_c(self) -> mpm_params:
+        return mpm_params(self.dt, self.N, self.gravity, self.boundary_mode, self.boundary,
+                          self.compat)
+
+
+def make_material(kind: int, volume: float, density: float, E: float = 1.4e5, nu: float = 0.2,
+                  hardening: float = 10.0, clamp_lo: float = 0.975, clamp_hi: float = 1.0075,
+                  bulk_modulus: float = 0.0) -> mpm_material:
+    """Build a material the way the reference's constructors do
+    (MaterialModel.cuh:27-29,50-54,75-82) through the library's own routine."""
+    m = mpm_material()
+    check(lib().mpm_material_init(C.byref(m), kind, volume, density, E, nu, hardening, clamp_lo,
+                                  clamp_hi, bulk_modulus))
+    return m
+
+
+def material_as_numpy(kind: int, volume: float, density: float, E: float = 1.4e5,
+                      nu: float = 0.2, hardening: float = 10.0, clamp_lo: float = 0.975,
+                      clamp_hi: float = 1.0075, bulk_modulus: float = 0.0) -> mpm_material:
+    """Same construction as make_material, written in numpy float32 (usable
+    without the HIP library; the tests check both agree bit for bit)."""
+    f = np.float32
+    E32, nu32, v32, d32 = f(E), f(nu), f(volume), f(density)
+    m = mpm_material()
+    m.kind = kind
+    m.particle_volume = float(v32)
+    m.particle_mass = float(d32 * v32)
+    m.mu0 = float(E32 / (f(2) * (f(1) + nu32)))
+    m.lambda0 = float(E32 * nu32 / ((f(1) + nu32) * (f(1) - f(2) * nu32)))
+    m.hardening = float(f(hardening))
+    m.clamp_lo = float(f(clamp_lo))
+    m.clamp_hi = float(f(clamp_hi))
+    m.bulk_modulus = float(f(bulk_modulus))
+    return m
+
+
+class Engine:
+    """One simulation on the current HIP device."""
+
+    def __init__(self, params: Params):
+        self.params = params
+        self._h = C.c_void_p()
+        self._cp = params.to_c()
+        check(lib().mpm_create(C.byref(self._cp), C.byref(self._h)), None, "mpm_create")
+        self._n = 0
+
+    # -- lifetime
+    def close(self) -> None:
+        if self._h:
+            lib().mpm_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def _chk(self, rc: int, what: str = "") -> None:
+        check(rc, self._h, what)
+
+    # -- configuration
+    def set_materials(self, mats: Sequence[mpm_material]) -> None:
+        arr = (mpm_material * len(mats))(*mats)
+        self._chk(lib().mpm_set_materials(self._h, arr, len(mats)))
+
+    def set_option(self, name: str, value: int) -> None:
+        self._chk(lib().mpm_set_option(self._h, name.encode(), int(value)))
+
+    def get_option(self, name: str) -> int:
+        v = C.c_int64()
+        self._chk(lib().mpm_get_option(self._h, name.encode(), C.byref(v)))
+        return v.value
+
+    # -- particles
+    def add_particles(self, x, material, v=None, C_=None, F=None, Jp=None) -> None:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+        n = x.shape[0]
+        mat = np.ascontiguousarray(material, dtype=np.uint8).reshape(n)
+        arrs = []
+        for a, shape in ((v, (n, 3)), (C_, (n, 3, 3)), (F, (n, 3, 3)), (Jp, (n,))):
+            arrs.append(None if a is None else np.ascontiguousarray(a, dtype=np.float32).reshape(shape))
+        self._chk(lib().mpm_add_particles_soa(self._h, n, _fp(x), _fp(arrs[0]), _fp(arrs[1]),
+                                              _fp(arrs[2]), _fp(arrs[3]),
+                                              mat.ctypes.data_as(C.POINTER(C.c_uint8))))
+
+    def add_particles_aos104(self, raw: np.ndarray) -> None:
+        raw = np.ascontiguousarray(raw, dtype=np.uint8)
+        assert raw.size % 104 == 0
+        self._chk(lib().mpm_add_particles_aos104(self._h, raw.ctypes.data, raw.size // 104))
+
+    def clear_particles(self) -> None:
+        self._chk(lib().mpm_clear_particles(self._h))
+
+    def upload(self, stream=None) -> None:
+        self._chk(lib().mpm_upload(self._h, _stream(stream)))
+        self._n = self.count()
+
+    def count(self) -> int:
+        v = C.c_int64()
+        self._chk(lib().mpm_count(self._h, C.byref(v)))
+        return v.value
+
+    # -- stepping
+    def advance(self, substeps: int = 1, stream=None) -> None:
+        self._chk(lib().mpm_advance(self._h, int(substeps), _stream(stream)))
+
+    def time(self) -> float:
+        v = C.c_double()
+        self._chk(lib().mpm_time(self._h, C.byref(v)))
+        return v.value
+
+    # -- per-kernel timing (option "timing" = 1)
+    def timing(self, phase: str):
+        """(total milliseconds, launches) recorded for one kernel phase."""
+        ms = C.c_double()
+        n = C.c_int64()
+        self._chk(lib().mpm_timing_read(self._h, phase.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def timing_reset(self) -> None:
+        self._chk(lib().mpm_timing_reset(self._h))
+
+    # -- results
+    def download(self, stream=None) -> dict:
+        n = self._n
+        out = {
+            "x": np.empty((n, 3), np.float32), "v": np.empty((n, 3), np.float32),
+            "C": np.empty((n, 3, 3), np.float32), "F": np.empty((n, 3, 3), np.float32),
+            "Jp": np.empty((n,), np.float32), "material": np.empty((n,), np.uint8),
+        }
+        self._chk(lib().mpm_download_soa(self._h, _fp(out["x"]), _fp(out["v"]), _fp(out["C"]),
+                                         _fp(out["F"]), _fp(out["Jp"]),
+                                         out["material"].ctypes.data_as(C.POINTER(C.c_uint8)),
+                                         _stream(stream)))
+        return out
+
+    def download_aos104(self, stream=None) -> np.ndarray:
+        raw = np.empty(self._n * 104, np.uint8)
+        self._chk(lib().mpm_download_aos104(self._h, raw.ctypes.data, _stream(stream)))
+        return raw
+
+    def download_grid(self, stream=None) -> np.ndarray:
+        N = self.params.N
+        g = np.empty((N, N, N, 4), np.float32)
+        self._chk(lib().mpm_download_grid(self._h, _fp(g), _stream(stream)))
+        return g
+
+    def debug_p2g(self, stream=None) -> np.ndarray:
+        N = self.params.N
+        g = np.empty((N, N, N, 4), np.float32)
+        self._chk(lib().mpm_debug_p2g(self._h, _fp(g), _stream(stream)))
+        return g
+
+    def debug_sort(self, stream=None):
+        keys = np.empty(self._n, np.uint32)
+        perm = np.empty(self._n, np.uint32)
+        self._chk(lib().mpm_debug_sort(self._h, keys.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                       perm.ctypes.data_as(C.POINTER(C.c_uint32)), _stream(stream)))
+        return keys, perm
+
+    def save_state(self, path: str) -> None:
+        self._chk(lib().mpm_save_state(self._h, str(path).encode()))
+
+    def load_state(self, path: str) -> None:
+        self._chk(lib().mpm_load_state(self._h, str(path).encode()))
+        self._n = self.count()
+
+
+def device_svd(A: np.ndarray):
+    """Batched device 3x3 SVD (the engine's routine). A: (n,3,3) float32."""
+    A = np.ascontiguousarray(A, dtype=np.float32).reshape(-1, 3, 3)
+    n = A.shape[0]
+    U = np.empty_like(A)
+    V = np.empty_like(A)
+    S = np.empty((n, 3), np.float32)
+    check(lib().mpm_debug_svd(n, _fp(A), _fp(U), _fp(S), _fp(V)))
+    return U, S, V
