@@ -41,6 +41,59 @@ __device__ __forceinline__ bool outside_domain(const int base[3], int N) {
          base[2] + 3 < 0 || base[2] >= N;
 }
 
+// ---------------------------------------------------------------------------
+// Morton order of grid blocks: 3-way bit interleave of 10-bit coordinates.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+__device__ __forceinline__ uint32_t compact3(uint32_t v) {
+  v &= 0x09249249u;
+  v = (v | (v >> 2)) & 0x030c30c3u;
+  v = (v | (v >> 4)) & 0x0300f00fu;
+  v = (v | (v >> 8)) & 0x030000ffu;
+  v = (v | (v >> 16)) & 0x000003ffu;
+  return v;
+}
+__device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  return (spread3(x) << 2) | (spread3(y) << 1) | spread3(z);
+}
+
+// ---------------------------------------------------------------------------
+// gridOpKernel for one node (reference src/mpm.cu:76-107): momentum / mass
+// (three sequential divides; mass becomes 1), gravity on y, then the walls.
+// Nodes without mass are left as they are.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float4 grid_node_update(float4 c, int xi, int yi, int zi,
+                                                   const StepConsts& k) {
+  if (!(c.w > 0.0f)) return c;
+  c.x = c.x / c.w;
+  c.y = c.y / c.w;
+  c.z = c.z / c.w;
+  c.w = 1.0f;
+  c.y += k.dt * k.gravity;
+  const int N = k.N;
+  bool sticky, floor_;
+  if (k.boundary_mode == MPM_BOUNDARY_CELLS) {
+    const int b = k.bcells;
+    sticky = xi < b || xi >= N - b || yi >= N - b || zi < b || zi >= N - b;
+    floor_ = yi < b;
+  } else {
+    const float b = k.bfrac, hi = 1.0f - k.bfrac, fn = (float)N;
+    const float xf = (float)xi / fn, yf = (float)yi / fn, zf = (float)zi / fn;
+    sticky = xf < b || xf > hi || yf > hi || zf < b || zf > hi;
+    floor_ = yf < b;
+  }
+  if (sticky) { c.x = 0.f; c.y = 0.f; c.z = 0.f; }
+  if (floor_) c.y = fmaxf(0.0f, c.y);
+  return c;
+}
+
 // 3x3 helpers on row-major float[9]
 __device__ __forceinline__ float det3(const float* m) {
   const float sub1 = m[3] * m[7] - m[4] * m[6];

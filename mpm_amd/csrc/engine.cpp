@@ -88,8 +88,10 @@ Sim::~Sim() {
   }
   if (d_hist_) (void)hipFree(d_hist_);
   if (d_mats_) (void)hipFree(d_mats_);
-  for (uint32_t* p : {d_block_start_, d_flag_, d_pos_, d_item_start_, d_item_block_, d_item_count_})
+  for (uint32_t* p : {d_block_start_, d_block_items_, d_flag_, d_pos_, d_dest_, d_item_start_,
+                      d_item_block_, d_item_waves_, d_item_count_, d_dirty_})
     if (p) (void)hipFree(p);
+  if (d_tiles_) (void)hipFree(d_tiles_);
   for (auto& p : ev_) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
   for (auto ev : ev_pool_) (void)hipEventDestroy(ev);
   for (auto ev : open_) if (ev) (void)hipEventDestroy(ev);
@@ -182,6 +184,30 @@ int Sim::ensure_grids() {
     HIPCHK(hipMemset(d_grid_[b], 0, nodes * sizeof(float4)), "hipMemset(grid)");
   }
   grid_nodes_ = nodes;
+  const size_t nbx = ((size_t)prm_.N + 3) / 4;
+  if (d_dirty_) (void)hipFree(d_dirty_);
+  d_dirty_ = nullptr;
+  HIPCHK(hipMalloc(&d_dirty_, nbx * nbx * nbx * sizeof(uint32_t)), "hipMalloc(dirty flags)");
+  HIPCHK(hipMemset(d_dirty_, 0, nbx * nbx * nbx * sizeof(uint32_t)), "hipMemset(dirty flags)");
+  dirty_cap_ = nbx * nbx * nbx;
+  side_clean_ = true;
+  vel_grid_ = nullptr;
+  primed_ = false;
+  // Morton space of the 4^3 home blocks
+  int bits = 0;
+  while ((1 << bits) < (int)nbx) ++bits;
+  nblk_ = (int64_t)1 << (3 * bits);
+  if ((size_t)nblk_ > block_start_cap_) {
+    for (uint32_t** p : {&d_block_start_, &d_block_items_}) {
+      if (*p) (void)hipFree(*p);
+      *p = nullptr;
+    }
+    HIPCHK(hipMalloc(&d_block_start_, 2 * sizeof(uint32_t) * nblk_), "hipMalloc(block ranges)");
+    HIPCHK(hipMalloc(&d_block_items_, 2 * sizeof(uint32_t) * nblk_), "hipMalloc(block items)");
+    HIPCHK(hipMemset(d_block_items_, 0, 2 * sizeof(uint32_t) * nblk_), "hipMemset(block items)");
+    block_start_cap_ = (size_t)nblk_;
+  }
+  items_valid_ = false;
   return MPM_OK;
 }
 
@@ -290,27 +316,17 @@ int Sim::ensure_capacity(int64_t n) {
     HIPCHK(hipMalloc(&d_keys_[b], sizeof(uint32_t) * cap), "hipMalloc(sort keys)");
     HIPCHK(hipMalloc(&d_vals_[b], sizeof(uint32_t) * cap), "hipMalloc(sort values)");
   }
-  for (uint32_t** p : {&d_flag_, &d_pos_, &d_item_start_, &d_item_block_}) {
+  for (uint32_t** p : {&d_flag_, &d_pos_, &d_dest_, &d_item_start_, &d_item_block_, &d_item_waves_}) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }
   HIPCHK(hipMalloc(&d_flag_, sizeof(uint32_t) * cap), "hipMalloc(item flags)");
   HIPCHK(hipMalloc(&d_pos_, sizeof(uint32_t) * cap), "hipMalloc(item positions)");
+  HIPCHK(hipMalloc(&d_dest_, sizeof(uint32_t) * cap), "hipMalloc(item arrangement)");
   HIPCHK(hipMalloc(&d_item_start_, sizeof(uint32_t) * (cap + 1)), "hipMalloc(item starts)");
   HIPCHK(hipMalloc(&d_item_block_, sizeof(uint32_t) * cap), "hipMalloc(item blocks)");
+  HIPCHK(hipMalloc(&d_item_waves_, sizeof(uint32_t) * cap), "hipMalloc(item waves)");
   if (!d_item_count_) HIPCHK(hipMalloc(&d_item_count_, sizeof(uint32_t)), "hipMalloc(item count)");
-  {
-    // Morton space of 4^3 blocks (the largest block count of the supported edges)
-    int bits = 0;
-    while ((1 << bits) < ((int)prm_.N + 3) / 4) ++bits;
-    const size_t nblk = (size_t)1 << (3 * bits);
-    if (nblk > block_start_cap_) {
-      if (d_block_start_) (void)hipFree(d_block_start_);
-      d_block_start_ = nullptr;
-      HIPCHK(hipMalloc(&d_block_start_, 2 * sizeof(uint32_t) * nblk), "hipMalloc(block ranges)");
-      block_start_cap_ = nblk;
-    }
-  }
   const size_t words = mpmk::radix_scratch_words(cap, 8);
   if (words > hist_cap_) {
     if (d_hist_) (void)hipFree(d_hist_);
@@ -370,9 +386,8 @@ int Sim::upload(hipStream_t s) {
   P_ = n;
   uploaded_ = true;
   items_valid_ = false;
-  lb_ = choose_block_log2();
   primed_ = false;
-  have_vel_ = false;
+  vel_grid_ = nullptr;
   steps_since_sort_ = 1 << 30;
   t_ = 0.0;
   return MPM_OK;
@@ -387,41 +402,29 @@ int Sim::upload(hipStream_t s) {
     if (rc_) return rc_;                \
   } while (0)
 
-// Block edge from the particle density of the staging set: 8^3-cell home
-// blocks when 4^3 blocks would hold fewer than 512 particles on average (so a
-// work item amortises its tile load and flush over enough particles), else 4^3.
-int Sim::choose_block_log2() const {
-  if (opt_.block == 4) return 2;
-  if (opt_.block == 8) return 3;
-  if (P_ == 0) return 2;
-  if ((int64_t)hx_.size() < 3 * P_) return lb_;  // staging set dropped since the upload
-  const float dxi = k_.dx_inv;
-  const int nb = (k_.N + 3) / 4;
-  std::vector<uint8_t> seen((size_t)nb * nb * nb, 0);
-  int64_t occupied = 0;
-  for (int64_t i = 0; i < P_; ++i) {
-    int c[3];
-    for (int a = 0; a < 3; ++a) {
-      int b = (int)(hx_[3 * i + a] * dxi - 0.5f);
-      b = b < 0 ? 0 : (b >= k_.N ? k_.N - 1 : b);
-      c[a] = b >> 2;
-    }
-    uint8_t& f = seen[((size_t)c[0] * nb + c[1]) * nb + c[2]];
-    if (!f) { f = 1; ++occupied; }
-  }
-  const double ppb = (double)P_ / (double)(occupied > 0 ? occupied : 1);
-  return ppb < 512.0 ? 3 : 2;
-}
-
 int Sim::sort_particles(hipStream_t s) {
-  if (P_ <= 0) return MPM_OK;
+  chunk_ = opt_.chunk > 0 ? (int)opt_.chunk : mpmk::kMaxItemParticles;
+  const int64_t items = std::max<int64_t>(1, max_items());
+  if (items > tiles_cap_) {
+    if (d_tiles_) (void)hipFree(d_tiles_);
+    d_tiles_ = nullptr;
+    tiles_cap_ = 0;
+    HIPCHK(hipMalloc(&d_tiles_, sizeof(float4) * mpmk::kTileNodes * (size_t)items), "hipMalloc(tiles)");
+    tiles_cap_ = items;
+  }
+  if (P_ <= 0) {
+    // no items: every home block has an empty item range
+    HIPCHK(mpmk::fill_u32(d_block_items_, 2 * nblk_, 0u, s), "clear block items");
+    steps_since_sort_ = 0;
+    items_valid_ = true;
+    return MPM_OK;
+  }
   const ParticlesDev src = particles();
   int out = 0;
   ParticlesDev dst = src;
   dst.f = d_f_[cur_ ^ 1];
   dst.mat = d_mat_[cur_ ^ 1];
   dst.id = d_id_[cur_ ^ 1];
-  chunk_ = opt_.chunk > 0 ? (int)opt_.chunk : mpmk::kMaxItemParticles;
   kinds_ = 1;
   for (const auto& m : mats_)
     if (m.kind != MPM_MATERIAL_WC_LIQUID) kinds_ = 0;
@@ -430,10 +433,11 @@ int Sim::sort_particles(hipStream_t s) {
     HIPCHK(mpmk::radix_sort(d_keys_, d_vals_, P_, mpmk::cell_key_bits(k_.N, lb_), (int)opt_.radix_bits,
                             d_hist_, &out, s),
            "radix sort");
-    HIPCHK(mpmk::reorder(src, dst, d_vals_[out], s), "reorder");
-    HIPCHK(mpmk::build_items(d_keys_[out], P_, lb_, chunk_, d_block_start_, d_flag_, d_pos_, d_hist_,
-                             d_item_count_, d_item_start_, d_item_block_, s),
+    HIPCHK(mpmk::build_items(d_keys_[out], P_, lb_, chunk_, nblk_, d_block_start_, d_block_items_,
+                             d_flag_, d_pos_, d_hist_, d_item_count_, d_item_start_, d_item_block_,
+                             d_item_waves_, d_dest_, items, s),
            "work items");
+    HIPCHK(mpmk::reorder(src, dst, d_vals_[out], d_dest_, s), "reorder");
   });
   cur_ ^= 1;
   steps_since_sort_ = 0;
@@ -441,57 +445,108 @@ int Sim::sort_particles(hipStream_t s) {
   return MPM_OK;
 }
 
+int64_t Sim::max_items() const {
+  // one item per started `chunk` particles, plus one per occupied block
+  return (P_ + chunk_ - 1) / chunk_ + std::min<int64_t>(P_, nblk_);
+}
+
 mpmk::WorkItems Sim::work_items() const {
   mpmk::WorkItems w;
   w.start = d_item_start_;
   w.block = d_item_block_;
+  w.waves = d_item_waves_;
   w.count = d_item_count_;
-  int bits = 0;
-  const int eb = 1 << lb_;
-  while ((1 << bits) < ((int)prm_.N + eb - 1) / eb) ++bits;
-  const int64_t nblk = (int64_t)1 << (3 * bits);
-  w.max_items = (P_ + chunk_ - 1) / chunk_ + std::min<int64_t>(P_, nblk);
-  w.lb = lb_;
+  w.max_items = max_items();
   return w;
 }
 
-int Sim::substep(hipStream_t s) {
-  float4* gA = d_grid_[gcur_];
-  float4* gB = d_grid_[gcur_ ^ 1];
-  const bool tiled = opt_.p2g_mode == 1;
-  if (tiled && (!items_valid_ || (opt_.sort_every > 0 && steps_since_sort_ >= opt_.sort_every))) {
+mpmk::TileSums Sim::tile_sums() const {
+  mpmk::TileSums t;
+  t.tiles = d_tiles_;
+  t.atomic = d_grid_[1];
+  t.dirty = d_dirty_;
+  t.block_items = d_block_items_;
+  return t;
+}
+
+// P2G of the current particle state into the tiles (and the side grid).
+// Sorts first when the items are stale.
+int Sim::tiled_p2g(hipStream_t s) {
+  if (!items_valid_ || (opt_.sort_every > 0 && steps_since_sort_ >= opt_.sort_every)) {
     int rc = sort_particles(s);
     if (rc) return rc;
   }
-  const ParticlesDev P = particles();
+  if (!side_clean_) {
+    PHASE(PH_CLEAR, {
+      HIPCHK(mpmk::fill_zero(d_grid_[1], grid_nodes_, s), "clear side grid");
+      HIPCHK(mpmk::fill_u32(d_dirty_, (int64_t)dirty_cap_, 0u, s), "clear dirty flags");
+    });
+    side_clean_ = true;
+  }
+  PHASE(PH_P2G, HIPCHK(mpmk::g2p2g_block(particles(), d_mats_, (int)mats_.size(), k_, work_items(),
+                                         nullptr, tile_sums(), 0, kinds_, s),
+                       "p2g"));
+  return MPM_OK;
+}
+
+int Sim::substep(hipStream_t s) {
+  const ParticlesDev P0 = particles();
   const int nm = (int)mats_.size();
-  const mpmk::WorkItems W = work_items();
+  if (opt_.p2g_mode == 1) {
+    if (!opt_.fused) {
+      // reference order: P2G, grid update, G2P
+      int rc = tiled_p2g(s);
+      if (rc) return rc;
+      PHASE(PH_GRID_UPDATE, HIPCHK(mpmk::tile_reduce(tile_sums(), k_, d_grid_[0], 1, s), "grid update"));
+      PHASE(PH_G2P, HIPCHK(mpmk::g2p(particles(), d_mats_, k_, d_grid_[0], s), "g2p"));
+      vel_grid_ = d_grid_[0];
+      primed_ = false;
+      ++steps_since_sort_;
+      return MPM_OK;
+    }
+    if (!primed_) {
+      int rc = tiled_p2g(s);
+      if (rc) return rc;
+      primed_ = true;
+    }
+    // the tiles hold the P2G of this substep (items of the current order)
+    PHASE(PH_GRID_UPDATE, HIPCHK(mpmk::tile_reduce(tile_sums(), k_, d_grid_[0], 1, s), "grid update"));
+    if (!items_valid_ || (opt_.sort_every > 0 && steps_since_sort_ >= opt_.sort_every)) {
+      int rc = sort_particles(s);
+      if (rc) return rc;
+    }
+    PHASE(PH_G2P2G, HIPCHK(mpmk::g2p2g_block(particles(), d_mats_, nm, k_, work_items(), d_grid_[0],
+                                             tile_sums(), 1, kinds_, s),
+                           "g2p2g"));
+    vel_grid_ = d_grid_[0];
+    ++steps_since_sort_;
+    return MPM_OK;
+  }
+  // direct P2G (global atomics), two grids
+  side_clean_ = false;
+  float4* gA = d_grid_[gcur_];
+  float4* gB = d_grid_[gcur_ ^ 1];
   if (!opt_.fused) {
     // reference order: clear, P2G, grid update, G2P
     PHASE(PH_CLEAR, HIPCHK(mpmk::fill_zero(gA, grid_nodes_, s), "clear grid"));
-    if (tiled) PHASE(PH_P2G, HIPCHK(mpmk::g2p2g_block(P, d_mats_, nm, k_, W, nullptr, gA, 0, kinds_, s), "p2g"));
-    else PHASE(PH_P2G, HIPCHK(mpmk::p2g_direct(P, d_mats_, k_, gA, s), "p2g"));
+    PHASE(PH_P2G, HIPCHK(mpmk::p2g_direct(P0, d_mats_, k_, gA, s), "p2g"));
     PHASE(PH_GRID_UPDATE, HIPCHK(mpmk::grid_update(gA, nullptr, k_, s), "grid update"));
-    PHASE(PH_G2P, HIPCHK(mpmk::g2p(P, d_mats_, k_, gA, s), "g2p"));
-    gcur_ ^= 1;  // gA now holds the velocities: keep it as "last update"
-    have_vel_ = true;
+    PHASE(PH_G2P, HIPCHK(mpmk::g2p(P0, d_mats_, k_, gA, s), "g2p"));
+    vel_grid_ = gA;
+    gcur_ ^= 1;
     primed_ = false;
-    ++steps_since_sort_;
     return MPM_OK;
   }
   if (!primed_) {
     PHASE(PH_CLEAR, HIPCHK(mpmk::fill_zero(gA, grid_nodes_, s), "clear grid"));
-    if (tiled) PHASE(PH_P2G, HIPCHK(mpmk::g2p2g_block(P, d_mats_, nm, k_, W, nullptr, gA, 0, kinds_, s), "p2g"));
-    else PHASE(PH_P2G, HIPCHK(mpmk::p2g_direct(P, d_mats_, k_, gA, s), "p2g"));
+    PHASE(PH_P2G, HIPCHK(mpmk::p2g_direct(P0, d_mats_, k_, gA, s), "p2g"));
     primed_ = true;
   }
   // gA: P2G of this substep -> velocities; gB cleared in the same pass
   PHASE(PH_GRID_UPDATE, HIPCHK(mpmk::grid_update(gA, gB, k_, s), "grid update"));
-  if (tiled) PHASE(PH_G2P2G, HIPCHK(mpmk::g2p2g_block(P, d_mats_, nm, k_, W, gA, gB, 1, kinds_, s), "g2p2g"));
-  else PHASE(PH_G2P2G, HIPCHK(mpmk::g2p2g_direct(P, d_mats_, k_, gA, gB, s), "g2p2g"));
+  PHASE(PH_G2P2G, HIPCHK(mpmk::g2p2g_direct(P0, d_mats_, k_, gA, gB, s), "g2p2g"));
+  vel_grid_ = gA;
   gcur_ ^= 1;  // gB (primed P2G of the next substep) becomes current
-  have_vel_ = true;
-  ++steps_since_sort_;
   return MPM_OK;
 }
 
@@ -583,10 +638,8 @@ int Sim::download_aos104(void* out, hipStream_t s) {
 
 int Sim::download_grid(float* grid, hipStream_t s) {
   if (!grid) return fail(MPM_ERR_INVALID, "mpm_download_grid: null output");
-  if (!have_vel_) return fail(MPM_ERR_STATE, "mpm_download_grid: no grid update has run yet");
-  // after a fused or unfused substep the velocities of the last update are in
-  // the grid that is not current
-  HIPCHK(hipMemcpyAsync(grid, d_grid_[gcur_ ^ 1], grid_nodes_ * sizeof(float4), hipMemcpyDeviceToHost, s), "download grid");
+  if (!vel_grid_) return fail(MPM_ERR_STATE, "mpm_download_grid: no grid update has run yet");
+  HIPCHK(hipMemcpyAsync(grid, vel_grid_, grid_nodes_ * sizeof(float4), hipMemcpyDeviceToHost, s), "download grid");
   HIPCHK(hipStreamSynchronize(s), "download grid sync");
   return MPM_OK;
 }
@@ -598,19 +651,24 @@ int Sim::debug_p2g(float* grid, hipStream_t s) {
   if (rc) return rc;
   rc = upload_materials();
   if (rc) return rc;
-  // scatter into the non-current grid so a primed grid survives
-  float4* g = d_grid_[gcur_ ^ 1];
-  HIPCHK(mpmk::fill_zero(g, grid_nodes_, s), "clear grid");
+  float4* g = nullptr;
   if (opt_.p2g_mode == 1) {
-    rc = sort_particles(s);
+    // fresh sort and P2G into the tiles; the raw sums land in d_grid_[0]
+    items_valid_ = false;
+    rc = tiled_p2g(s);
     if (rc) return rc;
-    HIPCHK(mpmk::g2p2g_block(particles(), d_mats_, (int)mats_.size(), k_, work_items(), nullptr, g, 0, kinds_, s), "p2g");
+    g = d_grid_[0];
+    HIPCHK(mpmk::tile_reduce(tile_sums(), k_, g, 0, s), "tile sums");
   } else {
+    // the non-current grid, so a primed direct-path grid survives
+    g = d_grid_[gcur_ ^ 1];
+    HIPCHK(mpmk::fill_zero(g, grid_nodes_, s), "clear grid");
     HIPCHK(mpmk::p2g_direct(particles(), d_mats_, k_, g, s), "p2g");
   }
   HIPCHK(hipMemcpyAsync(grid, g, grid_nodes_ * sizeof(float4), hipMemcpyDeviceToHost, s), "download grid");
   HIPCHK(hipStreamSynchronize(s), "p2g sync");
-  have_vel_ = false;
+  if (opt_.p2g_mode == 1) primed_ = false;  // the reduction consumed the side grid
+  vel_grid_ = nullptr;
   return MPM_OK;
 }
 
@@ -705,7 +763,12 @@ int Sim::set_option(const char* name, int64_t v) {
   if (!name) return fail(MPM_ERR_INVALID, "mpm_set_option: null name");
   const std::string n = name;
   if (n == "fused") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "fused must be 0 or 1"); opt_.fused = v; primed_ = false; return MPM_OK; }
-  if (n == "p2g_mode") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "p2g_mode must be 0 or 1"); opt_.p2g_mode = v; return MPM_OK; }
+  if (n == "p2g_mode") {
+    if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "p2g_mode must be 0 or 1");
+    if (v != opt_.p2g_mode) primed_ = false;
+    opt_.p2g_mode = v;
+    return MPM_OK;
+  }
   if (n == "sort_every") { if (v < 0) return fail(MPM_ERR_INVALID, "sort_every must be >= 0"); opt_.sort_every = v; return MPM_OK; }
   if (n == "radix_bits") { if (v != 4 && v != 6 && v != 8) return fail(MPM_ERR_INVALID, "radix_bits must be 4, 6 or 8"); opt_.radix_bits = v; return MPM_OK; }
   if (n == "chunk") {
@@ -716,10 +779,8 @@ int Sim::set_option(const char* name, int64_t v) {
     return MPM_OK;
   }
   if (n == "block") {
-    if (v != 0 && v != 4 && v != 8) return fail(MPM_ERR_INVALID, "block must be 0 (auto), 4 or 8");
+    if (v != 0 && v != 4) return fail(MPM_ERR_INVALID, "block must be 0 (default) or 4");
     opt_.block = v;
-    if (uploaded_) lb_ = choose_block_log2();
-    items_valid_ = false;
     return MPM_OK;
   }
   if (n == "timing") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "timing must be 0 or 1"); opt_.timing = v; return MPM_OK; }
@@ -734,7 +795,7 @@ int Sim::get_option(const char* name, int64_t* v) {
   else if (n == "sort_every") *v = opt_.sort_every;
   else if (n == "radix_bits") *v = opt_.radix_bits;
   else if (n == "chunk") *v = opt_.chunk > 0 ? opt_.chunk : chunk_;
-  else if (n == "block") *v = opt_.block > 0 ? opt_.block : (1 << lb_);
+  else if (n == "block") *v = 1 << lb_;
   else if (n == "timing") *v = opt_.timing;
   else return fail(MPM_ERR_INVALID, "mpm_get_option: unknown option '%s'", name);
   return MPM_OK;

@@ -26,7 +26,7 @@ struct Options {
   int64_t sort_every = 8;   // "sort_every": re-sort particles every k substeps (0: never)
   int64_t radix_bits = 8;   // "radix_bits": digit width of the radix sort (4, 6 or 8)
   int64_t chunk = 0;        // "chunk": max particles per work item (64..256, multiple of 64; 0 = 256)
-  int64_t block = 0;        // "block": edge of a home block in cells, 4 or 8 (0 = auto)
+  int64_t block = 0;        // "block": edge of a home block in cells (4; 0 = default)
   int64_t timing = 0;       // "timing": bracket every kernel with HIP events (per-phase time)
 };
 
@@ -109,26 +109,40 @@ class Sim {
   bool uploaded_ = false;
   uint8_t max_mat_ = 0;  // largest material index among uploaded particles
 
-  // grids: g_[0], g_[1]; `gcur_` holds the primed P2G of the next substep
+  // grids d_grid_[0], d_grid_[1]. Direct P2G path: d_grid_[gcur_] holds the
+  // primed P2G of the next substep. Tiled path: d_grid_[0] receives the
+  // updated grid, d_grid_[1] is the side grid of particles that leave their
+  // tiles (all zero, with d_dirty_, between passes while side_clean_).
   float4* d_grid_[2] = {nullptr, nullptr};
   size_t grid_nodes_ = 0;
   int gcur_ = 0;
-  bool primed_ = false;
-  bool have_vel_ = false;  // d_grid_[gcur_ ^ 1] holds velocities of the last update
+  bool primed_ = false;              // the P2G of the current particle state is pending
+  const float4* vel_grid_ = nullptr;  // grid holding the velocities of the last update
+  bool side_clean_ = false;
+  uint32_t* d_dirty_ = nullptr;      // one flag per 4^3 node block
+  size_t dirty_cap_ = 0;
 
-  // work items of the block-tiled kernel (valid after a sort)
-  uint32_t* d_block_start_ = nullptr;  // [first, end) of each grid block (Morton space)
+  // work items of the tiled kernel (valid after a sort)
+  int64_t nblk_ = 0;                   // Morton space of 4^3 home blocks
+  uint32_t* d_block_start_ = nullptr;  // [first, end) particles of each home block
+  uint32_t* d_block_items_ = nullptr;  // [first, end) items of each home block
   size_t block_start_cap_ = 0;
   uint32_t* d_flag_ = nullptr;         // cap entries
   uint32_t* d_pos_ = nullptr;          // cap entries
+  uint32_t* d_dest_ = nullptr;         // cap entries
   uint32_t* d_item_start_ = nullptr;   // cap + 1 entries
   uint32_t* d_item_block_ = nullptr;   // cap entries
+  uint32_t* d_item_waves_ = nullptr;   // cap entries
   uint32_t* d_item_count_ = nullptr;   // device scalar
-  int lb_ = 2;                         // log2 of the block edge in use
+  float4* d_tiles_ = nullptr;          // tiles_cap_ tiles of kTileNodes nodes
+  int64_t tiles_cap_ = 0;
+  const int lb_ = mpmk::kBlockLog2;    // log2 of the home block edge
   int chunk_ = 256;                    // particles per work item in use
   int kinds_ = 0;                      // 1: every material is WC_LIQUID
   bool items_valid_ = false;
-  int choose_block_log2() const;
+  int64_t max_items() const;
+  mpmk::TileSums tile_sums() const;
+  int tiled_p2g(hipStream_t s);        // P2G of the current state into the tiles
 
   // sort scratch
   uint32_t* d_keys_[2] = {nullptr, nullptr};

@@ -30,27 +30,56 @@ hipError_t radix_sort(uint32_t* keys[2], uint32_t* vals[2], int64_t n, int key_b
                       int radix_bits, uint32_t* scratch, int* out_idx, hipStream_t s);
 hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* scratch,
                           uint32_t* total, hipStream_t s);
-// Work items over sorted keys: item_start[0..*total] (sentinel n at the end)
-// and item_block[0..*total). Arrays of n+1 entries; block_range holds two
-// entries per grid block (Morton space).
-hipError_t build_items(const uint32_t* keys, int64_t n, int lb, int chunk, uint32_t* block_range,
-                       uint32_t* flag, uint32_t* pos, uint32_t* scratch, uint32_t* total,
-                       uint32_t* item_start, uint32_t* item_block, hipStream_t s);
-hipError_t reorder(const ParticlesDev& src, const ParticlesDev& dst, const uint32_t* perm,
-                   hipStream_t s);
-
-// p2g_block.hip: fused G2P + block-tiled P2G over work items
+hipError_t fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 // An item holds at most 256 particles (one per thread of its workgroup).
 constexpr int kMaxItemParticles = 256;
+// Home blocks of the tiled substep: 4^3 cells; a work item's tile spans the
+// block, the 2-node reach of its stencil and a 1-cell drift margin on each
+// side: kTileW = 8 nodes per axis.
+constexpr int kBlockLog2 = 2;
+constexpr int kTileW = 8;
+constexpr int kTileNodes = kTileW * kTileW * kTileW;
+
+// Work items over sorted keys: item_start[0..*total] (sentinel n at the end)
+// and item_block[0..*total); block_items[2b], [2b+1]: the items of block b
+// (cleared for all nblk blocks first). With dest, k_item_arrange fills dest
+// (new index of each sorted particle, a permutation inside each item) and
+// item_waves (per-wavefront particle counts). Arrays of n+1 entries;
+// block_range holds two entries per grid block (Morton space).
+hipError_t build_items(const uint32_t* keys, int64_t n, int lb, int chunk, int64_t nblk,
+                       uint32_t* block_range, uint32_t* block_items, uint32_t* flag, uint32_t* pos,
+                       uint32_t* scratch, uint32_t* total, uint32_t* item_start,
+                       uint32_t* item_block, uint32_t* item_waves, uint32_t* dest,
+                       int64_t max_items, hipStream_t s);
+// dst[dest ? dest[i] : i] = src[perm[i]]
+hipError_t reorder(const ParticlesDev& src, const ParticlesDev& dst, const uint32_t* perm,
+                   const uint32_t* dest, hipStream_t s);
+
+// p2g_block.hip: fused G2P + tiled P2G over work items, and the tile reduction
 struct WorkItems {
   const uint32_t* start;   // n_items + 1 entries (last = P)
   const uint32_t* block;   // Morton index of the item's home block
+  const uint32_t* waves;   // per-wavefront particle counts, 8 bits each
   const uint32_t* count;   // device scalar: number of items
   int64_t max_items;       // launch bound
-  int lb;                  // log2 of the block edge
+};
+// Tiled P2G output of one pass: one kTileNodes-node float4 tile per item,
+// plus the grid `atomic` (kept zero between passes) that particles leaving
+// their tile add into, with `dirty` flags (one per 4^3 node block) marking
+// where they did.
+struct TileSums {
+  float4* tiles;
+  float4* atomic;
+  uint32_t* dirty;
+  const uint32_t* block_items;  // 2 entries per home block (Morton space)
 };
 hipError_t g2p2g_block(const ParticlesDev& P, const MaterialDev* mats, int nmats,
-                       const StepConsts& k, const WorkItems& w, const float4* grid_in,
-                       float4* grid_out, int do_g2p, int kinds, hipStream_t s);
+                       const StepConsts& k, const WorkItems& w, const float4* grid_v,
+                       const TileSums& out, int do_g2p, int kinds, hipStream_t s);
+// Sums the tiles (and the dirty parts of the atomic grid, which it clears)
+// into every node of `grid`; update=1 applies the grid update (velocities),
+// update=0 leaves the raw P2G sums (momentum, mass).
+hipError_t tile_reduce(const TileSums& in, const StepConsts& k, float4* grid, int update,
+                       hipStream_t s);
 
 }  // namespace mpmk

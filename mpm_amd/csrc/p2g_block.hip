@@ -1,27 +1,33 @@
-// p2g_block.hip -- fused G2P + block-tiled P2G (gfx950).
+// p2g_block.hip -- fused G2P + tiled P2G over work items, and the tile
+// reduction that turns the P2G tiles into the updated grid (gfx950).
 //
-// The particles are cell-sorted (sort.hip) and cut into work items: runs of
-// at most 256 particles that share one home block of BE^3 grid cells. One
-// workgroup of 256 threads runs one item, one particle per thread:
-//   1. stage the grid velocities of the block's node region in LDS (the block,
-//      its 2-node stencil reach and a 1-cell drift margin on each side:
-//      W = BE + 4 nodes per axis);
-//   2. G2P of substep n (gridToParticle, reference src/mpm.cu:109-178)
-//      gathering the 27 nodes from LDS -- C/F update, plasticity with the SVD
-//      in registers, advection -- and the store of the new state;
-//   3. P2G of substep n+1 (particleToGrid, mpm.cu:14-74) as a gather: each
-//      particle writes its per-axis weights and momentum terms to LDS, slotted
-//      by a counting sort on its new cell (LDS integer counters); then each
-//      thread owns tile nodes and sums, in registers, the contributions of the
-//      particles in the 27 cells around each node. No floating-point atomics
-//      touch LDS (gfx950 retires LDS FP32 atomics at a small fraction of the
-//      rate of plain LDS reads), and the order of the sum is fixed;
-//   4. add the non-empty tile nodes to the global grid with FP32 atomics,
-//      consecutive lanes on consecutive nodes of a row.
-// Particles whose stencil leaves the tile (drift beyond the margin between
-// sorts) or the grid (where the reference's node bounds and index aliasing
-// apply) take the direct path: global gathers and global atomics with the
-// reference's index rules.
+// The particles are cell-sorted (sort.hip) and cut into work items: at most
+// 256 particles of one home block of 4^3 cells, arranged over the 4
+// wavefronts of a workgroup so that particles sharing a cell sit in different
+// wavefronts (k_item_arrange). One workgroup runs one item:
+//   1. stage the grid velocities of the item's node region in LDS: the block,
+//      its 2-node stencil reach and a 1-cell drift margin on each side
+//      (8^3 nodes);
+//   2. G2P of substep n (gridToParticle, reference src/mpm.cu:109-178), one
+//      particle per thread, gathering the 27 nodes from LDS -- C/F update,
+//      plasticity with the SVD in registers, advection -- and the store of
+//      the new state;
+//   3. P2G of substep n+1 (particleToGrid, mpm.cu:14-74) into a private LDS
+//      tile of each wavefront with plain read-modify-writes (gfx950 retires
+//      LDS FP32 atomics at a small fraction of the rate of plain LDS
+//      accesses). Within one instruction the lanes of a wavefront write 27
+//      distinct nodes when their cells are distinct; lanes that share a cell
+//      take turns (a per-wavefront claim byte per cell decides who goes);
+//   4. store the sum of the 4 wavefront tiles as the item's tile in HBM
+//      (plain coalesced stores, no atomics).
+// k_tile_reduce then builds every grid node from the tiles that cover it
+// (the items of the 27 home blocks around its node block), in a fixed order,
+// and applies the grid update (gridOpKernel, mpm.cu:76-107) in the same pass.
+// The sum order is a function of the sorted particle order only, so a run is
+// bit-reproducible. Particles whose stencil leaves their tile (drift beyond
+// the margin between sorts) or the grid (where the reference's node bounds
+// and index aliasing apply) take the direct path: global gathers, and global
+// atomics into a side grid that k_tile_reduce adds and clears.
 #include <hip/hip_runtime.h>
 
 #include "device_math.hpp"
@@ -113,10 +119,12 @@ __device__ __forceinline__ void gather(const PState& s, const int base[3], const
 // Direct scatter with the reference's node bounds and linear-index rules
 // (mpm.cu:39-73): compat REFERENCE bounds the upper end by N + base and lets
 // j, k >= N alias into the next row; indices >= N^3 are dropped.
+// With `dirty`, every touched node also flags its 4^3 node block.
 __device__ __forceinline__ void scatter_global(const float q0[3][3], const float q1[3][3],
                                                const float q2[3][3], float mass,
                                                const int base[3], const float w[3][3],
-                                               const StepConsts& k, float* __restrict__ grid) {
+                                               const StepConsts& k, float* __restrict__ grid,
+                                               uint32_t* __restrict__ dirty = nullptr) {
   const int N = k.N;
   int beg[3], end[3];
 #pragma unroll
@@ -140,6 +148,12 @@ __device__ __forceinline__ void scatter_global(const float q0[3][3], const float
         if ((uint64_t)idx >= n3) continue;
         const float wt = wij * w[2][kk];
         float* cell = grid + 4 * (size_t)idx;
+        if (dirty) {
+          const uint32_t nbx = ((uint32_t)N + 3u) >> 2;
+          const uint32_t gx = idx / ((uint32_t)N * (uint32_t)N), gy = (idx / (uint32_t)N) % (uint32_t)N,
+                         gz = idx % (uint32_t)N;
+          dirty[((gx >> 2) * nbx + (gy >> 2)) * nbx + (gz >> 2)] = 1u;
+        }
         atomic_add_global(cell + 0, wt * (q0[0][i] + (q1[0][j] + q2[0][kk])));
         atomic_add_global(cell + 1, wt * (q0[1][i] + (q1[1][j] + q2[1][kk])));
         atomic_add_global(cell + 2, wt * (q0[2][i] + (q1[2][j] + q2[2][kk])));
@@ -149,61 +163,39 @@ __device__ __forceinline__ void scatter_global(const float q0[3][3], const float
   }
 }
 
-__device__ __forceinline__ float pick3(const float a[3], int r) {
-  return r == 0 ? a[0] : (r == 1 ? a[1] : a[2]);
-}
-__device__ __forceinline__ int pick3i(const int a[3], int r) {
-  return r == 0 ? a[0] : (r == 1 ? a[1] : a[2]);
-}
-
-
-
-__device__ __forceinline__ uint32_t compact3(uint32_t v) {
-  // inverse of the 3-way bit interleave: keep bits 0, 3, 6, ...
-  v &= 0x09249249u;
-  v = (v | (v >> 2)) & 0x030c30c3u;
-  v = (v | (v >> 4)) & 0x0300f00fu;
-  v = (v | (v >> 8)) & 0x030000ffu;
-  v = (v | (v >> 16)) & 0x000003ffu;
-  return v;
-}
-
 }  // namespace
 
 // Material-kind sets compiled into an instantiation.
 enum KindSet : int { KINDS_ALL = 0, KINDS_LIQUID = 1 };
 
-template <int LB, int KS>
+template <int KS>
 __global__ void __launch_bounds__(256)
-    k_g2p2g_block(ParticlesDev P, const MaterialDev* __restrict__ mats, StepConsts k,
+    k_g2p2g_tiled(ParticlesDev P, const MaterialDev* __restrict__ mats, StepConsts k,
                   const uint32_t* __restrict__ item_start, const uint32_t* __restrict__ item_block,
-                  const uint32_t* __restrict__ item_count, const float4* __restrict__ grid_in,
-                  float* __restrict__ grid_out, int do_g2p) {
-  constexpr int BE = 1 << LB;
-  constexpr int MARGIN = 1;
-  constexpr int W = BE + 2 + 2 * MARGIN;  // tile width in nodes
-  constexpr int TN = W * W * W;           // tile nodes
-  constexpr int NCE = W - 2;              // cells whose 3x3x3 stencil fits the tile, per axis
-  constexpr int NC = NCE * NCE * NCE;
-  // The velocity tile (read by G2P) and the particle packets (written after
-  // the first barrier that follows every G2P read) share one LDS buffer.
-  constexpr int PK_F4 = 9 * 256, U_F4 = PK_F4 > TN ? PK_F4 : TN;
-  __shared__ float4 ubuf[U_F4];
-  float4* const vt = ubuf;                // staged grid velocities (G2P)
-  float4 (*const pk)[256] = reinterpret_cast<float4 (*)[256]>(ubuf);  // [axis*3+offset][slot]: (q, w)
-  __shared__ float pm[256];               // particle mass, by slot
-  __shared__ int cstart[NC + 1];          // per-cell counts, then first slot
-  __shared__ int red[8];
+                  const uint32_t* __restrict__ item_waves, const uint32_t* __restrict__ item_count,
+                  const float4* __restrict__ grid_v, float4* __restrict__ tiles,
+                  float* __restrict__ grid_atomic, uint32_t* __restrict__ dirty, int do_g2p) {
+  constexpr int BE = 1 << mpmk::kBlockLog2;
+  constexpr int W = mpmk::kTileW;        // BE + 2 (stencil reach) + 2 (drift margin)
+  constexpr int TN = mpmk::kTileNodes;
+  constexpr int NCE = W - 2;             // cells per axis whose stencil fits the tile
+  static_assert(NCE * NCE * NCE <= 256, "claim bytes per wavefront");
+  __shared__ float4 vt[TN];              // grid velocities; claim bytes once G2P is done
+  __shared__ float4 acc[4][TN];          // P2G tile of each wavefront
 
   const int item = blockIdx.x;
   if ((uint32_t)item >= *item_count) return;  // uniform for the workgroup
-  const int tid = threadIdx.x;
-  const uint32_t start = item_start[item], end = item_start[item + 1];  // end - start <= 256
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const uint32_t start = item_start[item];
+  const uint32_t wc = item_waves[item];
   const uint32_t blk = item_block[item];
+  int off_w = 0;
+  for (int q = 0; q < wave; ++q) off_w += (int)((wc >> (8 * q)) & 255u);
+  const int cnt_w = (int)((wc >> (8 * wave)) & 255u);
   const int N = k.N;
-  const int o0 = (int)compact3(blk >> 2) * BE - MARGIN;
-  const int o1 = (int)compact3(blk >> 1) * BE - MARGIN;
-  const int o2 = (int)compact3(blk) * BE - MARGIN;
+  const int o0 = (int)compact3(blk >> 2) * BE - 1;
+  const int o1 = (int)compact3(blk >> 1) * BE - 1;
+  const int o2 = (int)compact3(blk) * BE - 1;
 
   if (do_g2p) {
     for (int n = tid; n < TN; n += 256) {
@@ -211,15 +203,18 @@ __global__ void __launch_bounds__(256)
       const int gi = o0 + a, gj = o1 + b, gk = o2 + c;
       float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
       if (gi >= 0 && gi < N && gj >= 0 && gj < N && gk >= 0 && gk < N)
-        v = grid_in[((size_t)gi * N + (size_t)gj) * N + (size_t)gk];
+        v = grid_v[((size_t)gi * N + (size_t)gj) * N + (size_t)gk];
       vt[n] = v;
     }
   }
-  for (int t = tid; t <= NC; t += 256) cstart[t] = 0;
+  {
+    float4* const flat = &acc[0][0];
+    for (int n = tid; n < 4 * TN; n += 256) flat[n] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
   __syncthreads();
 
-  const uint32_t i = start + (uint32_t)tid;
-  const bool valid = i < end;
+  const bool valid = lane < cnt_w;
+  const uint32_t i = start + (uint32_t)(off_w + lane);
   PState s;
   MaterialDev m;
   float R[9];
@@ -262,7 +257,7 @@ __global__ void __launch_bounds__(256)
             }
           }
       } else {
-        gather(s, base, w, k, grid_in, s.v, s.C);
+        gather(s, base, w, k, grid_v, s.v, s.C);
       }
       float A[9];
 #pragma unroll
@@ -287,9 +282,10 @@ __global__ void __launch_bounds__(256)
   }
 
   // ---- P2G preparation (TransferScheme.h:66-86): stress and affine momentum.
-  // Node value_c(a,b,c) = wt * (q0[c][a] + (q1[c][b] + q2[c][c'])), with
+  // Node value (a,b,c) = wt * (q0[.][a] + (q1[.][b] + q2[.][c])), with
   // q0 = m v + aff(:,0) d0, q1 = aff(:,1) d1, q2 = aff(:,2) d2.
-  int cell = -1, rank = 0;
+  int cell = 0, nb = 0;
+  bool pending = false;
   float q0[3][3], q1[3][3], q2[3][3];
   if (inside) {
     float tau[9];
@@ -325,130 +321,166 @@ __global__ void __launch_bounds__(256)
                          base[1] + 2 < N && base[2] + 2 < N;
     if (in_tile) {
       cell = (l0 * NCE + l1) * NCE + l2;
-      rank = atomicAdd(&cstart[cell], 1);  // LDS integer counter: slot within the cell
+      nb = (l0 * W + l1) * W + l2;
+      pending = true;
     } else {
-      scatter_global(q0, q1, q2, m.mass, base, w, k, grid_out);
+      scatter_global(q0, q1, q2, m.mass, base, w, k, grid_atomic, dirty);
     }
   }
-  __syncthreads();
+  __syncthreads();  // every G2P read of vt is done: its bytes become claim bytes
 
-  // ---- cell ranges: exclusive scan of the NC counts (each thread owns a run)
-  {
-    constexpr int PER = (NC + 255) / 256;
-    int loc[PER];
-    int sum = 0;
+  // ---- P2G into the wavefront's tile. Per round, every pending lane claims
+  // its cell; the lane whose claim stands adds its 27 nodes. The claimed
+  // cells are distinct, so the 64 lanes of one read-modify-write address
+  // distinct nodes; the compiler barrier keeps the 27 read-modify-writes in
+  // program order (a lane's next read may address a node another lane is
+  // writing in the same round).
+  typedef __attribute__((address_space(3))) volatile uint8_t lds_byte;
+  lds_byte* const claim = (lds_byte*)(reinterpret_cast<uint8_t*>(vt) + wave * 256);
+  float4* const my = acc[wave];
+  while (__ballot(pending) != 0ull) {
+    if (pending) claim[cell] = (uint8_t)lane;
+    if (pending && claim[cell] == (uint8_t)lane) {
 #pragma unroll
-    for (int t = 0; t < PER; ++t) {
-      const int c = tid * PER + t;
-      loc[t] = (c < NC) ? cstart[c] : 0;
-      sum += loc[t];
-    }
-    // workgroup exclusive scan of `sum` (wave scans, then wave totals)
-    int incl = sum;
+      for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int y = __shfl_up(incl, off);
-      if ((tid & 63) >= off) incl += y;
-    }
-    if ((tid & 63) == 63) red[tid >> 6] = incl;
-    __syncthreads();
-    int wave_base = 0;
-    for (int q = 0; q < (tid >> 6); ++q) wave_base += red[q];
-    int run = wave_base + incl - sum;
+        for (int b = 0; b < 3; ++b) {
+          const float wab = w[0][a] * w[1][b];
 #pragma unroll
-    for (int t = 0; t < PER; ++t) {
-      const int c = tid * PER + t;
-      if (c < NC) cstart[c] = run;
-      run += loc[t];
-    }
-    if (tid == 255) cstart[NC] = run;
-  }
-  __syncthreads();
-  if (cell >= 0) {
-    const int slot = cstart[cell] + rank;
-#pragma unroll
-    for (int o = 0; o < 3; ++o) {
-      pk[0 * 3 + o][slot] = make_float4(q0[0][o], q0[1][o], q0[2][o], w[0][o]);
-      pk[1 * 3 + o][slot] = make_float4(q1[0][o], q1[1][o], q1[2][o], w[1][o]);
-      pk[2 * 3 + o][slot] = make_float4(q2[0][o], q2[1][o], q2[2][o], w[2][o]);
-    }
-    pm[slot] = m.mass;
-  }
-  __syncthreads();
-
-  // ---- P2G as a gather over the tile: each thread sums, in registers, the
-  // contributions of the particles in the 27 cells around each of its nodes,
-  // then adds the node to the global grid (coalesced FP32 atomics)
-  for (int n = tid; n < TN; n += 256) {
-    const int a = n / (W * W), r = n - a * W * W, b = r / W, c = r - b * W;
-    float ax = 0.f, ay = 0.f, az = 0.f, am = 0.f;
-    bool touched = false;
-#pragma unroll
-    for (int oa = 0; oa < 3; ++oa) {
-      const int ca = a - oa;
-      if (ca < 0 || ca >= NCE) continue;
-#pragma unroll
-      for (int ob = 0; ob < 3; ++ob) {
-        const int cb = b - ob;
-        if (cb < 0 || cb >= NCE) continue;
-#pragma unroll
-        for (int oc = 0; oc < 3; ++oc) {
-          const int cc = c - oc;
-          if (cc < 0 || cc >= NCE) continue;
-          const int cl = (ca * NCE + cb) * NCE + cc;
-          const int j0 = cstart[cl], j1 = cstart[cl + 1];
-          touched |= j1 > j0;
-          for (int j = j0; j < j1; ++j) {
-            const float4 A = pk[oa][j];
-            const float4 B = pk[3 + ob][j];
-            const float4 Cq = pk[6 + oc][j];
-            const float wt = (A.w * B.w) * Cq.w;
-            ax += wt * (A.x + (B.x + Cq.x));
-            ay += wt * (A.y + (B.y + Cq.y));
-            az += wt * (A.z + (B.z + Cq.z));
-            am += wt * pm[j];
+          for (int c = 0; c < 3; ++c) {
+            const float wt = wab * w[2][c];
+            float4* const p = my + nb + (a * W + b) * W + c;
+            float4 t = *p;
+            t.x += wt * (q0[0][a] + (q1[0][b] + q2[0][c]));
+            t.y += wt * (q0[1][a] + (q1[1][b] + q2[1][c]));
+            t.z += wt * (q0[2][a] + (q1[2][b] + q2[2][c]));
+            t.w += wt * m.mass;
+            *p = t;
+            __asm__ __volatile__("" ::: "memory");
           }
         }
-      }
-    }
-    if (touched) {
-      const size_t idx = ((size_t)(o0 + a) * N + (size_t)(o1 + b)) * N + (size_t)(o2 + c);
-      float* cellp = grid_out + 4 * idx;
-      atomic_add_global(cellp + 0, ax);
-      atomic_add_global(cellp + 1, ay);
-      atomic_add_global(cellp + 2, az);
-      atomic_add_global(cellp + 3, am);
+      pending = false;
     }
   }
+  __syncthreads();
+
+  // ---- the item's tile: sum of the wavefront tiles, fixed order
+  float4* const out = tiles + (size_t)item * TN;
+  for (int n = tid; n < TN; n += 256) {
+    const float4 a = acc[0][n], b = acc[1][n], c = acc[2][n], e = acc[3][n];
+    out[n] = make_float4(((a.x + b.x) + c.x) + e.x, ((a.y + b.y) + c.y) + e.y,
+                         ((a.z + b.z) + c.z) + e.z, ((a.w + b.w) + c.w) + e.w);
+  }
+}
+
+// One wavefront per 4^3 node block: each lane sums its node over the tiles of
+// the items of the home blocks around it, adds the side grid where particles
+// left their tiles (and clears it), and writes the node: grid update
+// (update=1) or raw sums. Home block h covers nodes 4h-1 .. 4h+6 per axis, so
+// a node with in-block coordinate u is covered by home blocks h-1 and h
+// (u < 3) or h and h+1 (u = 3) on each axis: 8 home blocks per node. Lane
+// j < 27 fetches the item range of neighbour j of the node block; each lane
+// picks the ranges of its 8 blocks with lane shuffles and reads the first
+// item of each in one batch of independent loads, further items (blocks
+// holding more than one item) after it. The order of the sum is fixed.
+__global__ void __launch_bounds__(256)
+    k_tile_reduce(const float4* __restrict__ tiles, const uint32_t* __restrict__ block_items,
+                  float4* __restrict__ grid_atomic, uint32_t* __restrict__ dirty,
+                  float4* __restrict__ grid, StepConsts k, int nbx, int update) {
+  constexpr int W = mpmk::kTileW, TN = mpmk::kTileNodes;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t nbk = (int64_t)blockIdx.x * 4 + wave;
+  if (nbk >= (int64_t)nbx * nbx * nbx) return;  // uniform for the wavefront
+  const int bz = (int)(nbk % nbx), by = (int)((nbk / nbx) % nbx), bx = (int)(nbk / ((int64_t)nbx * nbx));
+  const int u0 = lane >> 4, u1 = (lane >> 2) & 3, u2 = lane & 3;
+  uint32_t r0 = 0, r1 = 0;
+  if (lane < 27) {
+    const int hx = bx + lane / 9 - 1, hy = by + (lane / 3) % 3 - 1, hz = bz + lane % 3 - 1;
+    if (hx >= 0 && hx < nbx && hy >= 0 && hy < nbx && hz >= 0 && hz < nbx) {
+      const uint32_t mb = morton3((uint32_t)hx, (uint32_t)hy, (uint32_t)hz);
+      r0 = block_items[2 * (size_t)mb];
+      r1 = block_items[2 * (size_t)mb + 1];
+    }
+  }
+  const int N = k.N;
+  const int x = bx * 4 + u0, y = by * 4 + u1, z = bz * 4 + u2;
+  const bool in_grid = x < N && y < N && z < N;
+  const size_t node = ((size_t)x * N + (size_t)y) * N + (size_t)z;
+  const bool is_dirty = dirty[nbk] != 0u;
+  float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (__ballot(r0 < r1) != 0ull) {  // some home block around holds items
+    const int e0 = (u0 == 3) ? 0 : -1, e1 = (u1 == 3) ? 0 : -1, e2 = (u2 == 3) ? 0 : -1;
+    float4 v[8];
+    uint32_t a0[8], a1[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int f0 = e0 + (q >> 2), f1 = e1 + ((q >> 1) & 1), f2 = e2 + (q & 1);
+      const int j = 9 * (f0 + 1) + 3 * (f1 + 1) + (f2 + 1);
+      a0[q] = (uint32_t)__shfl((int)r0, j);
+      a1[q] = (uint32_t)__shfl((int)r1, j);
+      const int t = ((u0 + 1 - 4 * f0) * W + (u1 + 1 - 4 * f1)) * W + (u2 + 1 - 4 * f2);
+      a1[q] = a0[q] < a1[q] ? a1[q] : a0[q];
+      v[q] = a0[q] < a1[q] ? tiles[(size_t)a0[q] * TN + t] : make_float4(0.f, 0.f, 0.f, 0.f);
+      a0[q] = (uint32_t)t;  // tile node from here on
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      sum.x += v[q].x; sum.y += v[q].y; sum.z += v[q].z; sum.w += v[q].w;
+    }
+    // further items of blocks holding more than one
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int f0 = e0 + (q >> 2), f1 = e1 + ((q >> 1) & 1), f2 = e2 + (q & 1);
+      const int j = 9 * (f0 + 1) + 3 * (f1 + 1) + (f2 + 1);
+      const uint32_t first = (uint32_t)__shfl((int)r0, j);
+      for (uint32_t it = first + 1; it < a1[q]; ++it) {
+        const float4 w4 = tiles[(size_t)it * TN + a0[q]];
+        sum.x += w4.x; sum.y += w4.y; sum.z += w4.z; sum.w += w4.w;
+      }
+    }
+  }
+  if (is_dirty) {
+    if (in_grid) {
+      const float4 g = grid_atomic[node];
+      sum.x += g.x; sum.y += g.y; sum.z += g.z; sum.w += g.w;
+      grid_atomic[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (lane == 0) dirty[nbk] = 0u;
+  }
+  if (in_grid) grid[node] = update ? grid_node_update(sum, x, y, z, k) : sum;
 }
 
 namespace mpmk {
 
-template <int LB, int KS>
-static void launch_block(const ParticlesDev& P, const MaterialDev* mats, int nmats,
-                         const StepConsts& k, const WorkItems& w, const float4* gin, float4* gout,
-                         int do_g2p, hipStream_t s) {
-  hipLaunchKernelGGL((k_g2p2g_block<LB, KS>), dim3((unsigned)w.max_items), dim3(256), 0, s, P, mats,
-                     k, w.start, w.block, w.count, gin, (float*)gout, do_g2p);
+template <int KS>
+static void launch_tiled(const ParticlesDev& P, const MaterialDev* mats, const StepConsts& k,
+                         const WorkItems& w, const float4* gv, const TileSums& o, int do_g2p,
+                         hipStream_t s) {
+  hipLaunchKernelGGL((k_g2p2g_tiled<KS>), dim3((unsigned)w.max_items), dim3(256), 0, s, P, mats, k,
+                     w.start, w.block, w.waves, w.count, gv, o.tiles, (float*)o.atomic, o.dirty,
+                     do_g2p);
 }
 
 hipError_t g2p2g_block(const ParticlesDev& P, const MaterialDev* mats, int nmats,
-                       const StepConsts& k, const WorkItems& w, const float4* grid_in,
-                       float4* grid_out, int do_g2p, int kinds, hipStream_t s) {
+                       const StepConsts& k, const WorkItems& w, const float4* grid_v,
+                       const TileSums& out, int do_g2p, int kinds, hipStream_t s) {
   if (P.n <= 0 || w.max_items <= 0) return hipSuccess;
   if (nmats < 1 || nmats > MPM_MAX_MATERIALS) return hipErrorInvalidValue;
-  if (do_g2p && !grid_in) return hipErrorInvalidValue;
+  if (do_g2p && !grid_v) return hipErrorInvalidValue;
+  if (!out.tiles || !out.atomic || !out.dirty) return hipErrorInvalidValue;
   if (w.max_items > 0x7fffffff) return hipErrorInvalidValue;
-  const bool liquid = kinds == KINDS_LIQUID;
-  if (w.lb == 2) {
-    if (liquid) launch_block<2, KINDS_LIQUID>(P, mats, nmats, k, w, grid_in, grid_out, do_g2p, s);
-    else launch_block<2, KINDS_ALL>(P, mats, nmats, k, w, grid_in, grid_out, do_g2p, s);
-  } else if (w.lb == 3) {
-    if (liquid) launch_block<3, KINDS_LIQUID>(P, mats, nmats, k, w, grid_in, grid_out, do_g2p, s);
-    else launch_block<3, KINDS_ALL>(P, mats, nmats, k, w, grid_in, grid_out, do_g2p, s);
-  } else {
-    return hipErrorInvalidValue;
-  }
+  if (kinds == KINDS_LIQUID) launch_tiled<KINDS_LIQUID>(P, mats, k, w, grid_v, out, do_g2p, s);
+  else launch_tiled<KINDS_ALL>(P, mats, k, w, grid_v, out, do_g2p, s);
+  return hipGetLastError();
+}
+
+hipError_t tile_reduce(const TileSums& in, const StepConsts& k, float4* grid, int update,
+                       hipStream_t s) {
+  if (!in.tiles || !in.atomic || !in.dirty || !in.block_items || !grid) return hipErrorInvalidValue;
+  const int nbx = (k.N + 3) / 4;
+  const int64_t nblocks = (int64_t)nbx * nbx * nbx;
+  hipLaunchKernelGGL(k_tile_reduce, dim3((unsigned)((nblocks + 3) / 4)), dim3(256), 0, s, in.tiles,
+                     in.block_items, in.atomic, in.dirty, grid, k, nbx, update);
   return hipGetLastError();
 }
 

@@ -32,16 +32,6 @@ constexpr int kSortItems = 8;
 constexpr int kTile = kSortThreads * kSortItems;
 constexpr int kWaves = kSortThreads / 64;
 
-__device__ __forceinline__ uint32_t spread3(uint32_t v) {
-  // interleave the low 10 bits of v with two zero bits between each
-  v &= 0x3ffu;
-  v = (v | (v << 16)) & 0x030000ffu;
-  v = (v | (v << 8)) & 0x0300f00fu;
-  v = (v | (v << 4)) & 0x030c30c3u;
-  v = (v | (v << 2)) & 0x09249249u;
-  return v;
-}
-
 // Key of the cell containing x: blocks of BE^3 cells (BE = 1 << lb) in Morton
 // order, cells inside a block in row-major order. Cells outside the grid are
 // clamped to the nearest cell.
@@ -53,8 +43,7 @@ __device__ __forceinline__ uint32_t cell_key_of(const float x[3], const StepCons
     b = b < 0 ? 0 : (b >= k.N ? k.N - 1 : b);
     c[a] = b;
   }
-  const uint32_t m = (spread3((uint32_t)c[0] >> lb) << 2) | (spread3((uint32_t)c[1] >> lb) << 1) |
-                     spread3((uint32_t)c[2] >> lb);
+  const uint32_t m = mpmdev::morton3((uint32_t)c[0] >> lb, (uint32_t)c[1] >> lb, (uint32_t)c[2] >> lb);
   const uint32_t mask = (1u << lb) - 1u;
   const uint32_t local = ((((uint32_t)c[0] & mask) << lb | ((uint32_t)c[1] & mask)) << lb) |
                          ((uint32_t)c[2] & mask);
@@ -100,17 +89,118 @@ __global__ void __launch_bounds__(256)
   flag[i] = (((uint32_t)i - s) % size == 0u) ? 1u : 0u;
 }
 
+// Items in sorted order: item_start/item_block for every flagged particle, and
+// for every occupied block its items [block_items[2b], block_items[2b+1]).
 __global__ void __launch_bounds__(256)
     k_item_emit(const uint32_t* __restrict__ keys, int64_t n, int shift,
                 const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
                 const uint32_t* __restrict__ total, uint32_t* __restrict__ item_start,
-                uint32_t* __restrict__ item_block) {
+                uint32_t* __restrict__ item_block, uint32_t* __restrict__ block_items) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && flag[i]) {
-    item_start[pos[i]] = (uint32_t)i;
-    item_block[pos[i]] = keys[i] >> shift;
+  if (i < n) {
+    const uint32_t b = keys[i] >> shift;
+    if (flag[i]) {
+      item_start[pos[i]] = (uint32_t)i;
+      item_block[pos[i]] = b;
+    }
+    if (i == 0 || (keys[i - 1] >> shift) != b) block_items[2 * (size_t)b] = pos[i];
+    if (i == n - 1 || (keys[i + 1] >> shift) != b) block_items[2 * (size_t)b + 1] = pos[i] + flag[i];
   }
   if (i == 0) item_start[*total] = (uint32_t)n;  // sentinel: end of the last item
+}
+
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Arrangement of one item's particles over the 4 wavefronts of the substep
+// kernel, which gives every wavefront a private P2G tile: the particle of rank
+// r within its cell goes to wavefront (r + cell) mod 4, so particles sharing
+// a cell land in different wavefronts as long as a cell holds at most 4 of the
+// item's particles. A wavefront takes at most 64 particles; the excess fills
+// the free lanes of the other wavefronts in order. dest[i] is the new index of
+// sorted particle i (a permutation inside the item); item_waves packs the 4
+// per-wavefront counts (8 bits each). The layout is a function of the sorted
+// keys only.
+__global__ void __launch_bounds__(256)
+    k_item_arrange(const uint32_t* __restrict__ keys, int cell_bits,
+                   const uint32_t* __restrict__ item_start, const uint32_t* __restrict__ count,
+                   uint32_t* __restrict__ dest, uint32_t* __restrict__ item_waves) {
+  const uint32_t item = blockIdx.x;
+  if (item >= *count) return;  // uniform for the workgroup
+  __shared__ uint32_t sk[256];
+  __shared__ int wmax[4];
+  __shared__ unsigned long long wsum[4];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const uint32_t s = item_start[item];
+  const int n = (int)(item_start[item + 1] - s);  // 1..256
+  const bool valid = t < n;
+  const uint32_t key = valid ? keys[s + t] : 0xffffffffu;
+  sk[t] = key;
+  __syncthreads();
+  // rank within the cell: distance to the first particle of its run of equal keys
+  int head = (valid && (t == 0 || sk[t - 1] != key)) ? t : 0;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(head, off);
+    if (lane >= off) head = max(head, y);
+  }
+  if (lane == 63) wmax[wv] = head;
+  // per-wavefront counts as 4 x 16-bit fields, exclusive prefix over the item
+  const int cell = (int)(key & ((1u << cell_bits) - 1u));
+  __syncthreads();
+  for (int q = 0; q < wv; ++q) head = max(head, wmax[q]);
+  const int rank = t - head;
+  const int w = (rank + cell) & 3;
+  const unsigned long long one = valid ? (1ull << (16 * w)) : 0ull;
+  unsigned long long incl = one;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned long long y = __shfl_up(incl, off);
+    if (lane >= off) incl += y;
+  }
+  if (lane == 63) wsum[wv] = incl;
+  __syncthreads();
+  unsigned long long before = 0ull, total = 0ull;
+  for (int q = 0; q < 4; ++q) {
+    if (q < wv) before += wsum[q];
+    total += wsum[q];
+  }
+  const int idx = (int)(((before + incl - one) >> (16 * w)) & 0xffffull);
+  int cnt[4], c[4], fin[4];
+  int over = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    cnt[q] = (int)((total >> (16 * q)) & 0xffffull);
+    c[q] = min(cnt[q], 64);
+    over += cnt[q] - c[q];
+  }
+  // free lanes take the excess in wavefront order
+  int left = over;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int take = min(64 - c[q], left);
+    fin[q] = c[q] + take;
+    left -= take;
+  }
+  int fw = w, fl = idx;
+  if (idx >= 64) {
+    // the o-th excess particle takes the o-th free lane, wavefronts in order
+    int o = idx - 64;
+    for (int q = 0; q < w; ++q) o += cnt[q] - c[q];
+    for (int q = 0; q < 4; ++q) {
+      const int fr = 64 - c[q];
+      if (o < fr) { fw = q; fl = c[q] + o; break; }
+      o -= fr;
+    }
+  }
+  int off = 0;
+  for (int q = 0; q < fw; ++q) off += fin[q];
+  if (valid) dest[s + t] = s + (uint32_t)(off + fl);
+  if (t == 0)
+    item_waves[item] = (uint32_t)fin[0] | ((uint32_t)fin[1] << 8) | ((uint32_t)fin[2] << 16) |
+                       ((uint32_t)fin[3] << 24);
 }
 
 // ---- exclusive scan of u32 (tiles of kTile, then tile sums in one workgroup)
@@ -296,16 +386,18 @@ __global__ void __launch_bounds__(kSortThreads)
   }
 }
 
-// dst[i] = src[perm[i]] for every plane, material and id
+// dst[d] = src[perm[i]] with d = dest[i] (or i) for every plane, material and id
 __global__ void __launch_bounds__(256)
-    k_reorder(ParticlesDev src, ParticlesDev dst, const uint32_t* __restrict__ perm) {
+    k_reorder(ParticlesDev src, ParticlesDev dst, const uint32_t* __restrict__ perm,
+              const uint32_t* __restrict__ dest) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= src.n) return;
   const uint32_t j = perm[i];
+  const uint32_t d = dest ? dest[i] : (uint32_t)i;
 #pragma unroll
-  for (int p = 0; p < NUM_PLANES; ++p) dst.f[p * dst.stride + i] = src.f[p * src.stride + j];
-  dst.mat[i] = src.mat[j];
-  dst.id[i] = src.id[j];
+  for (int p = 0; p < NUM_PLANES; ++p) dst.f[p * dst.stride + d] = src.f[p * src.stride + j];
+  dst.mat[d] = src.mat[j];
+  dst.id[d] = src.id[j];
 }
 
 template <int BITS>
@@ -361,19 +453,32 @@ hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, int64_t n, uint32_t
   return hipGetLastError();
 }
 
-hipError_t build_items(const uint32_t* keys, int64_t n, int lb, int chunk, uint32_t* block_range,
-                       uint32_t* flag, uint32_t* pos, uint32_t* scratch, uint32_t* total,
-                       uint32_t* item_start, uint32_t* item_block, hipStream_t s) {
+hipError_t fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  if (chunk < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t build_items(const uint32_t* keys, int64_t n, int lb, int chunk, int64_t nblk,
+                       uint32_t* block_range, uint32_t* block_items, uint32_t* flag, uint32_t* pos,
+                       uint32_t* scratch, uint32_t* total, uint32_t* item_start,
+                       uint32_t* item_block, uint32_t* item_waves, uint32_t* dest,
+                       int64_t max_items, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (chunk < 1 || chunk > kMaxItemParticles || max_items < 1) return hipErrorInvalidValue;
   const int shift = 3 * lb;
   const unsigned g = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((2 * nblk + 255) / 256)), dim3(256), 0, s,
+                     block_items, 2 * nblk, 0u);
   hipLaunchKernelGGL(k_block_heads, dim3(g), dim3(256), 0, s, keys, n, shift, block_range);
   hipLaunchKernelGGL(k_item_flags, dim3(g), dim3(256), 0, s, keys, n, shift, chunk, block_range, flag);
   hipError_t e = exclusive_scan(flag, pos, n, scratch, total, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_item_emit, dim3(g), dim3(256), 0, s, keys, n, shift, flag, pos, total,
-                     item_start, item_block);
+                     item_start, item_block, block_items);
+  if (dest)
+    hipLaunchKernelGGL(k_item_arrange, dim3((unsigned)max_items), dim3(256), 0, s, keys, shift,
+                       item_start, total, dest, item_waves);
   return hipGetLastError();
 }
 
@@ -390,9 +495,9 @@ hipError_t radix_sort(uint32_t* keys[2], uint32_t* vals[2], int64_t n, int key_b
 }
 
 hipError_t reorder(const ParticlesDev& src, const ParticlesDev& dst, const uint32_t* perm,
-                   hipStream_t s) {
+                   const uint32_t* dest, hipStream_t s) {
   if (src.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_reorder, dim3((src.n + 255) / 256), dim3(256), 0, s, src, dst, perm);
+  hipLaunchKernelGGL(k_reorder, dim3((src.n + 255) / 256), dim3(256), 0, s, src, dst, perm, dest);
   return hipGetLastError();
 }
 

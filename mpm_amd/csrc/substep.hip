@@ -246,31 +246,12 @@ __global__ void __launch_bounds__(256)
   const size_t step = (size_t)gridDim.x * blockDim.x;
   for (size_t t = t0; t < n; t += step) {
     if (clear) clear[t] = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 c = g[t];
+    const float4 c = g[t];
     if (!(c.w > 0.0f)) continue;
-    // sequential divides as mpm.cu:86-88 does (mass becomes 1)
-    c.x = c.x / c.w;
-    c.y = c.y / c.w;
-    c.z = c.z / c.w;
-    c.w = 1.0f;
-    c.y += k.dt * k.gravity;
     const int zi = (int)(t % N);
     const int yi = (int)((t / N) % N);
     const int xi = (int)(t / ((size_t)N * N));
-    bool sticky, floor_;
-    if (k.boundary_mode == MPM_BOUNDARY_CELLS) {
-      const int b = k.bcells;
-      sticky = xi < b || xi >= N - b || yi >= N - b || zi < b || zi >= N - b;
-      floor_ = yi < b;
-    } else {
-      const float b = k.bfrac, hi = 1.0f - k.bfrac, fn = (float)N;
-      const float xf = (float)xi / fn, yf = (float)yi / fn, zf = (float)zi / fn;
-      sticky = xf < b || xf > hi || yf > hi || zf < b || zf > hi;
-      floor_ = yf < b;
-    }
-    if (sticky) { c.x = 0.f; c.y = 0.f; c.z = 0.f; }
-    if (floor_) c.y = fmaxf(0.0f, c.y);
-    g[t] = c;
+    g[t] = grid_node_update(c, xi, yi, zi, k);
   }
 }
 
