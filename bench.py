@@ -6,7 +6,8 @@ the box [0.05,0.45]x[0.05,0.65]x[0.05,0.95], grid N=128, dt=5e-5, weakly
 compressible liquid K=4e4, rho=1000, gravity -9.8 (seeded synthetic input,
 mpm_amd.scenes.dam_break). A step is one substep of the engine (grid update +
 fused G2P/P2G, with the periodic cell sort) over all particles, with the
-particle state resident in HBM. Prints one JSON line:
+particle state resident in HBM. The default run times 2,000 substeps, the
+dam break's length. Prints one JSON line:
 
   value        particle-substeps/s (P / seconds per substep), whole job
   roofline     the dominant kernel's algorithmic bytes per launch divided by
@@ -37,8 +38,10 @@ def alg_bytes(phase: str, P: int, N: int) -> int:
     g2p2g   G2P reads x,F,Jp,mat (53 B) and writes x,v,C,F,Jp (100 B); the
             fused P2G uses the state it just computed, whose algorithmic read
             (101 B) is counted as in the three-phase step: 254 B/particle;
-            grid: gather of the velocities and write of the scatter, 2 x 16 B/node
-    grid_update  read and write of every node: 2 x 16 B/node
+            grid: gather of the velocities and write of the P2G result,
+            2 x 16 B/node
+    grid_update  (the tile reduction with the grid update) read of the P2G
+            result and write of the updated node: 2 x 16 B/node
     """
     if phase == "g2p2g":
         return 254 * P + 32 * N ** 3
@@ -49,7 +52,8 @@ def alg_bytes(phase: str, P: int, N: int) -> int:
 
 def load_traffic(workload: str, kernel: str):
     """HBM bytes per launch of the kernel from the committed rocprofv3 PMC
-    summary (profiles/traffic_<workload>.json), or None."""
+    summary (profiles/traffic_<workload>.json: FETCH_SIZE + WRITE_SIZE of the
+    kernel, each scaled by the copy-kernel calibration), or None."""
     f = ROOT / "profiles" / f"traffic_{workload}.json"
     if not f.exists():
         return None
@@ -80,10 +84,10 @@ def cpu_baseline(scene, steps: int, threads: int):
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="dam_break", choices=["dam_break", "snow_spheres", "elastic_sphere"])
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=80)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -129,7 +133,7 @@ def main() -> int:
     dom = max(phases, key=lambda p: phases[p]["mean_us"] * phases[p]["launches"])
     mean_s = phases[dom]["mean_us"] * 1e-6
     achieved = alg_bytes(dom, P, N) / mean_s / 1e9
-    kernel_name = {"g2p2g": "k_g2p2g_tiled", "grid_update": "k_grid_update"}.get(dom, dom)
+    kernel_name = {"g2p2g": "k_g2p2g_tiled", "grid_update": "k_tile_reduce"}.get(dom, dom)
 
     result = {
         "metric": "particle-substeps/s at 1M particles N=128",

@@ -158,7 +158,19 @@ int mpm_download_grid(mpm_handle h, float* grid, void* stream);
 int mpm_save_state(mpm_handle h, const char* path);
 int mpm_load_state(mpm_handle h, const char* path);
 
-/* ---- tuning knobs (integer options; see the library source for names) ---- */
+/* ---- tuning knobs (integer options; no reference counterpart: the reference
+ *      fixes its launch shapes at compile time, mpm.cu:323-329) ----
+ *   "p2g_mode"   1 (default): cell-sorted particles, LDS-tiled fused kernel,
+ *                tile stores reduced per node block; 0: global FP32 atomics
+ *                per node in particle order (the reference's scheme)
+ *   "fused"      1 (default): G2P of substep n fused with P2G of substep n+1;
+ *                0: the reference's order P2G, grid update, G2P
+ *   "sort_every" re-sort the particles by cell every k substeps (default 8)
+ *   "chunk"      particles per work item, 64..256 in steps of 64 (0 = 256)
+ *   "radix_bits" digit width of the radix sort: 4, 6 or 8 (default 8)
+ *   "block"      home block edge in cells: 4 (0 = default 4)
+ *   "timing"     1: record HIP events around every kernel phase
+ * Results do not depend on the options beyond FP32 summation order. */
 int mpm_set_option(mpm_handle h, const char* name, int64_t value);
 int mpm_get_option(mpm_handle h, const char* name, int64_t* value);
 

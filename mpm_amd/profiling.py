@@ -1,0 +1,124 @@
+"""Profiling helpers for rocprofv3 runs of the engine.
+
+Subcommands (run as ``python -m mpm_amd.profiling <cmd>``):
+
+  run        advance one of the benchmark scenes for --steps substeps after
+             --warmup substeps (the program profiled under rocprofv3)
+  copy       a plain device-to-device copy of --mib MiB, --reps times (the
+             calibration kernel for the FETCH_SIZE / WRITE_SIZE counters)
+  summarize  per-kernel table of a rocprofv3 output: a rocpd database
+             (kernel trace: calls, mean/min/max duration) or counter CSV files
+             (mean value per dispatch of every counter)
+
+The first two need a GPU; `summarize` runs anywhere.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import re
+import sqlite3
+import sys
+from collections import defaultdict
+from pathlib import Path
+
+
+def _clean(name: str) -> str:
+    name = name.replace("(anonymous namespace)::", "")
+    return re.sub(r"\(.*", "", name).strip()
+
+
+def cmd_run(a) -> int:
+    import torch
+    from . import Engine, scenes
+    sc = getattr(scenes, a.workload)()
+    st = sc.state()
+    eng = Engine(sc.params)
+    eng.set_materials(sc.materials)
+    for kv in a.option or []:
+        k, v = kv.split("=")
+        eng.set_option(k, int(v))
+    eng.add_particles(st["x"], st["material"], st["v"], st["C"], st["F"], st["Jp"])
+    eng.upload()
+    eng.advance(a.warmup)
+    torch.cuda.synchronize()
+    eng.advance(a.steps)
+    torch.cuda.synchronize()
+    eng.close()
+    print(f"ran {a.workload}: {sc.n} particles, N={sc.params.N}, {a.warmup}+{a.steps} substeps",
+          flush=True)
+    return 0
+
+
+def cmd_copy(a) -> int:
+    import torch
+    n = a.mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(a.reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"copied {a.mib} MiB x {a.reps}", flush=True)
+    return 0
+
+
+def summarize_db(path: Path) -> str:
+    con = sqlite3.connect(str(path))
+    rows = con.execute("select name, start, end from kernels").fetchall()
+    per = defaultdict(list)
+    for n, s, e in rows:
+        per[_clean(n)].append((e - s) / 1000.0)
+    tot = sum(sum(v) for v in per.values()) or 1.0
+    out = [f"{'kernel':44s} {'calls':>6s} {'total_us':>11s} {'mean_us':>9s} {'min_us':>9s} "
+           f"{'max_us':>9s} {'pct':>6s}"]
+    for n, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+        out.append(f"{n[:44]:44s} {len(v):6d} {sum(v):11.1f} {sum(v) / len(v):9.2f} {min(v):9.2f} "
+                   f"{max(v):9.2f} {100 * sum(v) / tot:6.2f}")
+    return "\n".join(out)
+
+
+def summarize_csv(paths) -> str:
+    per = defaultdict(lambda: defaultdict(list))
+    for p in paths:
+        with open(p) as f:
+            for r in csv.DictReader(f):
+                per[_clean(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    out = []
+    for n, d in sorted(per.items()):
+        out.append(n)
+        for c, v in sorted(d.items()):
+            out.append(f"    {c:28s} {sum(v) / len(v):18.1f}   (mean of {len(v)} dispatches)")
+    return "\n".join(out)
+
+
+def cmd_summarize(a) -> int:
+    paths = [Path(p) for p in a.paths]
+    dbs = [p for p in paths if p.suffix == ".db"]
+    csvs = [p for p in paths if p.suffix == ".csv"]
+    for p in dbs:
+        print(summarize_db(p))
+    if csvs:
+        print(summarize_csv(csvs))
+    return 0
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    r = sub.add_parser("run")
+    r.add_argument("--workload", default="dam_break",
+                   choices=["dam_break", "snow_spheres", "elastic_sphere"])
+    r.add_argument("--steps", type=int, default=20)
+    r.add_argument("--warmup", type=int, default=5)
+    r.add_argument("--option", action="append", help="engine option name=value")
+    c = sub.add_parser("copy")
+    c.add_argument("--mib", type=int, default=512)
+    c.add_argument("--reps", type=int, default=3)
+    s = sub.add_parser("summarize")
+    s.add_argument("paths", nargs="+")
+    a = ap.parse_args(argv)
+    return {"run": cmd_run, "copy": cmd_copy, "summarize": cmd_summarize}[a.cmd](a)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
