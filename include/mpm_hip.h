@@ -21,7 +21,8 @@
  *   - Downloads return particles in the order they were added, whatever order
  *     the engine keeps them in on the device.
  *
- * Handles are not thread-safe across concurrent calls on the same handle;
+ * Calls on one handle are serialised by a mutex of the handle (mpm_last_error
+ * returns a pointer that the next failing call on the handle overwrites);
  * distinct handles are independent.
  */
 #ifndef MPM_HIP_H
@@ -166,7 +167,7 @@ int mpm_load_state(mpm_handle h, const char* path);
  *   "fused"      1 (default): G2P of substep n fused with P2G of substep n+1;
  *                0: the reference's order P2G, grid update, G2P
  *   "sort_every" re-sort the particles by cell every k substeps (default 8)
- *   "chunk"      particles per work item, 64..256 in steps of 64 (0 = 256)
+ *   "chunk"      particles per work item, 64..192 in steps of 64 (0 = 192)
  *   "radix_bits" digit width of the radix sort: 4, 6 or 8 (default 8)
  *   "block"      home block edge in cells: 4 (0 = default 4)
  *   "timing"     1: record HIP events around every kernel phase

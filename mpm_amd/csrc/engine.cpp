@@ -300,6 +300,11 @@ int Sim::clear_staging() {
 int Sim::ensure_capacity(int64_t n) {
   if (n <= cap_ && d_f_[0]) return MPM_OK;
   const int64_t cap = std::max<int64_t>(64, ((n + 63) / 64) * 64);
+  // the tiled kernel addresses the particle planes through one buffer
+  // resource, whose byte range is 32-bit
+  if ((uint64_t)cap * NUM_PLANES * sizeof(float) > 0xffffffffull)
+    return fail(MPM_ERR_CAPACITY, "%lld particles exceed one handle's 4 GiB particle planes",
+                (long long)n);
   for (int b = 0; b < 2; ++b) {
     if (d_f_[b]) (void)hipFree(d_f_[b]);
     if (d_mat_[b]) (void)hipFree(d_mat_[b]);
@@ -773,7 +778,8 @@ int Sim::set_option(const char* name, int64_t v) {
   if (n == "radix_bits") { if (v != 4 && v != 6 && v != 8) return fail(MPM_ERR_INVALID, "radix_bits must be 4, 6 or 8"); opt_.radix_bits = v; return MPM_OK; }
   if (n == "chunk") {
     if (v < 0 || v > mpmk::kMaxItemParticles || v % 64 != 0)
-      return fail(MPM_ERR_INVALID, "chunk must be 0 (auto = 256) or a multiple of 64 up to 256");
+      return fail(MPM_ERR_INVALID, "chunk must be 0 (auto = %d) or a multiple of 64 up to %d",
+                  mpmk::kMaxItemParticles, mpmk::kMaxItemParticles);
     opt_.chunk = v;
     items_valid_ = false;
     return MPM_OK;

@@ -25,7 +25,7 @@ struct Options {
   int64_t p2g_mode = 1;     // "p2g_mode": 0 direct atomics, 1 sorted + LDS tile
   int64_t sort_every = 8;   // "sort_every": re-sort particles every k substeps (0: never)
   int64_t radix_bits = 8;   // "radix_bits": digit width of the radix sort (4, 6 or 8)
-  int64_t chunk = 0;        // "chunk": max particles per work item (64..256, multiple of 64; 0 = 256)
+  int64_t chunk = 0;        // "chunk": max particles per work item (64..192, multiple of 64; 0 = 192)
   int64_t block = 0;        // "block": edge of a home block in cells (4; 0 = default)
   int64_t timing = 0;       // "timing": bracket every kernel with HIP events (per-phase time)
 };

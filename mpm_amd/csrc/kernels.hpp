@@ -31,8 +31,10 @@ hipError_t radix_sort(uint32_t* keys[2], uint32_t* vals[2], int64_t n, int key_b
 hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* scratch,
                           uint32_t* total, hipStream_t s);
 hipError_t fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
-// An item holds at most 256 particles (one per thread of its workgroup).
-constexpr int kMaxItemParticles = 256;
+// A work item runs on one workgroup of kItemWaves wavefronts, one particle
+// per thread: at most 64 kItemWaves particles.
+constexpr int kItemWaves = 3;
+constexpr int kMaxItemParticles = 64 * kItemWaves;
 // Home blocks of the tiled substep: 4^3 cells; a work item's tile spans the
 // block, the 2-node reach of its stencil and a 1-cell drift margin on each
 // side: kTileW = 8 nodes per axis.

@@ -2,9 +2,9 @@
 // reduction that turns the P2G tiles into the updated grid (gfx950).
 //
 // The particles are cell-sorted (sort.hip) and cut into work items: at most
-// 256 particles of one home block of 4^3 cells, arranged over the 4
-// wavefronts of a workgroup so that particles sharing a cell sit in different
-// wavefronts (k_item_arrange). One workgroup runs one item:
+// 64 kItemWaves particles of one home block of 4^3 cells, arranged over the
+// kItemWaves wavefronts of a workgroup so that particles sharing a cell sit in
+// different wavefronts (k_item_arrange). One workgroup runs one item:
 //   1. stage the grid velocities of the item's node region in LDS: the block,
 //      its 2-node stencil reach and a 1-cell drift margin on each side
 //      (8^3 nodes);
@@ -18,7 +18,7 @@
 //      accesses). Within one instruction the lanes of a wavefront write 27
 //      distinct nodes when their cells are distinct; lanes that share a cell
 //      take turns (a per-wavefront claim byte per cell decides who goes);
-//   4. store the sum of the 4 wavefront tiles as the item's tile in HBM
+//   4. store the sum of the wavefront tiles as the item's tile in HBM
 //      (plain coalesced stores, no atomics).
 // k_tile_reduce then builds every grid node from the tiles that cover it
 // (the items of the 27 home blocks around its node block), in a fixed order,
@@ -44,36 +44,56 @@ struct PState {
   float x[3], v[3], C[9], F[9], Jp;
 };
 
-__device__ __forceinline__ void load_state(const ParticlesDev& P, int i, PState& s) {
-  const float* f = P.f;
-  const size_t st = P.stride;
+// The particle planes through one buffer resource: the lane's byte offset in
+// a VGPR, the plane's offset in an SGPR, so the 25 loads and stores of a
+// particle need no per-plane 64-bit address arithmetic. The planes must span
+// less than 4 GiB (the engine checks its capacity against that).
+struct PlaneIO {
+  __amdgpu_buffer_rsrc_t rsrc;
+  uint32_t plane_bytes;
+  __device__ __forceinline__ explicit PlaneIO(const ParticlesDev& P)
+      : rsrc(__builtin_amdgcn_make_buffer_rsrc(
+            (void*)P.f, (short)0, (int)(uint32_t)((size_t)NUM_PLANES * P.stride * sizeof(float)),
+            0x00020000 /* gfx9 buffer format: 32-bit data */)),
+        plane_bytes((uint32_t)(P.stride * sizeof(float))) {}
+  __device__ __forceinline__ float ld(int plane, uint32_t voff) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)voff,
+                                                                          (int)(plane * plane_bytes), 0));
+  }
+  __device__ __forceinline__ void st(int plane, uint32_t voff, float v) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), rsrc, (int)voff,
+                                          (int)(plane * plane_bytes), 0);
+  }
+};
+
+__device__ __forceinline__ void load_state(const PlaneIO& io, uint32_t i, PState& s) {
+  const uint32_t o = i * 4u;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    s.x[a] = f[(PL_X + a) * st + i];
-    s.v[a] = f[(PL_V + a) * st + i];
+    s.x[a] = io.ld(PL_X + a, o);
+    s.v[a] = io.ld(PL_V + a, o);
   }
 #pragma unroll
   for (int a = 0; a < 9; ++a) {
-    s.C[a] = f[(PL_C + a) * st + i];
-    s.F[a] = f[(PL_F + a) * st + i];
+    s.C[a] = io.ld(PL_C + a, o);
+    s.F[a] = io.ld(PL_F + a, o);
   }
-  s.Jp = f[PL_JP * st + i];
+  s.Jp = io.ld(PL_JP, o);
 }
 
-__device__ __forceinline__ void store_state(const ParticlesDev& P, int i, const PState& s) {
-  float* f = P.f;
-  const size_t st = P.stride;
+__device__ __forceinline__ void store_state(const PlaneIO& io, uint32_t i, const PState& s) {
+  const uint32_t o = i * 4u;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    f[(PL_X + a) * st + i] = s.x[a];
-    f[(PL_V + a) * st + i] = s.v[a];
+    io.st(PL_X + a, o, s.x[a]);
+    io.st(PL_V + a, o, s.v[a]);
   }
 #pragma unroll
   for (int a = 0; a < 9; ++a) {
-    f[(PL_C + a) * st + i] = s.C[a];
-    f[(PL_F + a) * st + i] = s.F[a];
+    io.st(PL_C + a, o, s.C[a]);
+    io.st(PL_F + a, o, s.F[a]);
   }
-  f[PL_JP * st + i] = s.Jp;
+  io.st(PL_JP, o, s.Jp);
 }
 
 __device__ __forceinline__ void gather(const PState& s, const int base[3], const float w[3][3],
@@ -168,8 +188,37 @@ __device__ __forceinline__ void scatter_global(const float q0[3][3], const float
 // Material-kind sets compiled into an instantiation.
 enum KindSet : int { KINDS_ALL = 0, KINDS_LIQUID = 1 };
 
+// LDS slot of tile node (x, y, z) in an 8^3 tile: z is permuted within its
+// row (bit 2 flipped when x + y is odd), so the runs of 4 neighbouring nodes
+// that the lanes of one access touch in adjacent rows or planes land in
+// different halves of the 32 LDS banks instead of the same ones.
+__device__ __forceinline__ int tile_slot(int x, int y, int z) {
+  return (x * mpmk::kTileW + y) * mpmk::kTileW + (z ^ (((x ^ y) & 1) << 2));
+}
+
+// The 27 stencil slots of a cell at tile coordinates (l0, l1, l2):
+// slot(a, b, c) = row + (a W + b) W + z[(a ^ b) & 1][c].
+struct StencilSlots {
+  int row;
+  int z[2][3];
+  __device__ __forceinline__ StencilSlots(int l0, int l1, int l2) {
+    constexpr int W = mpmk::kTileW;
+    row = (l0 * W + l1) * W;
+    const int f = (l0 ^ l1) & 1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      z[0][c] = (l2 + c) ^ (f << 2);
+      z[1][c] = (l2 + c) ^ ((f ^ 1) << 2);
+    }
+  }
+  __device__ __forceinline__ int operator()(int a, int b, int c) const {
+    constexpr int W = mpmk::kTileW;
+    return row + (a * W + b) * W + z[(a ^ b) & 1][c];
+  }
+};
+
 template <int KS>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64 * mpmk::kItemWaves)
     k_g2p2g_tiled(ParticlesDev P, const MaterialDev* __restrict__ mats, StepConsts k,
                   const uint32_t* __restrict__ item_start, const uint32_t* __restrict__ item_block,
                   const uint32_t* __restrict__ item_waves, const uint32_t* __restrict__ item_count,
@@ -179,9 +228,13 @@ __global__ void __launch_bounds__(256)
   constexpr int W = mpmk::kTileW;        // BE + 2 (stencil reach) + 2 (drift margin)
   constexpr int TN = mpmk::kTileNodes;
   constexpr int NCE = W - 2;             // cells per axis whose stencil fits the tile
+  constexpr int NW = mpmk::kItemWaves, NT = 64 * NW;
   static_assert(NCE * NCE * NCE <= 256, "claim bytes per wavefront");
-  __shared__ float4 vt[TN];              // grid velocities; claim bytes once G2P is done
-  __shared__ float4 acc[4][TN];          // P2G tile of each wavefront
+  __shared__ float4 acc[NW][TN];         // P2G tile of each wavefront
+  __shared__ uint8_t claims[NW][256];    // claim byte per cell, per wavefront
+  // The grid velocities for G2P share wavefront 0's tile: wavefront 0 clears
+  // it after the barrier that ends every G2P read.
+  float4* const vt = acc[0];
 
   const int item = blockIdx.x;
   if ((uint32_t)item >= *item_count) return;  // uniform for the workgroup
@@ -198,18 +251,18 @@ __global__ void __launch_bounds__(256)
   const int o2 = (int)compact3(blk) * BE - 1;
 
   if (do_g2p) {
-    for (int n = tid; n < TN; n += 256) {
+    for (int n = tid; n < TN; n += NT) {
       const int a = n / (W * W), r = n - a * W * W, b = r / W, c = r - b * W;
       const int gi = o0 + a, gj = o1 + b, gk = o2 + c;
       float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
       if (gi >= 0 && gi < N && gj >= 0 && gj < N && gk >= 0 && gk < N)
         v = grid_v[((size_t)gi * N + (size_t)gj) * N + (size_t)gk];
-      vt[n] = v;
+      vt[tile_slot(a, b, c)] = v;
     }
   }
   {
-    float4* const flat = &acc[0][0];
-    for (int n = tid; n < 4 * TN; n += 256) flat[n] = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4* const flat = &acc[1][0];
+    for (int n = tid; n < (NW - 1) * TN; n += NT) flat[n] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   __syncthreads();
 
@@ -222,40 +275,61 @@ __global__ void __launch_bounds__(256)
   int base[3] = {0, 0, 0};
   float w[3][3];
   bool inside = false;
+  const PlaneIO io(P);
   if (valid) {
-    load_state(P, (int)i, s);
+    load_state(io, i, s);
     m = mats[P.mat[i]];
     quad_weights(s.x, k.dx_inv, base, w);
     if (do_g2p && P.id[i] <= k.guard && !outside_domain(base, N)) {
       // ---- G2P of substep n, gathering the 27 nodes from the LDS tile
       const int l0 = base[0] - o0, l1 = base[1] - o1, l2 = base[2] - o2;
       if (l0 >= 0 && l1 >= 0 && l2 >= 0 && l0 < NCE && l1 < NCE && l2 < NCE) {
-        float dd[3][3];
+        // v = sum w g, C = sum w g dd^T (g2p_node_contribution, TransferScheme.h:124-132),
+        // summed axis by axis: c innermost, then b, then a
+        float wd[3][3];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
-          for (int o = 0; o < 3; ++o) dd[a][o] = ((float)(uint32_t)(base[a] + o) * k.dx - s.x[a]) * k.dinv;
+          for (int o = 0; o < 3; ++o)
+            wd[a][o] = w[a][o] * (((float)(uint32_t)(base[a] + o) * k.dx - s.x[a]) * k.dinv);
+        const StencilSlots slot(l0, l1, l2);
+        float V[3][3], X[3][3], Y[3][3];  // [a][component]
 #pragma unroll
-        for (int a = 0; a < 3; ++a) s.v[a] = 0.0f;
+        for (int a = 0; a < 3; ++a) {
 #pragma unroll
-        for (int a = 0; a < 9; ++a) s.C[a] = 0.0f;
-        const int nb = (l0 * W + l1) * W + l2;
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
+          for (int i = 0; i < 3; ++i) { V[a][i] = 0.f; X[a][i] = 0.f; Y[a][i] = 0.f; }
 #pragma unroll
           for (int b = 0; b < 3; ++b) {
-            const float wab = w[0][a] * w[1][b];
+            float T[3] = {0.f, 0.f, 0.f}, U[3] = {0.f, 0.f, 0.f};
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-              const float4 g = vt[nb + (a * W + b) * W + c];
-              const float wt = wab * w[2][c];
-              const float wv0 = wt * g.x, wv1 = wt * g.y, wv2 = wt * g.z;
-              s.v[0] += wv0; s.v[1] += wv1; s.v[2] += wv2;
-              s.C[0] += wv0 * dd[0][a]; s.C[1] += wv0 * dd[1][b]; s.C[2] += wv0 * dd[2][c];
-              s.C[3] += wv1 * dd[0][a]; s.C[4] += wv1 * dd[1][b]; s.C[5] += wv1 * dd[2][c];
-              s.C[6] += wv2 * dd[0][a]; s.C[7] += wv2 * dd[1][b]; s.C[8] += wv2 * dd[2][c];
+              const float4 g = vt[slot(a, b, c)];
+              T[0] += w[2][c] * g.x; T[1] += w[2][c] * g.y; T[2] += w[2][c] * g.z;
+              U[0] += wd[2][c] * g.x; U[1] += wd[2][c] * g.y; U[2] += wd[2][c] * g.z;
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+              V[a][i] += w[1][b] * T[i];
+              X[a][i] += wd[1][b] * T[i];
+              Y[a][i] += w[1][b] * U[i];
             }
           }
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          float vi = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            vi += w[0][a] * V[a][i];
+            c0 += wd[0][a] * V[a][i];
+            c1 += w[0][a] * X[a][i];
+            c2 += w[0][a] * Y[a][i];
+          }
+          s.v[i] = vi;
+          s.C[3 * i + 0] = c0;
+          s.C[3 * i + 1] = c1;
+          s.C[3 * i + 2] = c2;
+        }
       } else {
         gather(s, base, w, k, grid_v, s.v, s.C);
       }
@@ -275,16 +349,17 @@ __global__ void __launch_bounds__(256)
       }
 #pragma unroll
       for (int a = 0; a < 3; ++a) s.x[a] = s.x[a] + k.dt * s.v[a];
-      store_state(P, (int)i, s);
+      store_state(io, i, s);
       quad_weights(s.x, k.dx_inv, base, w);
     }
     inside = !outside_domain(base, N);
   }
 
   // ---- P2G preparation (TransferScheme.h:66-86): stress and affine momentum.
-  // Node value (a,b,c) = wt * (q0[.][a] + (q1[.][b] + q2[.][c])), with
-  // q0 = m v + aff(:,0) d0, q1 = aff(:,1) d1, q2 = aff(:,2) d2.
-  int cell = 0, nb = 0;
+  // Node value (a,b,c) = wt * ((q0[.][a] + q1[.][b]) + q2[.][c]), with
+  // q0 = m v + aff(:,0) d0, q1 = aff(:,1) d1, q2 = aff(:,2) d2
+  // (p2g_node_contribution, TransferScheme.h:88-100, regrouped by axis).
+  int cell = 0, l0 = 0, l1 = 0, l2 = 0;
   bool pending = false;
   float q0[3][3], q1[3][3], q2[3][3];
   if (inside) {
@@ -313,7 +388,9 @@ __global__ void __launch_bounds__(256)
         q1[c][o] = aff[3 * c + 1] * d[1][o];
         q2[c][o] = aff[3 * c + 2] * d[2][o];
       }
-    const int l0 = base[0] - o0, l1 = base[1] - o1, l2 = base[2] - o2;
+    l0 = base[0] - o0;
+    l1 = base[1] - o1;
+    l2 = base[2] - o2;
     // tile path: the stencil lies in the tile and inside the grid (at the grid
     // faces the reference's node bounds and aliasing rules apply: direct path)
     const bool in_tile = l0 >= 0 && l1 >= 0 && l2 >= 0 && l0 < NCE && l1 < NCE && l2 < NCE &&
@@ -321,55 +398,126 @@ __global__ void __launch_bounds__(256)
                          base[1] + 2 < N && base[2] + 2 < N;
     if (in_tile) {
       cell = (l0 * NCE + l1) * NCE + l2;
-      nb = (l0 * W + l1) * W + l2;
       pending = true;
     } else {
       scatter_global(q0, q1, q2, m.mass, base, w, k, grid_atomic, dirty);
     }
   }
-  __syncthreads();  // every G2P read of vt is done: its bytes become claim bytes
+  __syncthreads();  // every G2P read of vt is done
+  if (wave == 0) {
+#pragma unroll
+    for (int n = lane; n < TN; n += 64) acc[0][n] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
 
-  // ---- P2G into the wavefront's tile. Per round, every pending lane claims
-  // its cell; the lane whose claim stands adds its 27 nodes. The claimed
-  // cells are distinct, so the 64 lanes of one read-modify-write address
-  // distinct nodes; the compiler barrier keeps the 27 read-modify-writes in
-  // program order (a lane's next read may address a node another lane is
-  // writing in the same round).
+  // ---- P2G into the wavefront's tile.
+  // Per slab of 9 nodes (x offset a):
+  // 1. the lane's 36 node contributions (9 nodes x momentum, mass);
+  // 2. lanes holding the same cell form contiguous runs (the particles are in
+  //    cell order within a wavefront); a segmented inclusive scan over the
+  //    wavefront with DPP lane shifts leaves each run's total in its last
+  //    lane: shifts of 1, 2, 4, 8 within rows of 16 and the two
+  //    row-broadcast steps that carry sums across rows. Done only when some
+  //    run is longer than 4; shorter runs go through step 3 lane by lane.
+  // 3. the writers (run tails after a reduction, else every lane with a tile
+  //    particle) add their 9 nodes with plain read-modify-writes, in rounds:
+  //    every round a writer claims its cell (one byte per cell) and only the
+  //    lane whose claim stands writes (particles that moved since the last
+  //    sort can also put one cell in two runs). Within
+  //    one read-modify-write the writing lanes address distinct nodes; the
+  //    compiler barrier keeps the read-modify-writes in program order (a
+  //    lane's next read may address a node another lane writes this round).
+  // runs of equal cells: lanes without a tile particle are runs of their own
+  const int cid = pending ? cell : 1024 + lane;
+  const int prev = __shfl_up(cid, 1), next = __shfl_down(cid, 1);
+  int head = (lane == 0 || prev != cid) ? lane : 0;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(head, off);
+    if (lane >= off) head = max(head, y);
+  }
+  const int rank = lane - head;  // position within the run
+  const bool tail = lane == 63 || next != cid;
+  int maxrun = rank + 1;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) maxrun = max(maxrun, __shfl_xor(maxrun, off));
+  const int lr = lane & 15;
+
   typedef __attribute__((address_space(3))) volatile uint8_t lds_byte;
-  lds_byte* const claim = (lds_byte*)(reinterpret_cast<uint8_t*>(vt) + wave * 256);
+  lds_byte* const claim = (lds_byte*)(&claims[wave][0]);
   float4* const my = acc[wave];
-  while (__ballot(pending) != 0ull) {
-    if (pending) claim[cell] = (uint8_t)lane;
-    if (pending && claim[cell] == (uint8_t)lane) {
+  const StencilSlots slot(l0, l1, l2);
+  // one slab of 9 nodes (offset a along x) at a time keeps 36 sums in registers
 #pragma unroll
-      for (int a = 0; a < 3; ++a)
+  for (int a = 0; a < 3; ++a) {
+    float cv[9][4];
 #pragma unroll
-        for (int b = 0; b < 3; ++b) {
-          const float wab = w[0][a] * w[1][b];
+    for (int b = 0; b < 3; ++b) {
+      const float wab = w[0][a] * w[1][b];
+      const float s0 = q0[0][a] + q1[0][b], s1 = q0[1][a] + q1[1][b], s2 = q0[2][a] + q1[2][b];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int o = b * 3 + c;
+        const float wt = pending ? wab * w[2][c] : 0.0f;
+        cv[o][0] = wt * (s0 + q2[0][c]);
+        cv[o][1] = wt * (s1 + q2[1][c]);
+        cv[o][2] = wt * (s2 + q2[2][c]);
+        cv[o][3] = wt * m.mass;
+      }
+    }
+#define MPM_SEG_STEP(CTRL, ROWS, COND)                                                      \
+    {                                                                                      \
+      const bool m_ = (COND);                                                              \
+      _Pragma("unroll") for (int o = 0; o < 9; ++o) _Pragma("unroll") for (int q = 0; q < 4; ++q) { \
+        const float t_ = __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(            \
+            0, __builtin_bit_cast(int, cv[o][q]), CTRL, ROWS, 0xf, true));                 \
+        cv[o][q] = m_ ? cv[o][q] + t_ : cv[o][q];                                          \
+      }                                                                                    \
+    }
+    if (maxrun > 4) {
+      MPM_SEG_STEP(0x111 /* row_shr:1 */, 0xf, rank >= 1 && lr >= 1)
+      MPM_SEG_STEP(0x112 /* row_shr:2 */, 0xf, rank >= 2 && lr >= 2)
+      if (maxrun > 4) MPM_SEG_STEP(0x114 /* row_shr:4 */, 0xf, rank >= 4 && lr >= 4)
+      if (maxrun > 8) MPM_SEG_STEP(0x118 /* row_shr:8 */, 0xf, rank >= 8 && lr >= 8)
+      // carry across rows: rows 1, 3 take lane 15 of the row below, rows 2, 3 lane 31
+      MPM_SEG_STEP(0x142 /* row_bcast:15 */, 0xa, ((lane >> 4) & 1) && rank > lr)
+      MPM_SEG_STEP(0x143 /* row_bcast:31 */, 0xc, lane >= 32 && rank > lane - 32)
+    }
+#undef MPM_SEG_STEP
+    bool todo = pending && (maxrun <= 4 || tail);
+    while (__ballot(todo) != 0ull) {
+      if (todo) claim[cell] = (uint8_t)lane;
+      if (todo && claim[cell] == (uint8_t)lane) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            const float wt = wab * w[2][c];
-            float4* const p = my + nb + (a * W + b) * W + c;
+            const int o = b * 3 + c;
+            float4* const p = my + slot(a, b, c);
             float4 t = *p;
-            t.x += wt * (q0[0][a] + (q1[0][b] + q2[0][c]));
-            t.y += wt * (q0[1][a] + (q1[1][b] + q2[1][c]));
-            t.z += wt * (q0[2][a] + (q1[2][b] + q2[2][c]));
-            t.w += wt * m.mass;
+            t.x += cv[o][0];
+            t.y += cv[o][1];
+            t.z += cv[o][2];
+            t.w += cv[o][3];
             *p = t;
             __asm__ __volatile__("" ::: "memory");
           }
-        }
-      pending = false;
+        todo = false;
+      }
     }
   }
   __syncthreads();
 
   // ---- the item's tile: sum of the wavefront tiles, fixed order
   float4* const out = tiles + (size_t)item * TN;
-  for (int n = tid; n < TN; n += 256) {
-    const float4 a = acc[0][n], b = acc[1][n], c = acc[2][n], e = acc[3][n];
-    out[n] = make_float4(((a.x + b.x) + c.x) + e.x, ((a.y + b.y) + c.y) + e.y,
-                         ((a.z + b.z) + c.z) + e.z, ((a.w + b.w) + c.w) + e.w);
+  for (int n = tid; n < TN; n += NT) {
+    const int sl = tile_slot(n / (W * W), (n / W) % W, n % W);
+    float4 t = acc[0][sl];
+#pragma unroll
+    for (int q = 1; q < NW; ++q) {
+      const float4 u = acc[q][sl];
+      t.x += u.x; t.y += u.y; t.z += u.z; t.w += u.w;
+    }
+    out[n] = t;
   }
 }
 
@@ -456,7 +604,8 @@ template <int KS>
 static void launch_tiled(const ParticlesDev& P, const MaterialDev* mats, const StepConsts& k,
                          const WorkItems& w, const float4* gv, const TileSums& o, int do_g2p,
                          hipStream_t s) {
-  hipLaunchKernelGGL((k_g2p2g_tiled<KS>), dim3((unsigned)w.max_items), dim3(256), 0, s, P, mats, k,
+  hipLaunchKernelGGL((k_g2p2g_tiled<KS>), dim3((unsigned)w.max_items), dim3(64 * kItemWaves), 0, s,
+                     P, mats, k,
                      w.start, w.block, w.waves, w.count, gv, o.tiles, (float*)o.atomic, o.dirty,
                      do_g2p);
 }

@@ -114,15 +114,16 @@ __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, int6
   if (i < n) p[i] = v;
 }
 
-// Arrangement of one item's particles over the 4 wavefronts of the substep
-// kernel, which gives every wavefront a private P2G tile: the particle of rank
-// r within its cell goes to wavefront (r + cell) mod 4, so particles sharing
-// a cell land in different wavefronts as long as a cell holds at most 4 of the
-// item's particles. A wavefront takes at most 64 particles; the excess fills
-// the free lanes of the other wavefronts in order. dest[i] is the new index of
+// Arrangement of one item's n particles over the first nw = ceil(n / 64) of
+// the kItemWaves wavefronts of the substep kernel, which gives every wavefront a
+// private P2G tile: the particle of rank r within its cell goes to wavefront
+// (r + cell) mod nw, so particles sharing a cell land in different wavefronts
+// as long as a cell holds at most nw of the item's particles. A wavefront
+// takes at most 64 particles; the excess fills the free lanes of the other
+// wavefronts in order (the first nw hold enough). dest[i] is the new index of
 // sorted particle i (a permutation inside the item); item_waves packs the 4
 // per-wavefront counts (8 bits each). The layout is a function of the sorted
-// keys only.
+// keys only. One workgroup of 256 threads per item (n <= kMaxItemParticles).
 __global__ void __launch_bounds__(256)
     k_item_arrange(const uint32_t* __restrict__ keys, int cell_bits,
                    const uint32_t* __restrict__ item_start, const uint32_t* __restrict__ count,
@@ -134,7 +135,7 @@ __global__ void __launch_bounds__(256)
   __shared__ unsigned long long wsum[4];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const uint32_t s = item_start[item];
-  const int n = (int)(item_start[item + 1] - s);  // 1..256
+  const int n = (int)(item_start[item + 1] - s);  // 1..kMaxItemParticles
   const bool valid = t < n;
   const uint32_t key = valid ? keys[s + t] : 0xffffffffu;
   sk[t] = key;
@@ -152,7 +153,8 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
   for (int q = 0; q < wv; ++q) head = max(head, wmax[q]);
   const int rank = t - head;
-  const int w = (rank + cell) & 3;
+  const int nw = (n + 63) >> 6;  // wavefronts the item needs
+  const int w = (rank + cell) % nw;
   const unsigned long long one = valid ? (1ull << (16 * w)) : 0ull;
   unsigned long long incl = one;
 #pragma unroll
@@ -163,15 +165,16 @@ __global__ void __launch_bounds__(256)
   if (lane == 63) wsum[wv] = incl;
   __syncthreads();
   unsigned long long before = 0ull, total = 0ull;
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < 4; ++q) {  // the 4 wavefronts of this workgroup
     if (q < wv) before += wsum[q];
     total += wsum[q];
   }
   const int idx = (int)(((before + incl - one) >> (16 * w)) & 0xffffull);
-  int cnt[4], c[4], fin[4];
+  constexpr int NW = mpmk::kItemWaves;
+  int cnt[NW], c[NW], fin[NW];
   int over = 0;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < NW; ++q) {
     cnt[q] = (int)((total >> (16 * q)) & 0xffffull);
     c[q] = min(cnt[q], 64);
     over += cnt[q] - c[q];
@@ -179,7 +182,7 @@ __global__ void __launch_bounds__(256)
   // free lanes take the excess in wavefront order
   int left = over;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < NW; ++q) {
     const int take = min(64 - c[q], left);
     fin[q] = c[q] + take;
     left -= take;
@@ -189,7 +192,7 @@ __global__ void __launch_bounds__(256)
     // the o-th excess particle takes the o-th free lane, wavefronts in order
     int o = idx - 64;
     for (int q = 0; q < w; ++q) o += cnt[q] - c[q];
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NW; ++q) {
       const int fr = 64 - c[q];
       if (o < fr) { fw = q; fl = c[q] + o; break; }
       o -= fr;
@@ -198,9 +201,12 @@ __global__ void __launch_bounds__(256)
   int off = 0;
   for (int q = 0; q < fw; ++q) off += fin[q];
   if (valid) dest[s + t] = s + (uint32_t)(off + fl);
-  if (t == 0)
-    item_waves[item] = (uint32_t)fin[0] | ((uint32_t)fin[1] << 8) | ((uint32_t)fin[2] << 16) |
-                       ((uint32_t)fin[3] << 24);
+  if (t == 0) {
+    uint32_t packed = 0;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) packed |= (uint32_t)fin[q] << (8 * q);
+    item_waves[item] = packed;
+  }
 }
 
 // ---- exclusive scan of u32 (tiles of kTile, then tile sums in one workgroup)
