@@ -188,9 +188,10 @@ int mpm_timing_reset(mpm_handle h);
  * grid and copy the raw grid (momentum, mass: no normalisation) to host
  * (the work of particleToGrid, mpm.cu:14-74). Does not change particles. */
 int mpm_debug_p2g(mpm_handle h, float* grid, void* stream);
-/* Stable cell-key sort of the current particle positions: writes the keys
- * and the permutation (original indices in sorted order) that the engine
- * uses to reorder its particles. keys/perm hold n entries. */
+/* Stable cell-key sort of the current particle positions with the engine's
+ * radix-sort kernels: writes the keys and the permutation (original indices
+ * in sorted order). The engine itself sorts by home block with the same
+ * kernels and orders each work item by cell. keys/perm hold n entries. */
 int mpm_debug_sort(mpm_handle h, uint32_t* keys, uint32_t* perm, void* stream);
 /* 3x3 SVD of n row-major matrices on the device (the McAdams routine the
  * reference vendors, include/svd3_cuda.h:35-1043): U, V rotations, sigma

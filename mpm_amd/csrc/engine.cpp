@@ -88,7 +88,7 @@ Sim::~Sim() {
   }
   if (d_hist_) (void)hipFree(d_hist_);
   if (d_mats_) (void)hipFree(d_mats_);
-  for (uint32_t* p : {d_block_start_, d_block_items_, d_flag_, d_pos_, d_dest_, d_item_start_,
+  for (uint32_t* p : {d_block_start_, d_block_items_, d_flag_, d_pos_, d_gather_, d_item_start_,
                       d_item_block_, d_item_waves_, d_item_count_, d_dirty_})
     if (p) (void)hipFree(p);
   if (d_tiles_) (void)hipFree(d_tiles_);
@@ -170,6 +170,9 @@ int Sim::hip_fail(hipError_t e, const char* what) {
 
 int Sim::init_device() {
   HIPCHK(hipGetDevice(&device_), "hipGetDevice");
+  HIPCHK(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, device_),
+         "hipDeviceGetAttribute(CUs)");
+  if (cus_ < 1) cus_ = 1;
   HIPCHK(hipMalloc(&d_mats_, sizeof(MaterialDev) * MPM_MAX_MATERIALS), "hipMalloc(materials)");
   return ensure_grids();
 }
@@ -321,13 +324,13 @@ int Sim::ensure_capacity(int64_t n) {
     HIPCHK(hipMalloc(&d_keys_[b], sizeof(uint32_t) * cap), "hipMalloc(sort keys)");
     HIPCHK(hipMalloc(&d_vals_[b], sizeof(uint32_t) * cap), "hipMalloc(sort values)");
   }
-  for (uint32_t** p : {&d_flag_, &d_pos_, &d_dest_, &d_item_start_, &d_item_block_, &d_item_waves_}) {
+  for (uint32_t** p : {&d_flag_, &d_pos_, &d_gather_, &d_item_start_, &d_item_block_, &d_item_waves_}) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }
   HIPCHK(hipMalloc(&d_flag_, sizeof(uint32_t) * cap), "hipMalloc(item flags)");
   HIPCHK(hipMalloc(&d_pos_, sizeof(uint32_t) * cap), "hipMalloc(item positions)");
-  HIPCHK(hipMalloc(&d_dest_, sizeof(uint32_t) * cap), "hipMalloc(item arrangement)");
+  HIPCHK(hipMalloc(&d_gather_, sizeof(uint32_t) * cap), "hipMalloc(item arrangement)");
   HIPCHK(hipMalloc(&d_item_start_, sizeof(uint32_t) * (cap + 1)), "hipMalloc(item starts)");
   HIPCHK(hipMalloc(&d_item_block_, sizeof(uint32_t) * cap), "hipMalloc(item blocks)");
   HIPCHK(hipMalloc(&d_item_waves_, sizeof(uint32_t) * cap), "hipMalloc(item waves)");
@@ -391,6 +394,7 @@ int Sim::upload(hipStream_t s) {
   P_ = n;
   uploaded_ = true;
   items_valid_ = false;
+  pending_gather_ = false;
   primed_ = false;
   vel_grid_ = nullptr;
   steps_since_sort_ = 1 << 30;
@@ -426,27 +430,45 @@ int Sim::sort_particles(hipStream_t s) {
   }
   const ParticlesDev src = particles();
   int out = 0;
-  ParticlesDev dst = src;
-  dst.f = d_f_[cur_ ^ 1];
-  dst.mat = d_mat_[cur_ ^ 1];
-  dst.id = d_id_[cur_ ^ 1];
+  PHASE(PH_SORT, {
+    HIPCHK(mpmk::cell_keys(src, k_, lb_, d_keys_[0], d_vals_[0], s), "cell keys");
+    // by home block only: k_item_arrange orders each item by cell
+    HIPCHK(mpmk::radix_sort(d_keys_, d_vals_, P_, 3 * lb_, mpmk::cell_key_bits(k_.N, lb_),
+                            (int)opt_.radix_bits, d_hist_, &out, s),
+           "radix sort");
+    HIPCHK(mpmk::build_items(d_keys_[out], d_vals_[out], P_, lb_, chunk_, nblk_, d_block_start_,
+                             d_block_items_, d_flag_, d_pos_, d_hist_, d_item_count_, d_item_start_,
+                             d_item_block_, d_item_waves_, d_gather_, items, s),
+           "work items");
+  });
+  // the particles move to their new slots in the next tiled pass
+  pending_gather_ = true;
+  steps_since_sort_ = 0;
+  items_valid_ = true;
+  return MPM_OK;
+}
+
+int Sim::run_tiled(const float4* grid_v, int do_g2p, hipStream_t s) {
+  // the liquid-only instantiation when every material is WC_LIQUID
   kinds_ = 1;
   for (const auto& m : mats_)
     if (m.kind != MPM_MATERIAL_WC_LIQUID) kinds_ = 0;
-  PHASE(PH_SORT, {
-    HIPCHK(mpmk::cell_keys(src, k_, lb_, d_keys_[0], d_vals_[0], s), "cell keys");
-    HIPCHK(mpmk::radix_sort(d_keys_, d_vals_, P_, mpmk::cell_key_bits(k_.N, lb_), (int)opt_.radix_bits,
-                            d_hist_, &out, s),
-           "radix sort");
-    HIPCHK(mpmk::build_items(d_keys_[out], P_, lb_, chunk_, nblk_, d_block_start_, d_block_items_,
-                             d_flag_, d_pos_, d_hist_, d_item_count_, d_item_start_, d_item_block_,
-                             d_item_waves_, d_dest_, items, s),
-           "work items");
-    HIPCHK(mpmk::reorder(src, dst, d_vals_[out], d_dest_, s), "reorder");
-  });
-  cur_ ^= 1;
-  steps_since_sort_ = 0;
-  items_valid_ = true;
+  const ParticlesDev P = particles();
+  ParticlesDev D = P;
+  const uint32_t* g = nullptr;
+  if (pending_gather_) {
+    D.f = d_f_[cur_ ^ 1];
+    D.mat = d_mat_[cur_ ^ 1];
+    D.id = d_id_[cur_ ^ 1];
+    g = d_gather_;
+  }
+  HIPCHK(mpmk::g2p2g_block(P, D, g, d_mats_, (int)mats_.size(), k_, work_items(), grid_v, tile_sums(),
+                           do_g2p, kinds_, s),
+         do_g2p ? "g2p2g" : "p2g");
+  if (pending_gather_) {
+    cur_ ^= 1;
+    pending_gather_ = false;
+  }
   return MPM_OK;
 }
 
@@ -462,6 +484,8 @@ mpmk::WorkItems Sim::work_items() const {
   w.waves = d_item_waves_;
   w.count = d_item_count_;
   w.max_items = max_items();
+  // one workgroup per possible item; those beyond the item count exit at once
+  w.launch = std::max<int64_t>(1, w.max_items);
   return w;
 }
 
@@ -488,15 +512,13 @@ int Sim::tiled_p2g(hipStream_t s) {
     });
     side_clean_ = true;
   }
-  PHASE(PH_P2G, HIPCHK(mpmk::g2p2g_block(particles(), d_mats_, (int)mats_.size(), k_, work_items(),
-                                         nullptr, tile_sums(), 0, kinds_, s),
-                       "p2g"));
-  return MPM_OK;
+  int rc = MPM_OK;
+  PHASE(PH_P2G, rc = run_tiled(nullptr, 0, s));
+  return rc;
 }
 
 int Sim::substep(hipStream_t s) {
   const ParticlesDev P0 = particles();
-  const int nm = (int)mats_.size();
   if (opt_.p2g_mode == 1) {
     if (!opt_.fused) {
       // reference order: P2G, grid update, G2P
@@ -520,9 +542,9 @@ int Sim::substep(hipStream_t s) {
       int rc = sort_particles(s);
       if (rc) return rc;
     }
-    PHASE(PH_G2P2G, HIPCHK(mpmk::g2p2g_block(particles(), d_mats_, nm, k_, work_items(), d_grid_[0],
-                                             tile_sums(), 1, kinds_, s),
-                           "g2p2g"));
+    int rc = MPM_OK;
+    PHASE(PH_G2P2G, rc = run_tiled(d_grid_[0], 1, s));
+    if (rc) return rc;
     vel_grid_ = d_grid_[0];
     ++steps_since_sort_;
     return MPM_OK;
@@ -684,7 +706,7 @@ int Sim::debug_sort(uint32_t* keys, uint32_t* perm, hipStream_t s) {
   const ParticlesDev P = particles();
   HIPCHK(mpmk::cell_keys(P, k_, lb_, d_keys_[0], d_vals_[0], s), "cell keys");
   int out = 0;
-  HIPCHK(mpmk::radix_sort(d_keys_, d_vals_, P_, mpmk::cell_key_bits(k_.N, lb_), (int)opt_.radix_bits, d_hist_, &out, s), "radix sort");
+  HIPCHK(mpmk::radix_sort(d_keys_, d_vals_, P_, 0, mpmk::cell_key_bits(k_.N, lb_), (int)opt_.radix_bits, d_hist_, &out, s), "radix sort");
   std::vector<uint32_t> vals(P_), ids(P_);
   HIPCHK(hipMemcpyAsync(keys, d_keys_[out], sizeof(uint32_t) * P_, hipMemcpyDeviceToHost, s), "download keys");
   HIPCHK(hipMemcpyAsync(vals.data(), d_vals_[out], sizeof(uint32_t) * P_, hipMemcpyDeviceToHost, s), "download perm");

@@ -88,6 +88,7 @@ class Sim {
   std::mutex mu_;
   std::string err_;
   int device_ = 0;
+  int cus_ = 1;  // compute units of the device
   mpm_params prm_;
   StepConsts k_;
   Options opt_;
@@ -129,7 +130,8 @@ class Sim {
   size_t block_start_cap_ = 0;
   uint32_t* d_flag_ = nullptr;         // cap entries
   uint32_t* d_pos_ = nullptr;          // cap entries
-  uint32_t* d_dest_ = nullptr;         // cap entries
+  uint32_t* d_gather_ = nullptr;       // cap entries: source slot of each new slot
+  bool pending_gather_ = false;        // a sort's reorder waits for the next tiled pass
   uint32_t* d_item_start_ = nullptr;   // cap + 1 entries
   uint32_t* d_item_block_ = nullptr;   // cap entries
   uint32_t* d_item_waves_ = nullptr;   // cap entries
@@ -143,6 +145,8 @@ class Sim {
   int64_t max_items() const;
   mpmk::TileSums tile_sums() const;
   int tiled_p2g(hipStream_t s);        // P2G of the current state into the tiles
+  // the fused kernel on the current particles; performs a pending reorder
+  int run_tiled(const float4* grid_v, int do_g2p, hipStream_t s);
 
   // sort scratch
   uint32_t* d_keys_[2] = {nullptr, nullptr};

@@ -24,9 +24,10 @@ int cell_key_bits(int N, int lb);
 size_t radix_scratch_words(int64_t n, int radix_bits);
 hipError_t cell_keys(const ParticlesDev& P, const StepConsts& k, int lb, uint32_t* keys,
                      uint32_t* vals, hipStream_t s);
-// Stable LSD radix sort of (keys[0], vals[0]); the result lands in
-// keys[*out_idx], vals[*out_idx]. scratch holds radix_scratch_words() words.
-hipError_t radix_sort(uint32_t* keys[2], uint32_t* vals[2], int64_t n, int key_bits,
+// Stable LSD radix sort of (keys[0], vals[0]) on key bits [start_bit,
+// key_bits); the result lands in keys[*out_idx], vals[*out_idx]. scratch
+// holds radix_scratch_words() words.
+hipError_t radix_sort(uint32_t* keys[2], uint32_t* vals[2], int64_t n, int start_bit, int key_bits,
                       int radix_bits, uint32_t* scratch, int* out_idx, hipStream_t s);
 hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* scratch,
                           uint32_t* total, hipStream_t s);
@@ -44,18 +45,19 @@ constexpr int kTileNodes = kTileW * kTileW * kTileW;
 
 // Work items over sorted keys: item_start[0..*total] (sentinel n at the end)
 // and item_block[0..*total); block_items[2b], [2b+1]: the items of block b
-// (cleared for all nblk blocks first). With dest, k_item_arrange fills dest
-// (new index of each sorted particle, a permutation inside each item) and
-// item_waves (per-wavefront particle counts). Arrays of n+1 entries;
-// block_range holds two entries per grid block (Morton space).
-hipError_t build_items(const uint32_t* keys, int64_t n, int lb, int chunk, int64_t nblk,
-                       uint32_t* block_range, uint32_t* block_items, uint32_t* flag, uint32_t* pos,
-                       uint32_t* scratch, uint32_t* total, uint32_t* item_start,
-                       uint32_t* item_block, uint32_t* item_waves, uint32_t* dest,
+// (cleared for all nblk blocks first). k_item_arrange then lays out each
+// item over its wavefronts: item_waves (per-wavefront particle counts) and
+// gather[j] = perm[i] for the sorted particle i that lands in slot j.
+// Arrays of n+1 entries; block_range holds two entries per grid block
+// (Morton space).
+hipError_t build_items(const uint32_t* keys, const uint32_t* perm, int64_t n, int lb, int chunk,
+                       int64_t nblk, uint32_t* block_range, uint32_t* block_items, uint32_t* flag,
+                       uint32_t* pos, uint32_t* scratch, uint32_t* total, uint32_t* item_start,
+                       uint32_t* item_block, uint32_t* item_waves, uint32_t* gather,
                        int64_t max_items, hipStream_t s);
-// dst[dest ? dest[i] : i] = src[perm[i]]
+// dst[i] = src[perm[i]]
 hipError_t reorder(const ParticlesDev& src, const ParticlesDev& dst, const uint32_t* perm,
-                   const uint32_t* dest, hipStream_t s);
+                   hipStream_t s);
 
 // p2g_block.hip: fused G2P + tiled P2G over work items, and the tile reduction
 struct WorkItems {
@@ -63,7 +65,8 @@ struct WorkItems {
   const uint32_t* block;   // Morton index of the item's home block
   const uint32_t* waves;   // per-wavefront particle counts, 8 bits each
   const uint32_t* count;   // device scalar: number of items
-  int64_t max_items;       // launch bound
+  int64_t max_items;       // bound on the number of items
+  int64_t launch;          // workgroups to launch (one per possible item)
 };
 // Tiled P2G output of one pass: one kTileNodes-node float4 tile per item,
 // plus the grid `atomic` (kept zero between passes) that particles leaving
@@ -75,9 +78,13 @@ struct TileSums {
   uint32_t* dirty;
   const uint32_t* block_items;  // 2 entries per home block (Morton space)
 };
-hipError_t g2p2g_block(const ParticlesDev& P, const MaterialDev* mats, int nmats,
-                       const StepConsts& k, const WorkItems& w, const float4* grid_v,
-                       const TileSums& out, int do_g2p, int kinds, hipStream_t s);
+// With `gather`, P is the particle order before the last sort: thread slot i
+// reads particle gather[i] of P and writes it to slot i of Pdst (the sort's
+// reorder folded into this pass). Without it Pdst must be P.
+hipError_t g2p2g_block(const ParticlesDev& P, const ParticlesDev& Pdst, const uint32_t* gather,
+                       const MaterialDev* mats, int nmats, const StepConsts& k,
+                       const WorkItems& w, const float4* grid_v, const TileSums& out,
+                       int do_g2p, int kinds, hipStream_t s);
 // Sums the tiles (and the dirty parts of the atomic grid, which it clears)
 // into every node of `grid`; update=1 applies the grid update (velocities),
 // update=0 leaves the raw P2G sums (momentum, mass).

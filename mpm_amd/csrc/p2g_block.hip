@@ -96,7 +96,7 @@ __device__ __forceinline__ void store_state(const PlaneIO& io, uint32_t i, const
   io.st(PL_JP, o, s.Jp);
 }
 
-__device__ __forceinline__ void gather(const PState& s, const int base[3], const float w[3][3],
+__device__ __forceinline__ void gather_global(const PState& s, const int base[3], const float w[3][3],
                                        const StepConsts& k, const float4* __restrict__ grid,
                                        float v[3], float C[9]) {
   const int N = k.N;
@@ -219,7 +219,8 @@ struct StencilSlots {
 
 template <int KS>
 __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
-    k_g2p2g_tiled(ParticlesDev P, const MaterialDev* __restrict__ mats, StepConsts k,
+    k_g2p2g_tiled(ParticlesDev P, ParticlesDev Pdst, const uint32_t* __restrict__ gather,
+                  const MaterialDev* __restrict__ mats, StepConsts k,
                   const uint32_t* __restrict__ item_start, const uint32_t* __restrict__ item_block,
                   const uint32_t* __restrict__ item_waves, const uint32_t* __restrict__ item_count,
                   const float4* __restrict__ grid_v, float4* __restrict__ tiles,
@@ -275,12 +276,22 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
   int base[3] = {0, 0, 0};
   float w[3][3];
   bool inside = false;
-  const PlaneIO io(P);
+  // After a sort the particles still sit in the order before it: thread i
+  // reads particle gather[i] of P and writes its state to slot i of Pdst.
+  const PlaneIO io(P), io_dst(Pdst);
   if (valid) {
-    load_state(io, i, s);
-    m = mats[P.mat[i]];
+    const uint32_t src = gather ? gather[i] : i;
+    load_state(io, src, s);
+    const uint8_t mat = P.mat[src];
+    const uint32_t pid = P.id[src];
+    m = mats[mat];
+    if (gather) {
+      Pdst.mat[i] = mat;
+      Pdst.id[i] = pid;
+    }
     quad_weights(s.x, k.dx_inv, base, w);
-    if (do_g2p && P.id[i] <= k.guard && !outside_domain(base, N)) {
+    const bool moved = do_g2p && pid <= k.guard && !outside_domain(base, N);
+    if (moved) {
       // ---- G2P of substep n, gathering the 27 nodes from the LDS tile
       const int l0 = base[0] - o0, l1 = base[1] - o1, l2 = base[2] - o2;
       if (l0 >= 0 && l1 >= 0 && l2 >= 0 && l0 < NCE && l1 < NCE && l2 < NCE) {
@@ -331,7 +342,7 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
           s.C[3 * i + 2] = c2;
         }
       } else {
-        gather(s, base, w, k, grid_v, s.v, s.C);
+        gather_global(s, base, w, k, grid_v, s.v, s.C);
       }
       float A[9];
 #pragma unroll
@@ -349,9 +360,9 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
       }
 #pragma unroll
       for (int a = 0; a < 3; ++a) s.x[a] = s.x[a] + k.dt * s.v[a];
-      store_state(io, i, s);
       quad_weights(s.x, k.dx_inv, base, w);
     }
+    if (moved || gather) store_state(io_dst, i, s);
     inside = !outside_domain(base, N);
   }
 
@@ -601,25 +612,28 @@ __global__ void __launch_bounds__(256)
 namespace mpmk {
 
 template <int KS>
-static void launch_tiled(const ParticlesDev& P, const MaterialDev* mats, const StepConsts& k,
-                         const WorkItems& w, const float4* gv, const TileSums& o, int do_g2p,
-                         hipStream_t s) {
-  hipLaunchKernelGGL((k_g2p2g_tiled<KS>), dim3((unsigned)w.max_items), dim3(64 * kItemWaves), 0, s,
-                     P, mats, k,
+static void launch_tiled(const ParticlesDev& P, const ParticlesDev& Pdst, const uint32_t* gather,
+                         const MaterialDev* mats, const StepConsts& k, const WorkItems& w,
+                         const float4* gv, const TileSums& o, int do_g2p, hipStream_t s) {
+  hipLaunchKernelGGL((k_g2p2g_tiled<KS>), dim3((unsigned)w.launch), dim3(64 * kItemWaves), 0, s,
+                     P, Pdst, gather, mats, k,
                      w.start, w.block, w.waves, w.count, gv, o.tiles, (float*)o.atomic, o.dirty,
                      do_g2p);
 }
 
-hipError_t g2p2g_block(const ParticlesDev& P, const MaterialDev* mats, int nmats,
-                       const StepConsts& k, const WorkItems& w, const float4* grid_v,
-                       const TileSums& out, int do_g2p, int kinds, hipStream_t s) {
+hipError_t g2p2g_block(const ParticlesDev& P, const ParticlesDev& Pdst, const uint32_t* gather,
+                       const MaterialDev* mats, int nmats, const StepConsts& k,
+                       const WorkItems& w, const float4* grid_v, const TileSums& out,
+                       int do_g2p, int kinds, hipStream_t s) {
   if (P.n <= 0 || w.max_items <= 0) return hipSuccess;
   if (nmats < 1 || nmats > MPM_MAX_MATERIALS) return hipErrorInvalidValue;
   if (do_g2p && !grid_v) return hipErrorInvalidValue;
   if (!out.tiles || !out.atomic || !out.dirty) return hipErrorInvalidValue;
-  if (w.max_items > 0x7fffffff) return hipErrorInvalidValue;
-  if (kinds == KINDS_LIQUID) launch_tiled<KINDS_LIQUID>(P, mats, k, w, grid_v, out, do_g2p, s);
-  else launch_tiled<KINDS_ALL>(P, mats, k, w, grid_v, out, do_g2p, s);
+  if (w.launch > 0x7fffffff || w.launch < 1) return hipErrorInvalidValue;
+  if (gather && (Pdst.f == P.f || Pdst.n != P.n)) return hipErrorInvalidValue;
+  if (!gather && Pdst.f != P.f) return hipErrorInvalidValue;
+  if (kinds == KINDS_LIQUID) launch_tiled<KINDS_LIQUID>(P, Pdst, gather, mats, k, w, grid_v, out, do_g2p, s);
+  else launch_tiled<KINDS_ALL>(P, Pdst, gather, mats, k, w, grid_v, out, do_g2p, s);
   return hipGetLastError();
 }
 

@@ -120,39 +120,75 @@ __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, int6
 // (r + cell) mod nw, so particles sharing a cell land in different wavefronts
 // as long as a cell holds at most nw of the item's particles. A wavefront
 // takes at most 64 particles; the excess fills the free lanes of the other
-// wavefronts in order (the first nw hold enough). dest[i] is the new index of
-// sorted particle i (a permutation inside the item); item_waves packs the 4
-// per-wavefront counts (8 bits each). The layout is a function of the sorted
+// wavefronts in order (the first nw hold enough). gather[j] receives the
+// original index perm[i] of the sorted particle i that takes slot j (a
+// permutation inside the item); item_waves packs the per-wavefront counts
+// (8 bits each). The layout is a function of the sorted
 // keys only. One workgroup of 256 threads per item (n <= kMaxItemParticles).
 __global__ void __launch_bounds__(256)
-    k_item_arrange(const uint32_t* __restrict__ keys, int cell_bits,
-                   const uint32_t* __restrict__ item_start, const uint32_t* __restrict__ count,
-                   uint32_t* __restrict__ dest, uint32_t* __restrict__ item_waves) {
+    k_item_arrange(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ perm,
+                   int cell_bits, const uint32_t* __restrict__ item_start,
+                   const uint32_t* __restrict__ count, uint32_t* __restrict__ gather,
+                   uint32_t* __restrict__ item_waves) {
   const uint32_t item = blockIdx.x;
   if (item >= *count) return;  // uniform for the workgroup
   __shared__ uint32_t sk[256];
-  __shared__ int wmax[4];
+  __shared__ uint8_t orig[256];
+  __shared__ uint32_t wcnt[4][64];
+  __shared__ uint32_t cstart[64];
   __shared__ unsigned long long wsum[4];
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int lane = threadIdx.x & 63;
   const uint32_t s = item_start[item];
   const int n = (int)(item_start[item + 1] - s);  // 1..kMaxItemParticles
-  const bool valid = t < n;
-  const uint32_t key = valid ? keys[s + t] : 0xffffffffu;
-  sk[t] = key;
-  __syncthreads();
-  // rank within the cell: distance to the first particle of its run of equal keys
-  int head = (valid && (t == 0 || sk[t - 1] != key)) ? t : 0;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int y = __shfl_up(head, off);
-    if (lane >= off) head = max(head, y);
+  const uint32_t cmask = (1u << cell_bits) - 1u;  // cell within the home block (<= 64 cells)
+  // -- stable counting sort of the item by cell (the keys arrive sorted by
+  //    home block only): rank among equal cells from wavefront ballots, then
+  //    per-wavefront counts and an exclusive scan over the cells
+  {
+    const int t0 = threadIdx.x, wv0 = t0 >> 6;
+    for (int i = t0; i < 4 * 64; i += 256) (&wcnt[0][0])[i] = 0u;
+    __syncthreads();
+    const bool v0 = t0 < n;
+    const uint32_t key0 = v0 ? keys[s + t0] : 0u;
+    const uint32_t c0 = key0 & cmask;
+    uint64_t peers = __ballot(v0);
+    for (int b = 0; b < cell_bits; ++b) {
+      const uint64_t m = __ballot((c0 >> b) & 1u);
+      peers &= ((c0 >> b) & 1u) ? m : ~m;
+    }
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t rank_w = (uint32_t)__popcll(peers & lt);
+    if (v0 && (peers & lt) == 0ull) wcnt[wv0][c0] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (t0 < 64) {
+      uint32_t acc = 0;
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t v = wcnt[q][t0];
+        wcnt[q][t0] = acc;
+        acc += v;
+      }
+      // exclusive scan of the cell totals over the 64 lanes of wavefront 0
+      uint32_t incl = acc;
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+      }
+      cstart[t0] = incl - acc;
+    }
+    __syncthreads();
+    if (v0) {
+      const uint32_t pos = cstart[c0] + wcnt[wv0][c0] + rank_w;
+      sk[pos] = key0;
+      orig[pos] = (uint8_t)t0;
+    }
+    __syncthreads();
   }
-  if (lane == 63) wmax[wv] = head;
-  // per-wavefront counts as 4 x 16-bit fields, exclusive prefix over the item
-  const int cell = (int)(key & ((1u << cell_bits) - 1u));
-  __syncthreads();
-  for (int q = 0; q < wv; ++q) head = max(head, wmax[q]);
-  const int rank = t - head;
+  // -- from here on thread t holds the t-th particle of the item in cell order
+  const int t = threadIdx.x, wv = t >> 6;
+  const bool valid = t < n;
+  const uint32_t key = valid ? sk[t] : 0u;
+  const int cell = (int)(key & cmask);
+  const int rank = valid ? t - (int)cstart[cell] : 0;  // rank within its cell
   const int nw = (n + 63) >> 6;  // wavefronts the item needs
   const int w = (rank + cell) % nw;
   const unsigned long long one = valid ? (1ull << (16 * w)) : 0ull;
@@ -200,7 +236,7 @@ __global__ void __launch_bounds__(256)
   }
   int off = 0;
   for (int q = 0; q < fw; ++q) off += fin[q];
-  if (valid) dest[s + t] = s + (uint32_t)(off + fl);
+  if (valid) gather[s + (uint32_t)(off + fl)] = perm[s + orig[t]];
   if (t == 0) {
     uint32_t packed = 0;
 #pragma unroll
@@ -392,29 +428,27 @@ __global__ void __launch_bounds__(kSortThreads)
   }
 }
 
-// dst[d] = src[perm[i]] with d = dest[i] (or i) for every plane, material and id
+// dst[i] = src[perm[i]] for every plane, material and id
 __global__ void __launch_bounds__(256)
-    k_reorder(ParticlesDev src, ParticlesDev dst, const uint32_t* __restrict__ perm,
-              const uint32_t* __restrict__ dest) {
+    k_reorder(ParticlesDev src, ParticlesDev dst, const uint32_t* __restrict__ perm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= src.n) return;
   const uint32_t j = perm[i];
-  const uint32_t d = dest ? dest[i] : (uint32_t)i;
 #pragma unroll
-  for (int p = 0; p < NUM_PLANES; ++p) dst.f[p * dst.stride + d] = src.f[p * src.stride + j];
-  dst.mat[d] = src.mat[j];
-  dst.id[d] = src.id[j];
+  for (int p = 0; p < NUM_PLANES; ++p) dst.f[p * dst.stride + i] = src.f[p * src.stride + j];
+  dst.mat[i] = src.mat[j];
+  dst.id[i] = src.id[j];
 }
 
 template <int BITS>
-hipError_t radix_sort_t(uint32_t* keys[2], uint32_t* vals[2], int64_t n, int key_bits,
+hipError_t radix_sort_t(uint32_t* keys[2], uint32_t* vals[2], int64_t n, int start_bit, int key_bits,
                         uint32_t* scratch, int* out_idx, hipStream_t s) {
   constexpr int R = 1 << BITS;
   const int ntiles = (int)((n + kTile - 1) / kTile);
   uint32_t* hist = scratch;
   uint32_t* total = scratch + (size_t)R * ntiles;
   int cur = 0;
-  for (int shift = 0; shift < key_bits; shift += BITS) {
+  for (int shift = start_bit; shift < key_bits; shift += BITS) {
     hipLaunchKernelGGL(k_radix_hist<BITS>, dim3(ntiles), dim3(kSortThreads), 0, s, keys[cur], n,
                        shift, hist, ntiles);
     hipLaunchKernelGGL(k_radix_scan, dim3(R), dim3(256), 0, s, hist, ntiles, total);
@@ -465,10 +499,10 @@ hipError_t fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t build_items(const uint32_t* keys, int64_t n, int lb, int chunk, int64_t nblk,
-                       uint32_t* block_range, uint32_t* block_items, uint32_t* flag, uint32_t* pos,
-                       uint32_t* scratch, uint32_t* total, uint32_t* item_start,
-                       uint32_t* item_block, uint32_t* item_waves, uint32_t* dest,
+hipError_t build_items(const uint32_t* keys, const uint32_t* perm, int64_t n, int lb, int chunk,
+                       int64_t nblk, uint32_t* block_range, uint32_t* block_items, uint32_t* flag,
+                       uint32_t* pos, uint32_t* scratch, uint32_t* total, uint32_t* item_start,
+                       uint32_t* item_block, uint32_t* item_waves, uint32_t* gather,
                        int64_t max_items, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   if (chunk < 1 || chunk > kMaxItemParticles || max_items < 1) return hipErrorInvalidValue;
@@ -482,28 +516,28 @@ hipError_t build_items(const uint32_t* keys, int64_t n, int lb, int chunk, int64
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_item_emit, dim3(g), dim3(256), 0, s, keys, n, shift, flag, pos, total,
                      item_start, item_block, block_items);
-  if (dest)
-    hipLaunchKernelGGL(k_item_arrange, dim3((unsigned)max_items), dim3(256), 0, s, keys, shift,
-                       item_start, total, dest, item_waves);
+  if (!perm || !gather || !item_waves) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_item_arrange, dim3((unsigned)max_items), dim3(256), 0, s, keys, perm, shift,
+                     item_start, total, gather, item_waves);
   return hipGetLastError();
 }
 
-hipError_t radix_sort(uint32_t* keys[2], uint32_t* vals[2], int64_t n, int key_bits,
+hipError_t radix_sort(uint32_t* keys[2], uint32_t* vals[2], int64_t n, int start_bit, int key_bits,
                       int radix_bits, uint32_t* scratch, int* out_idx, hipStream_t s) {
   *out_idx = 0;
-  if (n <= 1 || key_bits <= 0) return hipSuccess;
+  if (n <= 1 || key_bits <= start_bit) return hipSuccess;
   switch (radix_bits) {
-    case 4: return radix_sort_t<4>(keys, vals, n, key_bits, scratch, out_idx, s);
-    case 6: return radix_sort_t<6>(keys, vals, n, key_bits, scratch, out_idx, s);
-    case 8: return radix_sort_t<8>(keys, vals, n, key_bits, scratch, out_idx, s);
+    case 4: return radix_sort_t<4>(keys, vals, n, start_bit, key_bits, scratch, out_idx, s);
+    case 6: return radix_sort_t<6>(keys, vals, n, start_bit, key_bits, scratch, out_idx, s);
+    case 8: return radix_sort_t<8>(keys, vals, n, start_bit, key_bits, scratch, out_idx, s);
     default: return hipErrorInvalidValue;
   }
 }
 
 hipError_t reorder(const ParticlesDev& src, const ParticlesDev& dst, const uint32_t* perm,
-                   const uint32_t* dest, hipStream_t s) {
+                   hipStream_t s) {
   if (src.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_reorder, dim3((src.n + 255) / 256), dim3(256), 0, s, src, dst, perm, dest);
+  hipLaunchKernelGGL(k_reorder, dim3((src.n + 255) / 256), dim3(256), 0, s, src, dst, perm);
   return hipGetLastError();
 }
 
