@@ -66,19 +66,22 @@ struct PlaneIO {
   }
 };
 
-__device__ __forceinline__ void load_state(const PlaneIO& io, uint32_t i, PState& s) {
+// x, F, Jp: what G2P reads (v and C are recomputed by it)
+__device__ __forceinline__ void load_xfj(const PlaneIO& io, uint32_t i, PState& s) {
   const uint32_t o = i * 4u;
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    s.x[a] = io.ld(PL_X + a, o);
-    s.v[a] = io.ld(PL_V + a, o);
-  }
+  for (int a = 0; a < 3; ++a) s.x[a] = io.ld(PL_X + a, o);
 #pragma unroll
-  for (int a = 0; a < 9; ++a) {
-    s.C[a] = io.ld(PL_C + a, o);
-    s.F[a] = io.ld(PL_F + a, o);
-  }
+  for (int a = 0; a < 9; ++a) s.F[a] = io.ld(PL_F + a, o);
   s.Jp = io.ld(PL_JP, o);
+}
+// v, C: needed only for particles G2P leaves as they are
+__device__ __forceinline__ void load_vc(const PlaneIO& io, uint32_t i, PState& s) {
+  const uint32_t o = i * 4u;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) s.v[a] = io.ld(PL_V + a, o);
+#pragma unroll
+  for (int a = 0; a < 9; ++a) s.C[a] = io.ld(PL_C + a, o);
 }
 
 __device__ __forceinline__ void store_state(const PlaneIO& io, uint32_t i, const PState& s) {
@@ -188,6 +191,23 @@ __device__ __forceinline__ void scatter_global(const float q0[3][3], const float
 // Material-kind sets compiled into an instantiation.
 enum KindSet : int { KINDS_ALL = 0, KINDS_LIQUID = 1 };
 
+// Wavefront lane shifts and an inclusive max-scan with DPP (no LDS traffic).
+__device__ __forceinline__ int dpp_wave_shr1(int v) {  // lane l gets lane l-1; lane 0 gets 0
+  return __builtin_amdgcn_update_dpp(0, v, 0x138 /* wave_shr:1 */, 0xf, 0xf, true);
+}
+__device__ __forceinline__ int dpp_wave_shl1(int v) {  // lane l gets lane l+1; lane 63 gets 0
+  return __builtin_amdgcn_update_dpp(0, v, 0x130 /* wave_shl:1 */, 0xf, 0xf, true);
+}
+__device__ __forceinline__ int wave_max_scan(int v) {  // v >= 0
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111 /* row_shr:1 */, 0xf, 0xf, true));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112 /* row_shr:2 */, 0xf, 0xf, true));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114 /* row_shr:4 */, 0xf, 0xf, true));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118 /* row_shr:8 */, 0xf, 0xf, true));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142 /* row_bcast:15 */, 0xa, 0xf, true));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143 /* row_bcast:31 */, 0xc, 0xf, true));
+  return v;
+}
+
 // LDS slot of tile node (x, y, z) in an 8^3 tile: z is permuted within its
 // row (bit 2 flipped when x + y is odd), so the runs of 4 neighbouring nodes
 // that the lanes of one access touch in adjacent rows or planes land in
@@ -219,6 +239,7 @@ struct StencilSlots {
 
 template <int KS>
 __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
+    __attribute__((amdgpu_waves_per_eu(KS == KINDS_LIQUID ? 5 : 4)))
     k_g2p2g_tiled(ParticlesDev P, ParticlesDev Pdst, const uint32_t* __restrict__ gather,
                   const MaterialDev* __restrict__ mats, StepConsts k,
                   const uint32_t* __restrict__ item_start, const uint32_t* __restrict__ item_block,
@@ -251,6 +272,30 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
   const int o1 = (int)compact3(blk >> 1) * BE - 1;
   const int o2 = (int)compact3(blk) * BE - 1;
 
+  const bool valid = lane < cnt_w;
+  const uint32_t i = start + (uint32_t)(off_w + lane);
+  PState s;
+  MaterialDev m;
+  float R[9];
+  bool has_R = false;
+  int base[3] = {0, 0, 0};
+  float w[3][3];
+  bool inside = false;
+  // After a sort the particles still sit in the order before it: thread i
+  // reads particle gather[i] of P and writes its state to slot i of Pdst.
+  // The particle loads are issued before the tile staging so that both are
+  // in flight together.
+  const PlaneIO io(P), io_dst(Pdst);
+  uint32_t src = 0, pid = 0;
+  uint8_t mat = 0;
+  if (valid) {
+    src = gather ? gather[i] : i;
+    load_xfj(io, src, s);
+    mat = P.mat[src];
+    pid = P.id[src];
+    m = mats[mat];
+  }
+
   if (do_g2p) {
     for (int n = tid; n < TN; n += NT) {
       const int a = n / (W * W), r = n - a * W * W, b = r / W, c = r - b * W;
@@ -267,30 +312,14 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
   }
   __syncthreads();
 
-  const bool valid = lane < cnt_w;
-  const uint32_t i = start + (uint32_t)(off_w + lane);
-  PState s;
-  MaterialDev m;
-  float R[9];
-  bool has_R = false;
-  int base[3] = {0, 0, 0};
-  float w[3][3];
-  bool inside = false;
-  // After a sort the particles still sit in the order before it: thread i
-  // reads particle gather[i] of P and writes its state to slot i of Pdst.
-  const PlaneIO io(P), io_dst(Pdst);
   if (valid) {
-    const uint32_t src = gather ? gather[i] : i;
-    load_state(io, src, s);
-    const uint8_t mat = P.mat[src];
-    const uint32_t pid = P.id[src];
-    m = mats[mat];
     if (gather) {
       Pdst.mat[i] = mat;
       Pdst.id[i] = pid;
     }
     quad_weights(s.x, k.dx_inv, base, w);
     const bool moved = do_g2p && pid <= k.guard && !outside_domain(base, N);
+    if (!moved) load_vc(io, src, s);
     if (moved) {
       // ---- G2P of substep n, gathering the 27 nodes from the LDS tile
       const int l0 = base[0] - o0, l1 = base[1] - o1, l2 = base[2] - o2;
@@ -437,27 +466,31 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
   //    one read-modify-write the writing lanes address distinct nodes; the
   //    compiler barrier keeps the read-modify-writes in program order (a
   //    lane's next read may address a node another lane writes this round).
-  // runs of equal cells: lanes without a tile particle are runs of their own
-  const int cid = pending ? cell : 1024 + lane;
-  const int prev = __shfl_up(cid, 1), next = __shfl_down(cid, 1);
-  int head = (lane == 0 || prev != cid) ? lane : 0;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int y = __shfl_up(head, off);
-    if (lane >= off) head = max(head, y);
-  }
+  // runs of equal cells (DPP lane shifts; lanes without a tile particle are
+  // runs of their own)
+  const int cid1 = (pending ? cell : 1024 + lane) + 1;
+  const int prev = dpp_wave_shr1(cid1), next = dpp_wave_shl1(cid1);  // 0 beyond the ends
+  const int head = wave_max_scan(prev != cid1 ? lane : 0);
   const int rank = lane - head;  // position within the run
-  const bool tail = lane == 63 || next != cid;
-  int maxrun = rank + 1;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) maxrun = max(maxrun, __shfl_xor(maxrun, off));
+  const bool tail = next != cid1;
+  const int maxrun = __builtin_amdgcn_readlane(wave_max_scan(rank + 1), 63);
+  const bool reduce = maxrun > 4;
   const int lr = lane & 15;
 
   typedef __attribute__((address_space(3))) volatile uint8_t lds_byte;
   lds_byte* const claim = (lds_byte*)(&claims[wave][0]);
   float4* const my = acc[wave];
   const StencilSlots slot(l0, l1, l2);
-  // one slab of 9 nodes (offset a along x) at a time keeps 36 sums in registers
+  // Writers (run tails after a reduction, else every lane with a tile
+  // particle) claim their cell; the lanes whose claim stands add their nodes,
+  // one slab of 9 nodes (offset a along x) at a time so that 36 sums sit in
+  // registers. Writers that lost (a cell with more than one writer in the
+  // wavefront) retry slab by slab in further claim rounds.
+  const bool writer = pending && (!reduce || tail);
+  if (writer) claim[cell] = (uint8_t)lane;
+  const bool mine = writer && claim[cell] == (uint8_t)lane;
+  const bool lost = writer && !mine;
+  const bool any_lost = __ballot(lost) != 0ull;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     float cv[9][4];
@@ -484,37 +517,40 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
         cv[o][q] = m_ ? cv[o][q] + t_ : cv[o][q];                                          \
       }                                                                                    \
     }
-    if (maxrun > 4) {
+    if (reduce) {
       MPM_SEG_STEP(0x111 /* row_shr:1 */, 0xf, rank >= 1 && lr >= 1)
       MPM_SEG_STEP(0x112 /* row_shr:2 */, 0xf, rank >= 2 && lr >= 2)
-      if (maxrun > 4) MPM_SEG_STEP(0x114 /* row_shr:4 */, 0xf, rank >= 4 && lr >= 4)
+      MPM_SEG_STEP(0x114 /* row_shr:4 */, 0xf, rank >= 4 && lr >= 4)
       if (maxrun > 8) MPM_SEG_STEP(0x118 /* row_shr:8 */, 0xf, rank >= 8 && lr >= 8)
       // carry across rows: rows 1, 3 take lane 15 of the row below, rows 2, 3 lane 31
       MPM_SEG_STEP(0x142 /* row_bcast:15 */, 0xa, ((lane >> 4) & 1) && rank > lr)
       MPM_SEG_STEP(0x143 /* row_bcast:31 */, 0xc, lane >= 32 && rank > lane - 32)
     }
 #undef MPM_SEG_STEP
-    bool todo = pending && (maxrun <= 4 || tail);
-    while (__ballot(todo) != 0ull) {
-      if (todo) claim[cell] = (uint8_t)lane;
-      if (todo && claim[cell] == (uint8_t)lane) {
-#pragma unroll
-        for (int b = 0; b < 3; ++b)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            const int o = b * 3 + c;
-            float4* const p = my + slot(a, b, c);
-            float4 t = *p;
-            t.x += cv[o][0];
-            t.y += cv[o][1];
-            t.z += cv[o][2];
-            t.w += cv[o][3];
-            *p = t;
-            __asm__ __volatile__("" ::: "memory");
-          }
-        todo = false;
+#define MPM_RMW_SLAB                                                                       \
+    _Pragma("unroll") for (int b = 0; b < 3; ++b) _Pragma("unroll") for (int c = 0; c < 3; ++c) { \
+      const int o = b * 3 + c;                                                             \
+      float4* const p = my + slot(a, b, c);                                                \
+      float4 t = *p;                                                                       \
+      t.x += cv[o][0];                                                                     \
+      t.y += cv[o][1];                                                                     \
+      t.z += cv[o][2];                                                                     \
+      t.w += cv[o][3];                                                                     \
+      *p = t;                                                                              \
+      __asm__ __volatile__("" ::: "memory");                                               \
+    }
+    if (mine) { MPM_RMW_SLAB }
+    if (any_lost) {
+      bool todo = lost;
+      while (__ballot(todo) != 0ull) {
+        if (todo) claim[cell] = (uint8_t)lane;
+        if (todo && claim[cell] == (uint8_t)lane) {
+          MPM_RMW_SLAB
+          todo = false;
+        }
       }
     }
+#undef MPM_RMW_SLAB
   }
   __syncthreads();
 
