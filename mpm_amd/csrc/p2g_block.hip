@@ -311,6 +311,16 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
     for (int n = tid; n < (NW - 1) * TN; n += NT) flat[n] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   __syncthreads();
+  // The material arrives with the particle loads; take it into registers here,
+  // before any particle store: vector-memory waits count stores too, so a
+  // first use after the stores would wait for all of them to complete.
+  if (valid) {
+    // (through an opaque move, so the compiler keeps the registers instead of
+    // loading the read-only table again at a later use)
+    __asm__ __volatile__("" : "+v"(m.kind), "+v"(m.volume), "+v"(m.mass), "+v"(m.mu0),
+                         "+v"(m.lambda0), "+v"(m.hardening), "+v"(m.clamp_lo), "+v"(m.clamp_hi),
+                         "+v"(m.bulk), "+v"(m.stress_scale));
+  }
 
   if (valid) {
     if (gather) {
@@ -319,7 +329,14 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
     }
     quad_weights(s.x, k.dx_inv, base, w);
     const bool moved = do_g2p && pid <= k.guard && !outside_domain(base, N);
-    if (!moved) load_vc(io, src, s);
+    if (!moved) {
+      load_vc(io, src, s);
+      // complete these loads here: left in flight they would meet the
+      // particle stores of the other lanes at the next vector-memory wait
+      __asm__ __volatile__("" : "+v"(s.v[0]), "+v"(s.v[1]), "+v"(s.v[2]), "+v"(s.C[0]), "+v"(s.C[1]),
+                           "+v"(s.C[2]), "+v"(s.C[3]), "+v"(s.C[4]), "+v"(s.C[5]), "+v"(s.C[6]),
+                           "+v"(s.C[7]), "+v"(s.C[8]));
+    }
     if (moved) {
       // ---- G2P of substep n, gathering the 27 nodes from the LDS tile
       const int l0 = base[0] - o0, l1 = base[1] - o1, l2 = base[2] - o2;
