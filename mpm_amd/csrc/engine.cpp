@@ -89,7 +89,7 @@ Sim::~Sim() {
   if (d_hist_) (void)hipFree(d_hist_);
   if (d_mats_) (void)hipFree(d_mats_);
   for (uint32_t* p : {d_block_start_, d_block_items_, d_flag_, d_pos_, d_gather_, d_item_start_,
-                      d_item_block_, d_item_waves_, d_item_count_, d_dirty_})
+                      d_item_block_, d_item_waves_, d_item_count_, d_dirty_, d_covered_})
     if (p) (void)hipFree(p);
   if (d_tiles_) (void)hipFree(d_tiles_);
   for (auto& p : ev_) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -188,10 +188,12 @@ int Sim::ensure_grids() {
   }
   grid_nodes_ = nodes;
   const size_t nbx = ((size_t)prm_.N + 3) / 4;
-  if (d_dirty_) (void)hipFree(d_dirty_);
-  d_dirty_ = nullptr;
-  HIPCHK(hipMalloc(&d_dirty_, nbx * nbx * nbx * sizeof(uint32_t)), "hipMalloc(dirty flags)");
-  HIPCHK(hipMemset(d_dirty_, 0, nbx * nbx * nbx * sizeof(uint32_t)), "hipMemset(dirty flags)");
+  for (uint32_t** p : {&d_dirty_, &d_covered_}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    HIPCHK(hipMalloc(p, nbx * nbx * nbx * sizeof(uint32_t)), "hipMalloc(node block flags)");
+    HIPCHK(hipMemset(*p, 0, nbx * nbx * nbx * sizeof(uint32_t)), "hipMemset(node block flags)");
+  }
   dirty_cap_ = nbx * nbx * nbx;
   side_clean_ = true;
   vel_grid_ = nullptr;
@@ -422,8 +424,9 @@ int Sim::sort_particles(hipStream_t s) {
     tiles_cap_ = items;
   }
   if (P_ <= 0) {
-    // no items: every home block has an empty item range
+    // no items: every home block has an empty item range, no node block is reached
     HIPCHK(mpmk::fill_u32(d_block_items_, 2 * nblk_, 0u, s), "clear block items");
+    HIPCHK(mpmk::fill_u32(d_covered_, (int64_t)dirty_cap_, 0u, s), "clear node block flags");
     steps_since_sort_ = 0;
     items_valid_ = true;
     return MPM_OK;
@@ -437,8 +440,9 @@ int Sim::sort_particles(hipStream_t s) {
                             (int)opt_.radix_bits, d_hist_, &out, s),
            "radix sort");
     HIPCHK(mpmk::build_items(d_keys_[out], d_vals_[out], P_, lb_, chunk_, nblk_, d_block_start_,
-                             d_block_items_, d_flag_, d_pos_, d_hist_, d_item_count_, d_item_start_,
-                             d_item_block_, d_item_waves_, d_gather_, items, s),
+                             d_block_items_, d_covered_, (k_.N + 3) / 4, d_flag_, d_pos_, d_hist_,
+                             d_item_count_, d_item_start_, d_item_block_, d_item_waves_, d_gather_,
+                             items, s),
            "work items");
   });
   // the particles move to their new slots in the next tiled pass
@@ -495,6 +499,7 @@ mpmk::TileSums Sim::tile_sums() const {
   t.atomic = d_grid_[1];
   t.dirty = d_dirty_;
   t.block_items = d_block_items_;
+  t.covered = d_covered_;
   return t;
 }
 

@@ -121,6 +121,7 @@ class Sim {
   const float4* vel_grid_ = nullptr;  // grid holding the velocities of the last update
   bool side_clean_ = false;
   uint32_t* d_dirty_ = nullptr;      // one flag per 4^3 node block
+  uint32_t* d_covered_ = nullptr;    // one flag per 4^3 node block (set at the sort)
   size_t dirty_cap_ = 0;
 
   // work items of the tiled kernel (valid after a sort)

@@ -50,11 +50,13 @@ constexpr int kTileNodes = kTileW * kTileW * kTileW;
 // gather[j] = perm[i] for the sorted particle i that lands in slot j.
 // Arrays of n+1 entries; block_range holds two entries per grid block
 // (Morton space).
+// covered (nbx^3 flags, one per 4^3 node block) marks the node blocks some
+// item's tile can reach.
 hipError_t build_items(const uint32_t* keys, const uint32_t* perm, int64_t n, int lb, int chunk,
-                       int64_t nblk, uint32_t* block_range, uint32_t* block_items, uint32_t* flag,
-                       uint32_t* pos, uint32_t* scratch, uint32_t* total, uint32_t* item_start,
-                       uint32_t* item_block, uint32_t* item_waves, uint32_t* gather,
-                       int64_t max_items, hipStream_t s);
+                       int64_t nblk, uint32_t* block_range, uint32_t* block_items, uint32_t* covered,
+                       int nbx, uint32_t* flag, uint32_t* pos, uint32_t* scratch, uint32_t* total,
+                       uint32_t* item_start, uint32_t* item_block, uint32_t* item_waves,
+                       uint32_t* gather, int64_t max_items, hipStream_t s);
 // dst[i] = src[perm[i]]
 hipError_t reorder(const ParticlesDev& src, const ParticlesDev& dst, const uint32_t* perm,
                    hipStream_t s);
@@ -77,6 +79,7 @@ struct TileSums {
   float4* atomic;
   uint32_t* dirty;
   const uint32_t* block_items;  // 2 entries per home block (Morton space)
+  const uint32_t* covered;      // per node block: some tile reaches it
 };
 // With `gather`, P is the particle order before the last sort: thread slot i
 // reads particle gather[i] of P and writes it to slot i of Pdst (the sort's

@@ -597,14 +597,24 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
 // holding more than one item) after it. The order of the sum is fixed.
 __global__ void __launch_bounds__(256)
     k_tile_reduce(const float4* __restrict__ tiles, const uint32_t* __restrict__ block_items,
-                  float4* __restrict__ grid_atomic, uint32_t* __restrict__ dirty,
-                  float4* __restrict__ grid, StepConsts k, int nbx, int update) {
+                  const uint32_t* __restrict__ covered, float4* __restrict__ grid_atomic,
+                  uint32_t* __restrict__ dirty, float4* __restrict__ grid, StepConsts k, int nbx,
+                  int update) {
   constexpr int W = mpmk::kTileW, TN = mpmk::kTileNodes;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t nbk = (int64_t)blockIdx.x * 4 + wave;
   if (nbk >= (int64_t)nbx * nbx * nbx) return;  // uniform for the wavefront
   const int bz = (int)(nbk % nbx), by = (int)((nbk / nbx) % nbx), bx = (int)(nbk / ((int64_t)nbx * nbx));
   const int u0 = lane >> 4, u1 = (lane >> 2) & 3, u2 = lane & 3;
+  const int N = k.N;
+  const int x = bx * 4 + u0, y = by * 4 + u1, z = bz * 4 + u2;
+  const bool in_grid = x < N && y < N && z < N;
+  const size_t node = ((size_t)x * N + (size_t)y) * N + (size_t)z;
+  const bool is_dirty = dirty[nbk] != 0u;
+  if (!covered[nbk] && !is_dirty) {  // no tile and no side-grid scatter reaches it
+    if (in_grid) grid[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
   uint32_t r0 = 0, r1 = 0;
   if (lane < 27) {
     const int hx = bx + lane / 9 - 1, hy = by + (lane / 3) % 3 - 1, hz = bz + lane % 3 - 1;
@@ -614,11 +624,6 @@ __global__ void __launch_bounds__(256)
       r1 = block_items[2 * (size_t)mb + 1];
     }
   }
-  const int N = k.N;
-  const int x = bx * 4 + u0, y = by * 4 + u1, z = bz * 4 + u2;
-  const bool in_grid = x < N && y < N && z < N;
-  const size_t node = ((size_t)x * N + (size_t)y) * N + (size_t)z;
-  const bool is_dirty = dirty[nbk] != 0u;
   float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
   if (__ballot(r0 < r1) != 0ull) {  // some home block around holds items
     const int e0 = (u0 == 3) ? 0 : -1, e1 = (u1 == 3) ? 0 : -1, e2 = (u2 == 3) ? 0 : -1;
@@ -692,11 +697,12 @@ hipError_t g2p2g_block(const ParticlesDev& P, const ParticlesDev& Pdst, const ui
 
 hipError_t tile_reduce(const TileSums& in, const StepConsts& k, float4* grid, int update,
                        hipStream_t s) {
-  if (!in.tiles || !in.atomic || !in.dirty || !in.block_items || !grid) return hipErrorInvalidValue;
+  if (!in.tiles || !in.atomic || !in.dirty || !in.block_items || !in.covered || !grid)
+    return hipErrorInvalidValue;
   const int nbx = (k.N + 3) / 4;
   const int64_t nblocks = (int64_t)nbx * nbx * nbx;
   hipLaunchKernelGGL(k_tile_reduce, dim3((unsigned)((nblocks + 3) / 4)), dim3(256), 0, s, in.tiles,
-                     in.block_items, in.atomic, in.dirty, grid, k, nbx, update);
+                     in.block_items, in.covered, in.atomic, in.dirty, grid, k, nbx, update);
   return hipGetLastError();
 }
 

@@ -95,7 +95,8 @@ __global__ void __launch_bounds__(256)
     k_item_emit(const uint32_t* __restrict__ keys, int64_t n, int shift,
                 const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
                 const uint32_t* __restrict__ total, uint32_t* __restrict__ item_start,
-                uint32_t* __restrict__ item_block, uint32_t* __restrict__ block_items) {
+                uint32_t* __restrict__ item_block, uint32_t* __restrict__ block_items,
+                uint32_t* __restrict__ covered, int nbx) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     const uint32_t b = keys[i] >> shift;
@@ -103,7 +104,19 @@ __global__ void __launch_bounds__(256)
       item_start[pos[i]] = (uint32_t)i;
       item_block[pos[i]] = b;
     }
-    if (i == 0 || (keys[i - 1] >> shift) != b) block_items[2 * (size_t)b] = pos[i];
+    if (i == 0 || (keys[i - 1] >> shift) != b) {
+      block_items[2 * (size_t)b] = pos[i];
+      // the node blocks this home block's tiles can reach
+      const int hx = (int)mpmdev::compact3(b >> 2), hy = (int)mpmdev::compact3(b >> 1),
+                hz = (int)mpmdev::compact3(b);
+      for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dz = -1; dz <= 1; ++dz) {
+            const int x = hx + dx, y = hy + dy, z = hz + dz;
+            if (x >= 0 && x < nbx && y >= 0 && y < nbx && z >= 0 && z < nbx)
+              covered[((size_t)x * nbx + y) * nbx + z] = 1u;
+          }
+    }
     if (i == n - 1 || (keys[i + 1] >> shift) != b) block_items[2 * (size_t)b + 1] = pos[i] + flag[i];
   }
   if (i == 0) item_start[*total] = (uint32_t)n;  // sentinel: end of the last item
@@ -500,22 +513,24 @@ hipError_t fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
 }
 
 hipError_t build_items(const uint32_t* keys, const uint32_t* perm, int64_t n, int lb, int chunk,
-                       int64_t nblk, uint32_t* block_range, uint32_t* block_items, uint32_t* flag,
-                       uint32_t* pos, uint32_t* scratch, uint32_t* total, uint32_t* item_start,
-                       uint32_t* item_block, uint32_t* item_waves, uint32_t* gather,
-                       int64_t max_items, hipStream_t s) {
+                       int64_t nblk, uint32_t* block_range, uint32_t* block_items, uint32_t* covered,
+                       int nbx, uint32_t* flag, uint32_t* pos, uint32_t* scratch, uint32_t* total,
+                       uint32_t* item_start, uint32_t* item_block, uint32_t* item_waves,
+                       uint32_t* gather, int64_t max_items, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   if (chunk < 1 || chunk > kMaxItemParticles || max_items < 1) return hipErrorInvalidValue;
   const int shift = 3 * lb;
   const unsigned g = (unsigned)((n + 255) / 256);
+  const int64_t nnb = (int64_t)nbx * nbx * nbx;
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((2 * nblk + 255) / 256)), dim3(256), 0, s,
                      block_items, 2 * nblk, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nnb + 255) / 256)), dim3(256), 0, s, covered, nnb, 0u);
   hipLaunchKernelGGL(k_block_heads, dim3(g), dim3(256), 0, s, keys, n, shift, block_range);
   hipLaunchKernelGGL(k_item_flags, dim3(g), dim3(256), 0, s, keys, n, shift, chunk, block_range, flag);
   hipError_t e = exclusive_scan(flag, pos, n, scratch, total, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_item_emit, dim3(g), dim3(256), 0, s, keys, n, shift, flag, pos, total,
-                     item_start, item_block, block_items);
+                     item_start, item_block, block_items, covered, nbx);
   if (!perm || !gather || !item_waves) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_item_arrange, dim3((unsigned)max_items), dim3(256), 0, s, keys, perm, shift,
                      item_start, total, gather, item_waves);
