@@ -166,7 +166,7 @@ int mpm_load_state(mpm_handle h, const char* path);
  *                per node in particle order (the reference's scheme)
  *   "fused"      1 (default): G2P of substep n fused with P2G of substep n+1;
  *                0: the reference's order P2G, grid update, G2P
- *   "sort_every" re-sort the particles by cell every k substeps (default 8)
+ *   "sort_every" re-sort the particles by cell every k substeps (default 16)
  *   "chunk"      particles per work item, 64..192 in steps of 64 (0 = 192)
  *   "radix_bits" digit width of the radix sort: 4, 6 or 8 (default 8)
  *   "block"      home block edge in cells: 4 (0 = default 4)

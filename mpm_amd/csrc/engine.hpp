@@ -23,7 +23,7 @@ MaterialDev make_material_dev(const mpm_material& m, const StepConsts& k);
 struct Options {
   int64_t fused = 1;        // "fused": G2P of step n fused with P2G of step n+1
   int64_t p2g_mode = 1;     // "p2g_mode": 0 direct atomics, 1 sorted + LDS tile
-  int64_t sort_every = 8;   // "sort_every": re-sort particles every k substeps (0: never)
+  int64_t sort_every = 16;  // "sort_every": re-sort particles every k substeps (0: never)
   int64_t radix_bits = 8;   // "radix_bits": digit width of the radix sort (4, 6 or 8)
   int64_t chunk = 0;        // "chunk": max particles per work item (64..192, multiple of 64; 0 = 192)
   int64_t block = 0;        // "block": edge of a home block in cells (4; 0 = default)
