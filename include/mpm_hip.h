@@ -208,6 +208,30 @@ int mpm_device_info(char* buf, int32_t len);
  * the status of each into buf. No reference counterpart; for bring-up logs. */
 int mpm_debug_selftest(char* buf, int32_t len);
 
+/* ---- batches of independent scenes (no reference counterpart: the
+ *      reference runs one Simulation per process, src/main.cu:96-101) ----
+ * A batch advances several handles in lock step. Their substeps are recorded
+ * once into a HIP graph -- every scene's kernels on one of a few streams
+ * forked from the batch's own stream, so independent scenes run concurrently
+ * -- and replayed with one graph launch per cycle of 2 sort intervals. The
+ * results equal advancing each handle on its own, bit for bit. Handles stay
+ * owned by the caller and must outlive the batch; calls on a handle between
+ * batch advances are allowed (a changed scene is recorded again). */
+typedef struct mpm_batch_s* mpm_batch;
+int mpm_batch_create(mpm_batch* out);
+int mpm_batch_destroy(mpm_batch b);
+const char* mpm_batch_last_error(mpm_batch b);
+/* Append a handle (uploaded, with its materials set). */
+int mpm_batch_add(mpm_batch b, mpm_handle h);
+int mpm_batch_count(mpm_batch b, int32_t* out);
+/* Advance every handle by `substeps`, ordered after earlier work on `stream`;
+ * returns without waiting (later work on `stream` is ordered after it). */
+int mpm_batch_advance(mpm_batch b, int32_t substeps, void* stream);
+/* "graph" (1: record and replay, 0: launch kernel by kernel), "streams"
+ * (1..16, default 4); read-only "graph_launches". */
+int mpm_batch_set_option(mpm_batch b, const char* name, int64_t value);
+int mpm_batch_get_option(mpm_batch b, const char* name, int64_t* value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,10 +7,10 @@ generates seeded workloads. There is no CPU fallback.
 from ._lib import (BOUNDARY_CELLS, BOUNDARY_FRACTION, COMPAT_FIXED, COMPAT_REFERENCE,
                    MATERIAL_FIXED_COROTATED, MATERIAL_JELLY, MATERIAL_SNOW, MATERIAL_WC_LIQUID,
                    LibraryNotBuilt, MPMError, lib, mpm_material, mpm_params)
-from .engine import Engine, Params, device_svd, make_material, material_as_numpy
+from .engine import Batch, Engine, Params, device_svd, make_material, material_as_numpy
 
 __all__ = [
-    "Engine", "Params", "make_material", "material_as_numpy", "device_svd", "lib", "MPMError",
+    "Batch", "Engine", "Params", "make_material", "material_as_numpy", "device_svd", "lib", "MPMError",
     "LibraryNotBuilt", "mpm_material", "mpm_params", "MATERIAL_SNOW", "MATERIAL_FIXED_COROTATED",
     "MATERIAL_JELLY", "MATERIAL_WC_LIQUID", "BOUNDARY_FRACTION", "BOUNDARY_CELLS",
     "COMPAT_REFERENCE", "COMPAT_FIXED",

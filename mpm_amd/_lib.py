@@ -114,6 +114,14 @@ SIGNATURES = {
     "mpm_debug_svd": (C.c_int, [C.c_int64, _f, _f, _f, _f]),
     "mpm_device_info": (C.c_int, [C.c_char_p, C.c_int32]),
     "mpm_debug_selftest": (C.c_int, [C.c_char_p, C.c_int32]),
+    "mpm_batch_create": (C.c_int, [C.POINTER(_P)]),
+    "mpm_batch_destroy": (C.c_int, [_P]),
+    "mpm_batch_last_error": (C.c_char_p, [_P]),
+    "mpm_batch_add": (C.c_int, [_P, _P]),
+    "mpm_batch_count": (C.c_int, [_P, C.POINTER(C.c_int32)]),
+    "mpm_batch_advance": (C.c_int, [_P, C.c_int32, _P]),
+    "mpm_batch_set_option": (C.c_int, [_P, C.c_char_p, C.c_int64]),
+    "mpm_batch_get_option": (C.c_int, [_P, C.c_char_p, _i64p]),
 }
 
 

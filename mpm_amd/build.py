@@ -37,7 +37,8 @@ COMMON = [
     "-I", str(PKG.parent / "include"),
 ]
 
-SOURCES = ["capi.cpp", "engine.cpp", "substep.hip", "p2g_block.hip", "sort.hip", "linalg.hip"]
+SOURCES = ["capi.cpp", "engine.cpp", "batch.cpp", "substep.hip", "p2g_block.hip", "sort.hip",
+           "linalg.hip"]
 
 
 def _compile(src: Path) -> Path:

@@ -217,6 +217,7 @@ int Sim::ensure_grids() {
 }
 
 int Sim::set_materials(const mpm_material* m, int32_t n) {
+  ++gen_;
   if (n < 0 || n > MPM_MAX_MATERIALS || (n > 0 && !m))
     return fail(MPM_ERR_INVALID, "mpm_set_materials: need 0..%d materials", MPM_MAX_MATERIALS);
   for (int32_t i = 0; i < n; ++i) {
@@ -360,6 +361,7 @@ ParticlesDev Sim::particles() const {
 }
 
 int Sim::upload(hipStream_t s) {
+  ++gen_;
   const int64_t n = (int64_t)hJp_.size();
   for (int64_t i = 0; i < n; ++i)
     if (hmat_[i] >= mats_.size())
@@ -677,6 +679,7 @@ int Sim::download_grid(float* grid, hipStream_t s) {
 }
 
 int Sim::debug_p2g(float* grid, hipStream_t s) {
+  ++gen_;
   if (!uploaded_) return fail(MPM_ERR_STATE, "mpm_debug_p2g: nothing uploaded");
   if (!grid) return fail(MPM_ERR_INVALID, "mpm_debug_p2g: null output");
   int rc = check_materials_cover_particles("mpm_debug_p2g");
@@ -750,6 +753,7 @@ int Sim::save_state(const char* path) {
 }
 
 int Sim::load_state(const char* path) {
+  ++gen_;
   if (!path) return fail(MPM_ERR_INVALID, "mpm_load_state: null path");
   FILE* f = std::fopen(path, "rb");
   if (!f) return fail(MPM_ERR_IO, "mpm_load_state: cannot open %s", path);
@@ -792,6 +796,7 @@ int Sim::load_state(const char* path) {
 }
 
 int Sim::set_option(const char* name, int64_t v) {
+  ++gen_;
   if (!name) return fail(MPM_ERR_INVALID, "mpm_set_option: null name");
   const std::string n = name;
   if (n == "fused") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "fused must be 0 or 1"); opt_.fused = v; primed_ = false; return MPM_OK; }

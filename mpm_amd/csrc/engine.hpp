@@ -65,6 +65,22 @@ class Sim {
   int timing_read(const char* phase, double* total_ms, int64_t* launches);
   int timing_reset();
 
+  // ---- replay of captured substeps (batch.cpp) ----
+  // A capture cycle is 2 sort intervals: it starts when the next substep sorts
+  // and ends in the same state with the same particle buffer current.
+  int cycle_substeps() const { return 2 * (int)opt_.sort_every; }
+  bool at_cycle_start() const {
+    return uploaded_ && !mats_dirty_ && primed_ && items_valid_ && !pending_gather_ &&
+           opt_.p2g_mode == 1 && opt_.fused == 1 && opt_.sort_every > 0 && !opt_.timing &&
+           steps_since_sort_ == opt_.sort_every;
+  }
+  // identifies the buffers and settings a captured cycle was recorded with
+  uint64_t capture_signature() const { return (gen_ << 1) | (uint64_t)cur_; }
+  // host-side bookkeeping of `substeps` substeps that ran from a replay
+  void replayed(int32_t substeps) {
+    for (int32_t i = 0; i < substeps; ++i) t_ += (double)prm_.dt;
+  }
+
   // device views for other entry points of the library
   ParticlesDev particles() const;
   mpmk::WorkItems work_items() const;
@@ -157,6 +173,7 @@ class Sim {
   int64_t steps_since_sort_ = 1 << 30;
 
   double t_ = 0.0;
+  uint64_t gen_ = 0;  // bumped by every change of settings, materials or particles
 
   struct EventPair { int phase; hipEvent_t a, b; };
   std::vector<EventPair> ev_;      // recorded, not yet read

@@ -225,6 +225,50 @@ class Engine:
         self._n = self.count()
 
 
+class Batch:
+    """Independent scenes advanced together (mpm_batch_*): their substeps are
+    recorded into one HIP graph and replayed. The engines stay owned by the
+    caller and must outlive the batch."""
+
+    def __init__(self, engines: Sequence["Engine"] = ()):
+        self._b = C.c_void_p()
+        check(lib().mpm_batch_create(C.byref(self._b)), None, "mpm_batch_create")
+        self._engines = []
+        for e in engines:
+            self.add(e)
+
+    def _chk(self, rc: int, what: str = "") -> None:
+        if rc != 0:
+            msg = lib().mpm_batch_last_error(self._b)
+            raise _lib.MPMError(rc, (msg.decode() if msg else "") or what)
+
+    def add(self, engine: "Engine") -> None:
+        self._chk(lib().mpm_batch_add(self._b, engine.handle), "mpm_batch_add")
+        self._engines.append(engine)
+
+    def advance(self, substeps: int = 1, stream=None) -> None:
+        self._chk(lib().mpm_batch_advance(self._b, int(substeps), _stream(stream)))
+
+    def set_option(self, name: str, value: int) -> None:
+        self._chk(lib().mpm_batch_set_option(self._b, name.encode(), int(value)))
+
+    def get_option(self, name: str) -> int:
+        v = C.c_int64()
+        self._chk(lib().mpm_batch_get_option(self._b, name.encode(), C.byref(v)))
+        return v.value
+
+    def close(self) -> None:
+        if self._b:
+            lib().mpm_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def device_svd(A: np.ndarray):
     """Batched device 3x3 SVD (the engine's routine). A: (n,3,3) float32."""
     A = np.ascontiguousarray(A, dtype=np.float32).reshape(-1, 3, 3)

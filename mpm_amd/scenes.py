@@ -208,6 +208,46 @@ def rubber_elastic(n: int, seed: int, N: int = 16, compat=_lib.COMPAT_REFERENCE,
     return Scene("rubber_ball", p, [mat], x, np.zeros(n, np.uint8), v=v)
 
 
+def batched_scenes(count: int = 64, seed: int = 4, scale: float = 1.0) -> List[Scene]:
+    """The batched workload: `count` independent scenes. Per scene: particle
+    count log-uniform in [5e3, 2e5] (times `scale`), N from {32, 48, 64},
+    material 1/3 each elastic (E log-uniform [5e3, 5e4], nu=0.3, rho=1000,
+    snow model with hardening 0 and no clamps), snow (as snow_spheres) and
+    weakly compressible liquid (K=4e4, rho=1000); a ball or a box with a random
+    centre and a size (radius or half edge) in [0.1, 0.25]; dt=1e-4, gravity
+    -9.8, walls 0.05."""
+    s = Sampler(seed)
+    out = []
+    for j in range(count):
+        u = s.uniform(8).astype(np.float64)
+        n = max(64, int(round(math.exp(math.log(5e3) + u[0] * (math.log(2e5) - math.log(5e3))) * scale)))
+        N = (32, 48, 64)[min(2, int(u[1] * 3))]
+        kind = min(2, int(u[2] * 3))
+        size = 0.1 + u[3] * 0.15
+        center = tuple(0.5 + (uu - 0.5) * (0.8 - 2 * size) * 0.5 for uu in u[4:7])
+        if u[7] < 0.5:
+            x = sample_ball(s, n, center, size)
+            volume = 4.0 / 3.0 * math.pi * size ** 3
+        else:
+            c = np.asarray(center)
+            x = sample_box(s, n, c - size, c + size)
+            volume = (2 * size) ** 3
+        vol = volume / n
+        if kind == 0:
+            E = math.exp(math.log(5e3) + float(s.uniform(1)[0]) * (math.log(5e4) - math.log(5e3)))
+            mat = material_as_numpy(_lib.MATERIAL_SNOW, vol, 1000.0, E=E, nu=0.3, hardening=0.0,
+                                    clamp_lo=0.0, clamp_hi=1e30)
+        elif kind == 1:
+            mat = material_as_numpy(_lib.MATERIAL_SNOW, vol, 400.0, E=1.4e5, nu=0.2, hardening=10.0,
+                                    clamp_lo=1.0 - 2.5e-2, clamp_hi=1.0 + 7.5e-3)
+        else:
+            mat = material_as_numpy(_lib.MATERIAL_WC_LIQUID, vol, 1000.0, E=0.0, nu=0.0,
+                                    hardening=0.0, clamp_lo=0.0, clamp_hi=0.0, bulk_modulus=4e4)
+        p = Params(dt=1e-4, N=N, gravity=-9.8)
+        out.append(Scene(f"batch{j}_k{kind}_N{N}", p, [mat], x, np.zeros(n, np.uint8)))
+    return out
+
+
 def algorithmic_bytes(P: int, N: int) -> int:
     """Minimal field traffic of one substep (dense FP32 grid): 254 B per
     particle (P2G reads x,v,C,F,Jp,mat = 101 B; G2P reads x,F,Jp,mat = 53 B and
