@@ -15,6 +15,11 @@ dam break's length. Prints one JSON line:
                launch stream during the timed region), against 8 TB/s HBM
   cpu_baseline the FP32 CPU oracle (a C++/OpenMP restatement of the
                reference step) on a bounded sample of the same workload
+  batched      64 independent scenes advanced one after the other vs. as one
+               batch (HIP graph replay): seconds and speedup
+  adjoint      the differentiable elastic cube (32,768 particles, N=32, 200
+               substeps, checkpoint every 20): forward and backward wall time
+               and their ratio
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -81,6 +86,99 @@ def cpu_baseline(scene, steps: int, threads: int):
     }
 
 
+def batched_leg(warm: int, steps: int):
+    """The batched workload (64 independent scenes, mpm_amd.scenes.batched_scenes):
+    seconds for `steps` substeps of every scene advanced one scene after the
+    other (each handle's own mpm_advance), against the same substeps through
+    one batch (HIP graph replay). Both sets are warmed by `warm` substeps."""
+    import torch
+    from mpm_amd import Batch, Engine, scenes
+
+    def make(sc):
+        st = sc.state()
+        e = Engine(sc.params)
+        e.set_materials(sc.materials)
+        e.add_particles(st["x"], st["material"], st["v"], st["C"], st["F"], st["Jp"])
+        e.upload()
+        return e
+
+    scs = scenes.batched_scenes()
+    seq = [make(sc) for sc in scs]
+    for e in seq:
+        e.advance(warm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e in seq:
+        e.advance(steps)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    for e in seq:
+        e.close()
+    bat = [make(sc) for sc in scs]
+    b = Batch(bat)
+    b.advance(warm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.advance(steps)
+    torch.cuda.synchronize()
+    t_bat = time.perf_counter() - t0
+    launches = b.get_option("graph_launches")
+    b.close()
+    for e in bat:
+        e.close()
+    P = sum(sc.n for sc in scs)
+    return {
+        "workload": f"{len(scs)} independent scenes, {P} particles, N in {{32,48,64}}",
+        "substeps": steps, "warmup": warm,
+        "sequential_s": t_seq, "batched_s": t_bat, "speedup": t_seq / t_bat,
+        "batched_particle_substeps_per_s": P * steps / t_bat,
+        "graph_launches": launches,
+    }
+
+
+def adjoint_leg(repeats: int = 3):
+    """The differentiable elastic cube (mpm_amd.scenes.diff_cube: 32,768
+    particles, N=32, 200 substeps, checkpoint every 20): wall time of the
+    forward rollout and of the backward (segment recompute + adjoint kernels),
+    best of `repeats`."""
+    import torch
+    from mpm_amd import scenes
+    from mpm_amd.diff import DiffStepper, lame_from_logE, pack_state, rollout
+
+    c = scenes.diff_cube()
+    n = c.x.shape[0]
+    dev = torch.device("cuda")
+    stp = DiffStepper(c.params, c.particle_mass, c.particle_volume, n)
+    x = torch.from_numpy(c.x).to(dev)
+    target = torch.tensor(c.target, dtype=torch.float32, device=dev)
+    best = None
+    for r in range(repeats + 1):  # the first round warms up
+        v0 = torch.zeros(3, device=dev, requires_grad=True)
+        logE = torch.from_numpy(c.logE).to(dev).requires_grad_(True)
+        mu, lam = lame_from_logE(logE, c.nu)
+        st0 = pack_state(x, v0.expand(n, 3))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = rollout(stp, st0, mu, lam, c.substeps, 20)
+        loss = ((st[0:3].mean(1) - target) ** 2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if r > 0 and (best is None or (t2 - t1) / (t1 - t0) < best[1] / best[0]):
+            best = (t1 - t0, t2 - t1)
+    fwd, bwd = best
+    return {
+        "workload": f"differentiable elastic cube, {n} particles, N={c.params.N}, "
+                    f"{c.substeps} substeps, checkpoint every 20",
+        "forward_ms": fwd * 1e3, "backward_ms": bwd * 1e3,
+        "forward_us_per_substep": fwd * 1e6 / c.substeps,
+        "backward_us_per_substep": bwd * 1e6 / c.substeps,
+        "bwd_fwd_ratio": bwd / fwd,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +187,9 @@ def main() -> int:
     ap.add_argument("--workload", default="dam_break", choices=["dam_break", "snow_spheres", "elastic_sphere"])
     ap.add_argument("--cpu-steps", type=int, default=80)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=128)
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -136,7 +237,7 @@ def main() -> int:
     kernel_name = {"g2p2g": "k_g2p2g_tiled", "grid_update": "k_tile_reduce"}.get(dom, dom)
 
     result = {
-        "metric": "particle-substeps/s at 1M particles N=128",
+        "metric": "particle-substeps/s at 1M particles N=128; batched speedup; bwd/fwd ratio",
         "value": P / sec_per_step,
         "unit": "particle-substeps/s",
         "n_gpus": 1,
@@ -170,7 +271,15 @@ def main() -> int:
             "traffic": load_traffic(scene.name, kernel_name),
         },
         "cpu_baseline": None,
+        "batched": None,
+        "adjoint": None,
     }
+    if not args.no_batched:
+        result["batched"] = batched_leg(48, args.batched_steps)
+        print(f"batched: {result['batched']}", file=sys.stderr, flush=True)
+    if not args.no_adjoint:
+        result["adjoint"] = adjoint_leg()
+        print(f"adjoint: {result['adjoint']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         result["cpu_baseline"] = cpu_baseline(scene, args.cpu_steps, threads)

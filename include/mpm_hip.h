@@ -232,6 +232,32 @@ int mpm_batch_advance(mpm_batch b, int32_t substeps, void* stream);
 int mpm_batch_set_option(mpm_batch b, const char* name, int64_t value);
 int mpm_batch_get_option(mpm_batch b, const char* name, int64_t* value);
 
+/* ---- differentiable substep (no reference counterpart: the reference has no
+ *      gradients) ----
+ * One substep of the fixed-corotated model (MMFixedCorotated,
+ * MaterialModel.cuh:56-61, with J = det F) under compat FIXED, with
+ * per-particle Lame parameters mu[n], lambda[n], one particle mass and volume,
+ * and no plasticity; and its adjoint. Stateless: every buffer is a device
+ * pointer the caller owns. State is plane-major, 24 planes of n floats:
+ * x0 x1 x2, v0 v1 v2, C (9, row-major), F (9, row-major); plane q of particle
+ * p at state[q*n + p]. params->compat must be MPM_COMPAT_FIXED.
+ * Scratch sizes (floats) come from mpm_diff_sizes: `grid` (4 N^3) is scratch
+ * of one call; `tape` (8 N^3) is written by the forward and read by the
+ * backward of the same substep. */
+int mpm_diff_sizes(const mpm_params* params, int64_t* grid_floats, int64_t* tape_floats);
+/* state_out = substep(state); state_out must not alias state. */
+int mpm_diff_substep(const mpm_params* params, float particle_mass, float particle_volume,
+                     int64_t n, const float* state, float* state_out, const float* mu,
+                     const float* lambda, float* grid, float* tape, void* stream);
+/* Given adj_out = dL/d(state_out) of the substep that read `state` (and wrote
+ * `tape`), writes adj_in = dL/d(state) and ADDS dL/dmu, dL/dlambda into
+ * adj_mu[n], adj_lambda[n]. adj_in must not alias adj_out. */
+int mpm_diff_substep_backward(const mpm_params* params, float particle_mass,
+                              float particle_volume, int64_t n, const float* state,
+                              const float* mu, const float* lambda, const float* tape,
+                              const float* adj_out, float* adj_in, float* adj_mu,
+                              float* adj_lambda, float* grid, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

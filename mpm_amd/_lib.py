@@ -122,6 +122,12 @@ SIGNATURES = {
     "mpm_batch_advance": (C.c_int, [_P, C.c_int32, _P]),
     "mpm_batch_set_option": (C.c_int, [_P, C.c_char_p, C.c_int64]),
     "mpm_batch_get_option": (C.c_int, [_P, C.c_char_p, _i64p]),
+    "mpm_diff_sizes": (C.c_int, [C.POINTER(mpm_params), _i64p, _i64p]),
+    # device pointers travel as void*
+    "mpm_diff_substep": (C.c_int, [C.POINTER(mpm_params), C.c_float, C.c_float, C.c_int64,
+                                   _P, _P, _P, _P, _P, _P, _P]),
+    "mpm_diff_substep_backward": (C.c_int, [C.POINTER(mpm_params), C.c_float, C.c_float,
+                                            C.c_int64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 

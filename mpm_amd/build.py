@@ -38,7 +38,7 @@ COMMON = [
 ]
 
 SOURCES = ["capi.cpp", "engine.cpp", "batch.cpp", "substep.hip", "p2g_block.hip", "sort.hip",
-           "linalg.hip"]
+           "linalg.hip", "adjoint.hip"]
 
 
 def _compile(src: Path) -> Path:

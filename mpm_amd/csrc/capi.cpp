@@ -333,3 +333,95 @@ extern "C" int mpm_debug_selftest(char* buf, int32_t len) {
   std::snprintf(buf, (size_t)len, "%s", out.c_str());
   return MPM_OK;
 }
+
+// ---------------------------------------------------------------------------
+// differentiable substep (adjoint.hip)
+// ---------------------------------------------------------------------------
+namespace {
+
+int diff_check(const char* fn, const mpm_params* p, float mass, float volume, int64_t n,
+               StepConsts* k) {
+  if (!p) {
+    set_global_error("%s: null params", fn);
+    return MPM_ERR_INVALID;
+  }
+  std::string why;
+  if (!mpm_engine::validate_params(*p, &why)) {
+    set_global_error("%s: %s", fn, why.c_str());
+    return MPM_ERR_INVALID;
+  }
+  if (p->compat != MPM_COMPAT_FIXED) {
+    set_global_error("%s: the differentiable substep implements compat FIXED only", fn);
+    return MPM_ERR_INVALID;
+  }
+  if (n < 0 || !(mass > 0.0f) || !(volume > 0.0f) || !std::isfinite(mass) || !std::isfinite(volume)) {
+    set_global_error("%s: need n >= 0 and finite positive particle mass and volume", fn);
+    return MPM_ERR_INVALID;
+  }
+  *k = mpm_engine::make_consts(*p);
+  return MPM_OK;
+}
+
+}  // namespace
+
+extern "C" int mpm_diff_sizes(const mpm_params* params, int64_t* grid_floats, int64_t* tape_floats) {
+  std::string why;
+  if (!params || !grid_floats || !tape_floats || !mpm_engine::validate_params(*params, &why)) {
+    set_global_error("mpm_diff_sizes: bad arguments %s", why.c_str());
+    return MPM_ERR_INVALID;
+  }
+  const int64_t nodes = (int64_t)params->N * params->N * params->N;
+  *grid_floats = 4 * nodes;
+  *tape_floats = 8 * nodes;
+  return MPM_OK;
+}
+
+extern "C" int mpm_diff_substep(const mpm_params* params, float particle_mass, float particle_volume,
+                                int64_t n, const float* state, float* state_out, const float* mu,
+                                const float* lambda, float* grid, float* tape, void* stream) {
+  StepConsts k;
+  int rc = diff_check(__func__, params, particle_mass, particle_volume, n, &k);
+  if (rc) return rc;
+  if (!grid || !tape || (n > 0 && (!state || !state_out || !mu || !lambda))) {
+    set_global_error("mpm_diff_substep: null buffer");
+    return MPM_ERR_INVALID;
+  }
+  if (n > 0 && state == state_out) {
+    set_global_error("mpm_diff_substep: state_out must not alias state");
+    return MPM_ERR_INVALID;
+  }
+  const hipError_t e = mpmk::diff_forward(k, particle_mass, particle_volume, n, state, state_out, mu,
+                                          lambda, (float4*)grid, tape, (hipStream_t)stream);
+  if (e != hipSuccess) {
+    set_global_error("mpm_diff_substep: %s", hipGetErrorString(e));
+    return MPM_ERR_HIP;
+  }
+  return MPM_OK;
+}
+
+extern "C" int mpm_diff_substep_backward(const mpm_params* params, float particle_mass,
+                                         float particle_volume, int64_t n, const float* state,
+                                         const float* mu, const float* lambda, const float* tape,
+                                         const float* adj_out, float* adj_in, float* adj_mu,
+                                         float* adj_lambda, float* grid, void* stream) {
+  StepConsts k;
+  int rc = diff_check(__func__, params, particle_mass, particle_volume, n, &k);
+  if (rc) return rc;
+  if (!grid || !tape ||
+      (n > 0 && (!state || !mu || !lambda || !adj_out || !adj_in || !adj_mu || !adj_lambda))) {
+    set_global_error("mpm_diff_substep_backward: null buffer");
+    return MPM_ERR_INVALID;
+  }
+  if (n > 0 && adj_in == adj_out) {
+    set_global_error("mpm_diff_substep_backward: adj_in must not alias adj_out");
+    return MPM_ERR_INVALID;
+  }
+  const hipError_t e = mpmk::diff_backward(k, particle_mass, particle_volume, n, state, mu, lambda,
+                                           tape, adj_out, adj_in, adj_mu, adj_lambda, (float4*)grid,
+                                           (hipStream_t)stream);
+  if (e != hipSuccess) {
+    set_global_error("mpm_diff_substep_backward: %s", hipGetErrorString(e));
+    return MPM_ERR_HIP;
+  }
+  return MPM_OK;
+}

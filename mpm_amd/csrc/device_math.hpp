@@ -69,16 +69,11 @@ __device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) 
 // (three sequential divides; mass becomes 1), gravity on y, then the walls.
 // Nodes without mass are left as they are.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float4 grid_node_update(float4 c, int xi, int yi, int zi,
-                                                   const StepConsts& k) {
-  if (!(c.w > 0.0f)) return c;
-  c.x = c.x / c.w;
-  c.y = c.y / c.w;
-  c.z = c.z / c.w;
-  c.w = 1.0f;
-  c.y += k.dt * k.gravity;
+// The walls of node (xi, yi, zi): sticky (all velocity removed) on five
+// sides, a separating floor (no downward velocity) at the bottom.
+__device__ __forceinline__ void wall_flags(int xi, int yi, int zi, const StepConsts& k,
+                                           bool& sticky, bool& floor_) {
   const int N = k.N;
-  bool sticky, floor_;
   if (k.boundary_mode == MPM_BOUNDARY_CELLS) {
     const int b = k.bcells;
     sticky = xi < b || xi >= N - b || yi >= N - b || zi < b || zi >= N - b;
@@ -89,6 +84,18 @@ __device__ __forceinline__ float4 grid_node_update(float4 c, int xi, int yi, int
     sticky = xf < b || xf > hi || yf > hi || zf < b || zf > hi;
     floor_ = yf < b;
   }
+}
+
+__device__ __forceinline__ float4 grid_node_update(float4 c, int xi, int yi, int zi,
+                                                   const StepConsts& k) {
+  if (!(c.w > 0.0f)) return c;
+  c.x = c.x / c.w;
+  c.y = c.y / c.w;
+  c.z = c.z / c.w;
+  c.w = 1.0f;
+  c.y += k.dt * k.gravity;
+  bool sticky, floor_;
+  wall_flags(xi, yi, zi, k, sticky, floor_);
   if (sticky) { c.x = 0.f; c.y = 0.f; c.z = 0.f; }
   if (floor_) c.y = fmaxf(0.0f, c.y);
   return c;

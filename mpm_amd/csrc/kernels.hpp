@@ -94,4 +94,14 @@ hipError_t g2p2g_block(const ParticlesDev& P, const ParticlesDev& Pdst, const ui
 hipError_t tile_reduce(const TileSums& in, const StepConsts& k, float4* grid, int update,
                        hipStream_t s);
 
+// adjoint.hip: the differentiable substep (24 state planes of n floats) and
+// its adjoint; grid is 4 N^3 floats of scratch, tape 8 N^3 floats.
+hipError_t diff_forward(const StepConsts& k, float mass, float volume, int64_t n, const float* st,
+                        float* out, const float* mu, const float* lam, float4* grid, float* tape,
+                        hipStream_t s);
+hipError_t diff_backward(const StepConsts& k, float mass, float volume, int64_t n, const float* st,
+                         const float* mu, const float* lam, const float* tape, const float* adj_out,
+                         float* adj_in, float* adj_mu, float* adj_lam, float4* grid_adj,
+                         hipStream_t s);
+
 }  // namespace mpmk

@@ -149,27 +149,68 @@ def dam_break(n: int = 1_000_000, seed: int = 3, N: int = 128, compat=_lib.COMPA
     return Scene("dam_break", p, [mat], x, np.zeros(n, np.uint8), substeps=2000)
 
 
+def _jittered_lattice(s: Sampler, side: int, lo, h: float):
+    """side^3 lattice of spacing h from corner `lo` (cell centres), each point
+    moved by up to +-h/4 per axis; float64."""
+    n = side ** 3
+    g = [(np.arange(side, dtype=np.float64) + 0.5) * h + lo[a] for a in range(3)]
+    X, Y, Z = np.meshgrid(*g, indexing="ij")
+    x = np.stack([X, Y, Z], -1).reshape(-1, 3)
+    jit = (s.uniform(3 * n).astype(np.float64).reshape(n, 3) - 0.5) * 0.5 * h
+    return x + jit
+
+
+def _log_e(s: Sampler, n: int, noise: float) -> np.ndarray:
+    """log E = log 1e4 + noise * N(0,1) per particle (Box-Muller on the stream)."""
+    u1 = s.uniform(n).astype(np.float64)
+    u2 = s.uniform(n).astype(np.float64)
+    z = np.sqrt(-2.0 * np.log(1.0 - u1)) * np.cos(2.0 * math.pi * u2)
+    return (math.log(1e4) + noise * z).astype(np.float32)
+
+
 def elastic_cube(side: int = 32, seed: int = 5, N: int = 32, logE_noise: float = 0.1,
                  compat=_lib.COMPAT_REFERENCE) -> Scene:
     s = Sampler(seed)
     n = side ** 3
     h = 0.3 / side
-    g = (np.arange(side, dtype=np.float64) + 0.5) * h + 0.35
-    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
-    x = np.stack([X, Y, Z], -1).reshape(-1, 3)
-    jit = (s.uniform(3 * n).astype(np.float64).reshape(n, 3) - 0.5) * 0.5 * h
-    x = (x + jit).astype(np.float32)
+    x = _jittered_lattice(s, side, (0.35, 0.35, 0.35), h).astype(np.float32)
     vol = 0.3 ** 3 / n
-    # per-particle log E = log 1e4 + noise * N(0,1) (Box-Muller on the stream)
-    u1 = s.uniform(n).astype(np.float64)
-    u2 = s.uniform(n).astype(np.float64)
-    z = np.sqrt(-2.0 * np.log(1.0 - u1)) * np.cos(2.0 * math.pi * u2)
-    logE = (math.log(1e4) + logE_noise * z).astype(np.float32)
+    logE = _log_e(s, n, logE_noise)
     mat = material_as_numpy(_lib.MATERIAL_SNOW, vol, 1000.0, E=1e4, nu=0.3, hardening=0.0,
                             clamp_lo=0.0, clamp_hi=1e30)
     p = Params(dt=1e-4, N=N, gravity=-9.8, compat=compat)
     return Scene("elastic_cube", p, [mat], x, np.zeros(n, np.uint8), substeps=200,
                  extra={"logE": logE, "target": (0.6, 0.4, 0.5)})
+
+
+@dataclass
+class DiffCube:
+    """A cube for the differentiable substep (fixed-corotated, compat FIXED,
+    per-particle log E)."""
+    params: Params
+    x: np.ndarray           # (n, 3) float32
+    logE: np.ndarray        # (n,) float32
+    nu: float
+    particle_volume: float
+    particle_mass: float
+    target: tuple
+    substeps: int
+
+
+def diff_cube(side: int = 32, seed: int = 5, N: int = 32, lo=(0.35, 0.35, 0.35),
+              spacing: float = 0.3 / 32, logE_noise: float = 0.1, substeps: int = 200) -> DiffCube:
+    """The differentiable elastic cube: a jittered side^3 lattice of the given
+    spacing from corner `lo` (defaults: 32^3 particles in [0.35, 0.65]^3),
+    N=32, dt=1e-4, rho=1000, nu=0.3, log E = log 1e4 + 0.1 N(0,1), target
+    (0.6, 0.4, 0.5). The particle volume is spacing^3."""
+    s = Sampler(seed)
+    n = side ** 3
+    x = _jittered_lattice(s, side, lo, spacing).astype(np.float32)
+    logE = _log_e(s, n, logE_noise)
+    vol = float(np.float32(spacing ** 3))
+    mass = float(np.float32(1000.0) * np.float32(vol))
+    p = Params(dt=1e-4, N=N, gravity=-9.8, compat=_lib.COMPAT_FIXED)
+    return DiffCube(p, x, logE, 0.3, vol, mass, (0.6, 0.4, 0.5), substeps)
 
 
 def small_scene(kind: int, n: int, seed: int, N: int = 16, compat=_lib.COMPAT_REFERENCE,

@@ -97,6 +97,25 @@ def advance(params, materials, state, steps: int, dtype=np.float32, threads: int
     return (s, grid) if return_grid else s
 
 
+def advance_lame(params, material, state, mu, lam, steps: int, dtype=np.float32):
+    """`steps` substeps with every particle of `material` but its mu0, lambda0
+    replaced per particle by mu[i], lam[i] (the differentiable step's
+    forward: fixed-corotated with compat FIXED). Returns the new state dict."""
+    s = _state_arrays(state, dtype)
+    n = s["x"].shape[0]
+    ct = C.c_float if dtype == np.float32 else C.c_double
+    fn = lib().oracle_advance_lame_f32 if dtype == np.float32 else lib().oracle_advance_lame_f64
+    lame = np.ascontiguousarray(np.stack([np.asarray(mu, dtype).reshape(n),
+                                          np.asarray(lam, dtype).reshape(n)], -1))
+    cp = _conv_params(params)
+    cm = _conv_mats([material])
+    rc = fn(C.byref(cp), cm, C.c_int64(n), _ptr(s["x"], ct), _ptr(s["v"], ct), _ptr(s["C"], ct),
+            _ptr(s["F"], ct), _ptr(s["Jp"], ct), _ptr(lame, ct), int(steps))
+    if rc != 0:
+        raise RuntimeError("oracle advance_lame rejected its arguments")
+    return s
+
+
 def p2g(params, materials, state, dtype=np.float32):
     s = _state_arrays(state, dtype)
     n = s["x"].shape[0]

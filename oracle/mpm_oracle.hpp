@@ -453,9 +453,11 @@ template <class S> inline void quad_weights(const S x[3], S dx_inv, int base[3],
 // MMSnow: MaterialModel.cuh:85-93; MMFixedCorotated: :56-61; MMJelly: :131-139.
 // The reference evaluates exp(hardening (1 - Jp)) and lambda ((J - 1) J) in
 // double and rounds to float; that is kept for S = float.
+// `lame` (mu, lambda), when given, replaces the material's mu0, lambda0 for
+// this particle (the differentiable step's per-particle parameters).
 template <class S>
 inline void material_stress(const Particle<S>& p, const mpm_material& m, int compat,
-                            const S R[3][3], S PF[3][3]) {
+                            const S R[3][3], S PF[3][3], const S* lame = nullptr) {
   if (m.kind == MPM_MATERIAL_WC_LIQUID) {
     const S J = p.Jp;
     const S pr = S(double(S(m.bulk_modulus)) * ((double(J) - 1.0) * double(J)));
@@ -463,11 +465,12 @@ inline void material_stress(const Particle<S>& p, const mpm_material& m, int com
       for (int c = 0; c < 3; ++c) PF[r][c] = (r == c) ? pr : S(0);
     return;
   }
-  S mu = S(m.mu0), lambda = S(m.lambda0);
+  const S mu0 = lame ? lame[0] : S(m.mu0), lambda0 = lame ? lame[1] : S(m.lambda0);
+  S mu = mu0, lambda = lambda0;
   if (m.kind != MPM_MATERIAL_FIXED_COROTATED) {
     const S e = S(std::exp(double(S(m.hardening)) * (1.0 - double(p.Jp))));
-    mu = S(m.mu0) * e;
-    lambda = S(m.lambda0) * e;
+    mu = mu0 * e;
+    lambda = lambda0 * e;
   }
   // volumetric J: the reference uses the plastic Jp; compat FIXED uses det F
   const S J = (compat == MPM_COMPAT_FIXED) ? det3<S>(p.F) : p.Jp;
@@ -486,11 +489,11 @@ inline void material_stress(const Particle<S>& p, const mpm_material& m, int com
 // TransferScheme.h:66-86): affine = -Dinv dt V PF + m C.
 template <class S>
 inline void p2g_affine(const Particle<S>& p, const mpm_material& m, const StepConsts<S>& k,
-                       S affine[3][3]) {
+                       S affine[3][3], const S* lame = nullptr) {
   S R[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
   if (m.kind != MPM_MATERIAL_WC_LIQUID) polar_R<S>(p.F, R);
   S PF[3][3];
-  material_stress<S>(p, m, k.compat, R, PF);
+  material_stress<S>(p, m, k.compat, R, PF, lame);
   // stress = ((-Dinv) * dt) * V * PF with diagonal Dinv
   const S d = ((-k.dinv) * k.dt) * S(m.particle_volume);
   const S mass = S(m.particle_mass);
@@ -501,7 +504,7 @@ inline void p2g_affine(const Particle<S>& p, const mpm_material& m, const StepCo
 // Grid stored as N^3 x 4 (momentum xyz, mass), index N*N*i + N*j + k.
 template <class S, class Add>
 inline void p2g_particle(const Particle<S>& p, const mpm_material& m, const StepConsts<S>& k,
-                         Add&& add) {
+                         Add&& add, const S* lame = nullptr) {
   const uint32_t N = k.N;
   int base[3];
   S w[3][3];
@@ -510,7 +513,7 @@ inline void p2g_particle(const Particle<S>& p, const mpm_material& m, const Step
   for (int a = 0; a < 3; ++a)
     if (base[a] + 3 < 0 || base[a] >= (int)N) return;
   S affine[3][3];
-  p2g_affine<S>(p, m, k, affine);
+  p2g_affine<S>(p, m, k, affine, lame);
   const S mass = S(m.particle_mass);
   int beg[3], end[3];
   for (int a = 0; a < 3; ++a) {
@@ -661,9 +664,12 @@ inline void g2p_particle(Particle<S>& p, const mpm_material& m, const StepConsts
 
 // One substep (Simulation::advance, mpm.cu:323-329). `grid` is scratch of
 // 4 N^3 scalars; after the call it holds the updated grid velocities.
+// `lame` (2 per particle: mu, lambda), when given, replaces the materials'
+// mu0, lambda0 per particle.
 template <class S>
 inline void step(std::vector<Particle<S>>& parts, const mpm_material* mats, int nmat,
-                 const StepConsts<S>& k, std::vector<S>& grid, int threads) {
+                 const StepConsts<S>& k, std::vector<S>& grid, int threads,
+                 const S* lame = nullptr) {
   const uint32_t N = k.N;
   const size_t nodes = (size_t)N * N * N;
   grid.assign(4 * nodes, S(0));
@@ -674,20 +680,26 @@ inline void step(std::vector<Particle<S>>& parts, const mpm_material* mats, int 
   if (threads > 1) {
 #pragma omp parallel for num_threads(threads) schedule(static)
     for (int64_t pi = 0; pi < P; ++pi) {
-      p2g_particle<S>(parts[pi], mats[parts[pi].mat], k, [&](uint64_t idx, const S c4[4]) {
-        for (int w = 0; w < 4; ++w) {
+      p2g_particle<S>(
+          parts[pi], mats[parts[pi].mat], k,
+          [&](uint64_t idx, const S c4[4]) {
+            for (int w = 0; w < 4; ++w) {
 #pragma omp atomic
-          g[4 * idx + w] += c4[w];
-        }
-      });
+              g[4 * idx + w] += c4[w];
+            }
+          },
+          lame ? lame + 2 * pi : nullptr);
     }
   } else
 #endif
   {
     for (int64_t pi = 0; pi < P; ++pi)
-      p2g_particle<S>(parts[pi], mats[parts[pi].mat], k, [&](uint64_t idx, const S c4[4]) {
-        for (int w = 0; w < 4; ++w) g[4 * idx + w] += c4[w];
-      });
+      p2g_particle<S>(
+          parts[pi], mats[parts[pi].mat], k,
+          [&](uint64_t idx, const S c4[4]) {
+            for (int w = 0; w < 4; ++w) g[4 * idx + w] += c4[w];
+          },
+          lame ? lame + 2 * pi : nullptr);
   }
 #ifdef _OPENMP
 #pragma omp parallel for num_threads(threads > 1 ? threads : 1) schedule(static)

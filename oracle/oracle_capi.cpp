@@ -66,6 +66,23 @@ int advance(const mpm_params* prm, const mpm_material* mats, int nmat, int64_t n
   return 0;
 }
 
+// All particles of material 0 with per-particle Lame parameters (lame: mu,
+// lambda per particle): the forward of the differentiable step.
+template <class S>
+int advance_lame(const mpm_params* prm, const mpm_material* m, int64_t n, S* x, S* v, S* C, S* F,
+                 S* Jp, const S* lame, int steps) {
+  if (!prm || !m || (n > 0 && (!x || !v || !C || !F || !Jp || !lame)) || prm->N < 4 || prm->N > 1024)
+    return 1;
+  std::vector<uint8_t> mat((size_t)n, 0);
+  std::vector<Particle<S>> ps;
+  load<S>(ps, n, x, v, C, F, Jp, mat.data());
+  const StepConsts<S> k = make_consts<S>(*prm);
+  std::vector<S> grid;
+  for (int s = 0; s < steps; ++s) step<S>(ps, m, 1, k, grid, 1, lame);
+  store<S>(ps, x, v, C, F, Jp);
+  return 0;
+}
+
 template <class S>
 int p2g_only(const mpm_params* prm, const mpm_material* mats, int nmat, int64_t n, const S* x,
              const S* v, const S* C, const S* F, const S* Jp, const uint8_t* mat, S* grid_out) {
@@ -181,6 +198,15 @@ int oracle_advance_f64(const mpm_params* p, const mpm_material* m, int nm, int64
                        double* v, double* C, double* F, double* Jp, const uint8_t* mat, int steps,
                        int threads, double* grid_out) {
   return advance<double>(p, m, nm, n, x, v, C, F, Jp, mat, steps, threads, grid_out);
+}
+
+int oracle_advance_lame_f32(const mpm_params* p, const mpm_material* m, int64_t n, float* x, float* v,
+                            float* C, float* F, float* Jp, const float* lame, int steps) {
+  return advance_lame<float>(p, m, n, x, v, C, F, Jp, lame, steps);
+}
+int oracle_advance_lame_f64(const mpm_params* p, const mpm_material* m, int64_t n, double* x, double* v,
+                            double* C, double* F, double* Jp, const double* lame, int steps) {
+  return advance_lame<double>(p, m, n, x, v, C, F, Jp, lame, steps);
 }
 
 // Wall-clock seconds of `steps` FP32 substeps (the CPU baseline leg).

@@ -1,0 +1,161 @@
+"""The differentiable substep on the GPU (mpm_diff_substep and its adjoint,
+through mpm_amd.diff).
+
+  * forward: k substeps against the FP32 oracle of the same step
+    (oracle.advance_lame), norm-relative 1e-4 per field after 20 substeps;
+  * gradients: the loss |com(x_T) - target|^2 of a cube dropped on the floor
+    (so the stiffness matters), T = 200 substeps with a checkpoint every 20,
+    differentiated w.r.t. the initial velocity (3-vector) and per-particle
+    log E; each directional derivative (the three v0 components and two
+    log E directions) against FP64 central differences (eps = 1e-3) of the
+    FP64 oracle: within 1e-3 relative for each v0 component and for log E
+    along its own gradient direction, and within 1e-3 |grad| along random
+    unit log E directions;
+  * the checkpoint interval changes nothing beyond FP32 atomic-order noise.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import state_errors
+from mpm_amd import MATERIAL_FIXED_COROTATED, material_as_numpy, scenes
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def dropped_cube(side=12):
+    # a cube just above the floor walls (node layers y < 0.05)
+    return scenes.diff_cube(side=side, lo=(0.45, 0.08, 0.45), spacing=0.009375)
+
+
+V0 = (0.3, -1.5, 0.2)
+
+
+def lame64(logE, nu):
+    E = np.exp(np.asarray(logE, np.float64))
+    return E / (2.0 * (1.0 + nu)), E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+
+
+def oracle_state(x, v0):
+    n = x.shape[0]
+    return {"x": np.asarray(x, np.float64), "v": np.tile(np.asarray(v0, np.float64), (n, 1)),
+            "C": np.zeros((n, 3, 3)), "F": np.tile(np.eye(3), (n, 1, 1)), "Jp": np.ones(n),
+            "material": np.zeros(n, np.uint8)}
+
+
+def oracle_material(c):
+    m = material_as_numpy(MATERIAL_FIXED_COROTATED, c.particle_volume, 1000.0, E=1e4, nu=c.nu)
+    m.particle_mass = c.particle_mass
+    m.particle_volume = c.particle_volume
+    return m
+
+
+def make_stepper(c):
+    from mpm_amd.diff import DiffStepper
+    return DiffStepper(c.params, c.particle_mass, c.particle_volume, c.x.shape[0])
+
+
+def test_forward_matches_oracle(gpu_lib):
+    from mpm_amd.diff import pack_state, rollout, unpack_state
+    c = dropped_cube(10)
+    n = c.x.shape[0]
+    mu64, lam64 = lame64(c.logE, c.nu)
+    mu = mu64.astype(np.float32)
+    lam = lam64.astype(np.float32)
+    dev = torch.device("cuda")
+    x = torch.from_numpy(c.x).to(dev)
+    v = torch.tensor(V0, dtype=torch.float32, device=dev).expand(n, 3)
+    st0 = pack_state(x, v)
+    stp = make_stepper(c)
+    with torch.no_grad():
+        out = rollout(stp, st0, torch.from_numpy(mu).to(dev), torch.from_numpy(lam).to(dev), 20)
+    torch.cuda.synchronize()
+    gx, gv, gC, gF = (t.cpu().numpy() for t in unpack_state(out))
+    got = {"x": gx, "v": gv, "C": gC, "F": gF}
+    ref_in = oracle_state(c.x, V0)
+    ref32 = oracle.advance_lame(c.params, oracle_material(c), ref_in, mu, lam, 20, dtype=np.float32)
+    ref64 = oracle.advance_lame(c.params, oracle_material(c), ref_in, mu64, lam64, 20,
+                                dtype=np.float64)
+    e32 = state_errors(got, ref32, c.params.N, keys=("x", "v", "F", "C"))
+    e64 = state_errors(got, ref64, c.params.N, keys=("x", "v", "F", "C"))
+    print("vs fp32", e32, "vs fp64", e64)
+    for k, e in e32.items():
+        assert e < 1e-4, (k, e)
+
+
+def gpu_grads(c, steps, every, v0=V0):
+    from mpm_amd.diff import lame_from_logE, pack_state, rollout
+    dev = torch.device("cuda")
+    n = c.x.shape[0]
+    x = torch.from_numpy(c.x).to(dev)
+    v0_t = torch.tensor(v0, dtype=torch.float32, device=dev, requires_grad=True)
+    logE = torch.from_numpy(c.logE).to(dev).requires_grad_(True)
+    mu, lam = lame_from_logE(logE, c.nu)
+    st0 = pack_state(x, v0_t.expand(n, 3))
+    st = rollout(make_stepper(c), st0, mu, lam, steps, every)
+    com = st[0:3].mean(1)
+    target = torch.tensor(c.target, dtype=torch.float32, device=dev)
+    loss = ((com - target) ** 2).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    return float(loss.detach()), v0_t.grad.cpu().numpy().astype(np.float64), logE.grad.cpu().numpy().astype(np.float64)
+
+
+def oracle_loss(c, v0, logE, steps):
+    mu, lam = lame64(logE, c.nu)
+    out = oracle.advance_lame(c.params, oracle_material(c), oracle_state(c.x, v0), mu, lam, steps,
+                              dtype=np.float64)
+    com = out["x"].mean(0)
+    return float(((com - np.asarray(c.target)) ** 2).sum())
+
+
+def test_gradients_match_fp64_central_differences(gpu_lib):
+    c = dropped_cube(12)
+    steps, eps = 200, 1e-3
+    loss, g_v0, g_logE = gpu_grads(c, steps, 20)
+    logE = c.logE.astype(np.float64)
+    v0 = np.asarray(V0, np.float64)
+    l64 = oracle_loss(c, v0, logE, steps)
+    print("loss gpu", loss, "fp64", l64)
+    assert abs(loss - l64) < 1e-4 * l64
+    checks = []
+    for a in range(3):
+        e = np.eye(3)[a] * eps
+        fd = (oracle_loss(c, v0 + e, logE, steps) - oracle_loss(c, v0 - e, logE, steps)) / (2 * eps)
+        checks.append((f"v0[{a}]", g_v0[a], fd))
+    for name, g, fd in checks:
+        print(name, g, fd, abs(g - fd) / abs(fd))
+        assert abs(g - fd) <= 1e-3 * abs(fd), (name, g, fd)
+    # log E: along the unit gradient u the derivative is |g|, so a wrong
+    # direction or size shows; along random unit directions d the error is
+    # taken relative to |g| (|g . d| itself can be arbitrarily small)
+    gnorm = float(np.linalg.norm(g_logE))
+    u = g_logE / gnorm
+
+    def fd_logE(d):
+        return (oracle_loss(c, v0, logE + eps * d, steps) -
+                oracle_loss(c, v0, logE - eps * d, steps)) / (2 * eps)
+
+    fd = fd_logE(u)
+    print("logE along gradient", gnorm, fd, abs(gnorm - fd) / abs(fd))
+    assert abs(gnorm - fd) <= 1e-3 * abs(fd)
+    rng = np.random.default_rng(11)
+    for t in range(2):
+        d = rng.standard_normal(logE.shape[0])
+        d /= np.linalg.norm(d)
+        fd = fd_logE(d)
+        g = float(g_logE @ d)
+        print(f"logE.d{t}", g, fd, abs(g - fd) / gnorm)
+        assert abs(g - fd) <= 1e-3 * gnorm
+
+
+def test_checkpoint_interval_does_not_change_gradients(gpu_lib):
+    c = dropped_cube(8)
+    _, gv_a, gl_a = gpu_grads(c, 60, 20)
+    _, gv_b, gl_b = gpu_grads(c, 60, 60)
+    _, gv_c, gl_c = gpu_grads(c, 60, 7)
+    for gv, gl in ((gv_b, gl_b), (gv_c, gl_c)):
+        assert np.abs(gv - gv_a).max() <= 1e-4 * np.abs(gv_a).max()
+        assert np.abs(gl - gl_a).max() <= 1e-3 * np.abs(gl_a).max()

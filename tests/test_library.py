@@ -111,3 +111,38 @@ def test_engine_shares_torchs_hip_runtime():
     libs = {line.split()[-1] for line in maps.splitlines() if "libamdhip64" in line}
     torch_dir = str(Path(torch.__file__).resolve().parent)
     assert libs and all(p.startswith(torch_dir) for p in libs), libs
+
+
+def test_diff_sizes():
+    p = mpm_amd.Params(dt=1e-4, N=32, compat=_lib.COMPAT_FIXED).to_c()
+    g, t = C.c_int64(), C.c_int64()
+    assert mpm_amd.lib().mpm_diff_sizes(C.byref(p), C.byref(g), C.byref(t)) == 0
+    assert (g.value, t.value) == (4 * 32 ** 3, 8 * 32 ** 3)
+
+
+@pytest.mark.parametrize("bad", [
+    dict(compat=_lib.COMPAT_REFERENCE), dict(N=2), dict(mass=0.0), dict(volume=float("nan")),
+    dict(n=-1), dict(null_grid=True),
+])
+def test_diff_substep_validates_arguments(bad):
+    # rejected before any device work: every pointer is null or a dummy host address
+    p = mpm_amd.Params(dt=1e-4, N=bad.get("N", 16),
+                       compat=bad.get("compat", _lib.COMPAT_FIXED)).to_c()
+    dummy = C.c_void_p(8)
+    grid = None if bad.get("null_grid") else dummy
+    rc = mpm_amd.lib().mpm_diff_substep(C.byref(p), bad.get("mass", 1.0), bad.get("volume", 1.0),
+                                        bad.get("n", 0), None, None, None, None, grid, dummy, None)
+    assert rc == _lib.MPM_ERR_INVALID
+    assert mpm_amd.lib().mpm_last_error(None)
+    rc = mpm_amd.lib().mpm_diff_substep_backward(
+        C.byref(p), bad.get("mass", 1.0), bad.get("volume", 1.0), bad.get("n", 0), None, None, None,
+        dummy, None, None, None, None, grid, None)
+    assert rc == _lib.MPM_ERR_INVALID
+
+
+def test_diff_substep_rejects_aliased_output():
+    p = mpm_amd.Params(dt=1e-4, N=16, compat=_lib.COMPAT_FIXED).to_c()
+    d = C.c_void_p(8)
+    rc = mpm_amd.lib().mpm_diff_substep(C.byref(p), 1.0, 1.0, 4, d, d, d, d, d, d, None)
+    assert rc == _lib.MPM_ERR_INVALID
+    assert b"alias" in mpm_amd.lib().mpm_last_error(None)
