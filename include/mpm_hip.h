@@ -232,6 +232,46 @@ int mpm_batch_advance(mpm_batch b, int32_t substeps, void* stream);
 int mpm_batch_set_option(mpm_batch b, const char* name, int64_t value);
 int mpm_batch_get_option(mpm_batch b, const char* name, int64_t* value);
 
+/* ---- scene input (src/main.cu:31-66, Simulation::loadMesh / addParticles
+ *      src/mpm.cu:331-394) ----
+ * Loads a scene TOML file: [[material]] tables (density, E, Nu, hardening,
+ * plast_clamp_lower, plast_clamp_higher with the reference's defaults;
+ * extension: model = "snow" | "fixed_corotated" | "jelly" | "wc_liquid" and
+ * bulk_modulus) and [[object]] tables (material, mesh under <scene dir>/meshes/,
+ * size, position, velocity, lifetime_begin, lifetime_end). Each object's OBJ
+ * mesh is scaled to `size`, moved to `position`, and filled by rejection
+ * sampling at u32(particle_count) particles per unit volume, drawing with
+ * glibc's rand() sequence for `seed` (the reference never seeds: seed 1).
+ * Host memory only; no GPU needed. Errors: mpm_scene_last_error(). */
+typedef struct mpm_scene_s* mpm_scene;
+int mpm_scene_load(const char* toml_path, uint32_t particle_count, uint32_t seed, mpm_scene* out);
+int mpm_scene_destroy(mpm_scene s);
+const char* mpm_scene_last_error(void);
+int mpm_scene_info(mpm_scene s, int32_t* n_materials, int32_t* n_objects, int64_t* n_particles);
+/* n_materials entries, particle volume 1/particle_count (main.cu:36) */
+int mpm_scene_materials(mpm_scene s, mpm_material* out);
+/* object i: its material, its particles [first, first+count) of the scene's
+ * particle list, and its lifetime [begin, end) in simulated seconds */
+int mpm_scene_object(mpm_scene s, int32_t i, int32_t* material, int64_t* first, int64_t* count,
+                     double* lifetime_begin, double* lifetime_end);
+/* every object's particles in object order: x[n][3], v[n][3], material[n] */
+int mpm_scene_particles_soa(mpm_scene s, float* x, float* v, uint8_t* material);
+/* One mesh at a time (Simulation::addObject's loadMesh + addParticles,
+ * mpm.cu:237-251,331-394): a sampler owns one generator stream, so objects
+ * filled one after the other continue the sequence as the reference's global
+ * rand() does. fill_mesh scales the OBJ at obj_path to `size`, moves its
+ * lowest corner to `position`, samples it at u32(1/particle_volume) points
+ * per unit volume and keeps the points; mpm_sampler_points copies them
+ * (count x 3 floats). */
+typedef struct mpm_sampler_s* mpm_sampler;
+int mpm_sampler_create(uint32_t seed, mpm_sampler* out);
+int mpm_sampler_destroy(mpm_sampler s);
+int mpm_sampler_fill_mesh(mpm_sampler s, const char* obj_path, double size, const float position[3],
+                          float particle_volume, int64_t* count);
+int mpm_sampler_points(mpm_sampler s, float* x);
+/* the first n outputs of the generator for `seed` (tests) */
+int mpm_debug_glibc_rand(uint32_t seed, int32_t n, int32_t* out);
+
 /* ---- differentiable substep (no reference counterpart: the reference has no
  *      gradients) ----
  * One substep of the fixed-corotated model (MMFixedCorotated,

@@ -1,0 +1,297 @@
+// mpm_simulation.hpp -- C++ facade over the C-ABI (mpm_hip.h) with the
+// reference's class and method names (class Simulation, include/mpm.cuh:44-106;
+// CLIOptions, include/options.h:6-56; SimObject, mpm.cuh:30-41), so a caller
+// written against the reference's Simulation compiles against this header.
+// Header-only; link with -lmpm_hip.
+//
+// Differences from the reference: errors throw mpm::Error (the reference
+// ignores CUDA errors); objects whose lifetime starts or ends are brought onto
+// or taken off the device at the first advance() where t crosses the bound
+// (the reference adds them at the next syncDevice, mpm.cu:288-314, and never
+// removes them); the material list is copied, not referenced.
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mpm_hip.h"
+
+namespace mpm {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// the library's messages name the failing entry point already
+inline void check(int rc, mpm_handle h, const char* what) {
+  if (rc == MPM_OK) return;
+  const std::string m = mpm_last_error(h);
+  throw Error(rc, m.empty() ? std::string(what) + " failed" : m);
+}
+
+// The reference's particle bytes (MLS_APIC_Particle, include/types.h:24-35 +
+// TransferScheme.h:46-54): 104 bytes, F and C column-major.
+struct Particle {
+  uint8_t material_type;
+  uint8_t pad_[3];
+  float x[3], v[3];
+  float F[9], C[9];
+  float Jp;
+  Particle() : material_type(0), pad_{0, 0, 0}, x{0, 0, 0}, v{0, 0, 0},
+               F{1, 0, 0, 0, 1, 0, 0, 0, 1}, C{0, 0, 0, 0, 0, 0, 0, 0, 0}, Jp(1.0f) {}
+  Particle(uint8_t m, const float* pos, const float* vel) : Particle() {
+    material_type = m;
+    for (int a = 0; a < 3; ++a) { x[a] = pos[a]; v[a] = vel[a]; }
+  }
+};
+static_assert(sizeof(Particle) == 104, "the reference's particle is 104 bytes");
+
+// include/options.h:6-56, plus the parameters the reference hard-codes
+// (gravity mpm.cu:6, walls mpm.cu:92) and this build's options.
+struct CLIOptions {
+  float dt = 1e-4f;
+  uint32_t N = 60;
+  uint32_t particle_count = 500000;
+  float dx = 1.0f / 60.0f;
+  float N_real = 60.0f;
+  std::string save_dir;
+  std::string scene = "scenes/snowman.toml";
+  uint32_t mesh_grid = 250;
+  uint32_t mesh_particle_radius = 5;
+  int32_t mesh_face_count = -1;
+  int laplacian_smooth = 0;
+  // not reference flags
+  float gravity = -9.81f;
+  float boundary = 0.05f;
+  int32_t compat = MPM_COMPAT_REFERENCE;
+  uint32_t seed = 1;
+  int64_t steps = 1000;
+  uint32_t sync_every = 20;
+
+  CLIOptions() = default;
+  // --name value or --name=value; unknown flags and bad values print a
+  // message and exit(1), as cxxopts' failure does in options.h:48-51.
+  CLIOptions(int argc, char* argv[]) {
+    for (int i = 1; i < argc; ++i) {
+      std::string a = argv[i], val;
+      if (a.rfind("--", 0) != 0) die("unexpected argument '" + a + "'");
+      a = a.substr(2);
+      const size_t eq = a.find('=');
+      if (eq != std::string::npos) {
+        val = a.substr(eq + 1);
+        a = a.substr(0, eq);
+      } else if (a == "help") {
+        usage();
+        std::exit(0);
+      } else {
+        if (i + 1 >= argc) die("option '" + a + "' needs a value");
+        val = argv[++i];
+      }
+      if (a == "dt") dt = to_f(a, val);
+      else if (a == "N") N = (uint32_t)to_i(a, val, 1);
+      else if (a == "save-dir") save_dir = val;
+      else if (a == "scene") scene = val;
+      else if (a == "particle-count") particle_count = (uint32_t)to_i(a, val, 1);
+      else if (a == "mesh-grid") mesh_grid = (uint32_t)to_i(a, val, 1);
+      else if (a == "mesh-particle-radius") mesh_particle_radius = (uint32_t)to_i(a, val, 0);
+      else if (a == "mesh-face-count") mesh_face_count = (int32_t)to_i(a, val, -1);
+      else if (a == "laplacian_smooth") laplacian_smooth = (int)to_i(a, val, 0);
+      else if (a == "gravity") gravity = to_f(a, val);
+      else if (a == "boundary") boundary = to_f(a, val);
+      else if (a == "compat") compat = val == "fixed" ? MPM_COMPAT_FIXED
+                                     : val == "reference" ? MPM_COMPAT_REFERENCE
+                                     : (die("compat must be 'reference' or 'fixed'"), 0);
+      else if (a == "seed") seed = (uint32_t)to_i(a, val, 0);
+      else if (a == "steps") steps = to_i(a, val, 0);
+      else if (a == "sync-every") sync_every = (uint32_t)to_i(a, val, 1);
+      else die("unknown option '" + a + "'");
+    }
+    dx = (float)(1.0 / N);  // options.h:41
+    N_real = (float)N;
+  }
+
+  static void usage() {
+    std::printf(
+        "MPM simulation.\n"
+        "  --dt arg                    Physics simulation time-step (default 1e-4)\n"
+        "  --N arg                     Grid dimensions (default 60)\n"
+        "  --save-dir arg              Where to save particles (default none)\n"
+        "  --scene arg                 Scene file to load (default scenes/snowman.toml)\n"
+        "  --particle-count arg        Particles per unit volume (default 500000)\n"
+        "  --mesh-grid arg             accepted, unused (no mesh export)\n"
+        "  --mesh-particle-radius arg  accepted, unused\n"
+        "  --mesh-face-count arg       accepted, unused\n"
+        "  --laplacian_smooth arg      accepted, unused\n"
+        "  --steps arg                 substeps to run (default 1000)\n"
+        "  --gravity arg               y acceleration (default -9.81)\n"
+        "  --boundary arg              wall fraction (default 0.05)\n"
+        "  --compat reference|fixed    reference quirks on or off (default reference)\n"
+        "  --seed arg                  sampler seed (default 1, the reference's unseeded rand())\n"
+        "  --sync-every arg            substeps between device syncs (default 20)\n");
+  }
+
+ private:
+  [[noreturn]] static void die(const std::string& m) {
+    std::printf("%s\n", m.c_str());
+    std::exit(1);
+  }
+  static float to_f(const std::string& k, const std::string& v) {
+    char* e = nullptr;
+    const double d = std::strtod(v.c_str(), &e);
+    if (!e || *e || v.empty()) die("option '" + k + "': bad number '" + v + "'");
+    return (float)d;
+  }
+  static long long to_i(const std::string& k, const std::string& v, long long lo) {
+    char* e = nullptr;
+    const long long d = std::strtoll(v.c_str(), &e, 10);
+    if (!e || *e || v.empty() || d < lo) die("option '" + k + "': bad integer '" + v + "'");
+    return d;
+  }
+};
+
+// mpm.cuh:30-41
+struct SimObject {
+  explicit SimObject(const mpm_material& model) : materialModel(model) {}
+  std::vector<Particle> particles;
+  mpm_material materialModel;
+  double lifetime_begin = 0.0;
+  double lifetime_end = std::numeric_limits<double>::max();
+  bool isActive(double t) const { return t >= lifetime_begin && t < lifetime_end; }
+};
+
+class Simulation {
+ public:
+  mpm_params par;
+  double t = 0.0;
+  std::vector<SimObject> objects;
+  int64_t active_particle_count = 0;
+  std::vector<mpm_material> material_models;
+
+  Simulation(const CLIOptions& opts, const std::vector<mpm_material>& materials)
+      : material_models(materials), seed_(opts.seed) {
+    par.dt = opts.dt;
+    par.N = opts.N;
+    par.gravity = opts.gravity;
+    par.boundary_mode = MPM_BOUNDARY_FRACTION;
+    par.boundary = opts.boundary;
+    par.compat = opts.compat;
+    check(mpm_create(&par, &h_), nullptr, "mpm_create");
+  }
+  ~Simulation() {
+    if (sampler_) mpm_sampler_destroy(sampler_);
+    mpm_destroy(h_);
+  }
+  Simulation(const Simulation&) = delete;
+  Simulation& operator=(const Simulation&) = delete;
+
+  mpm_handle handle() const { return h_; }
+
+  // Simulation::addObject (mpm.cu:237-251): fill the OBJ mesh at `filepath`,
+  // scaled so its longest bounding-box edge is `size` with its lowest corner
+  // at `position`, with particles of `material_model_index` moving at `velocity`.
+  void addObject(const std::string& filepath, int material_model_index, float size,
+                 const float position[3], const float velocity[3], double lifetime_begin = 0.0,
+                 double lifetime_end = std::numeric_limits<double>::max()) {
+    if (material_model_index < 0 || material_model_index >= (int)material_models.size())
+      throw Error(MPM_ERR_INVALID, "addObject: no material " + std::to_string(material_model_index));
+    if (!sampler_) check(mpm_sampler_create(seed_, &sampler_), nullptr, "mpm_sampler_create");
+    const mpm_material& m = material_models[(size_t)material_model_index];
+    int64_t n = 0;
+    const int rc = mpm_sampler_fill_mesh(sampler_, filepath.c_str(), (double)size, position,
+                                         m.particle_volume, &n);
+    if (rc) throw Error(rc, mpm_scene_last_error());
+    std::vector<float> x((size_t)n * 3);
+    mpm_sampler_points(sampler_, x.data());
+    SimObject o(m);
+    o.lifetime_begin = lifetime_begin;
+    o.lifetime_end = lifetime_end;
+    for (int64_t i = 0; i < n; ++i)
+      o.particles.emplace_back((uint8_t)material_model_index, &x[(size_t)(3 * i)], velocity);
+    objects.push_back(std::move(o));
+  }
+
+  // An object whose particles the caller already sampled (e.g. from a scene file).
+  void addObject(SimObject o) { objects.push_back(std::move(o)); }
+
+  // Simulation::initCuda (mpm.cu:197-207)
+  void initCuda() {
+    check(mpm_set_materials(h_, material_models.data(), (int32_t)material_models.size()), h_,
+          "mpm_set_materials");
+    upload_active();
+  }
+
+  // Simulation::advance (mpm.cu:323-329): one substep, then t += dt
+  void advance() {
+    if (active_mask(t) != uploaded_) {
+      syncDevice();
+      upload_active();
+    }
+    if (active_particle_count > 0) check(mpm_advance(h_, 1, nullptr), h_, "mpm_advance");
+    t += par.dt;
+  }
+
+  // Simulation::syncDevice (mpm.cu:209-211, 288-306): device particles back
+  // into the objects that are on the device
+  void syncDevice() {
+    if (active_particle_count == 0) return;
+    std::vector<Particle> buf((size_t)active_particle_count);
+    check(mpm_download_aos104(h_, buf.data(), nullptr), h_, "mpm_download_aos104");
+    size_t i = 0;
+    for (size_t k = 0; k < objects.size(); ++k) {
+      if (!uploaded_[k]) continue;
+      for (Particle& p : objects[k].particles) p = buf[i++];
+    }
+  }
+
+  size_t getFullParticleCount() const {
+    size_t n = 0;
+    for (const SimObject& o : objects) n += o.particles.size();
+    return n;
+  }
+  std::vector<Particle>& getFullParticleList() {
+    all_.clear();
+    for (const SimObject& o : objects) all_.insert(all_.end(), o.particles.begin(), o.particles.end());
+    return all_;
+  }
+  std::vector<Particle>& getActiveParticleList() {
+    all_.clear();
+    for (const SimObject& o : objects)
+      if (o.isActive(t)) all_.insert(all_.end(), o.particles.begin(), o.particles.end());
+    return all_;
+  }
+
+ private:
+  std::vector<char> active_mask(double at) const {
+    std::vector<char> m(objects.size());
+    for (size_t k = 0; k < objects.size(); ++k) m[k] = objects[k].isActive(at) ? 1 : 0;
+    return m;
+  }
+  void upload_active() {
+    uploaded_ = active_mask(t);
+    check(mpm_clear_particles(h_), h_, "mpm_clear_particles");
+    int64_t n = 0;
+    for (size_t k = 0; k < objects.size(); ++k) {
+      if (!uploaded_[k] || objects[k].particles.empty()) continue;
+      check(mpm_add_particles_aos104(h_, objects[k].particles.data(), (int64_t)objects[k].particles.size()),
+            h_, "mpm_add_particles_aos104");
+      n += (int64_t)objects[k].particles.size();
+    }
+    active_particle_count = n;
+    if (n > 0) check(mpm_upload(h_, nullptr), h_, "mpm_upload");
+  }
+
+  mpm_handle h_ = nullptr;
+  mpm_sampler sampler_ = nullptr;
+  uint32_t seed_ = 1;
+  std::vector<char> uploaded_;
+  std::vector<Particle> all_;
+};
+
+}  // namespace mpm

@@ -122,6 +122,19 @@ SIGNATURES = {
     "mpm_batch_advance": (C.c_int, [_P, C.c_int32, _P]),
     "mpm_batch_set_option": (C.c_int, [_P, C.c_char_p, C.c_int64]),
     "mpm_batch_get_option": (C.c_int, [_P, C.c_char_p, _i64p]),
+    "mpm_scene_load": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
+    "mpm_scene_destroy": (C.c_int, [_P]),
+    "mpm_scene_last_error": (C.c_char_p, []),
+    "mpm_scene_info": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _i64p]),
+    "mpm_scene_materials": (C.c_int, [_P, C.POINTER(mpm_material)]),
+    "mpm_scene_object": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), _i64p, _i64p,
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "mpm_scene_particles_soa": (C.c_int, [_P, _f, _f, _u8]),
+    "mpm_sampler_create": (C.c_int, [C.c_uint32, C.POINTER(_P)]),
+    "mpm_sampler_destroy": (C.c_int, [_P]),
+    "mpm_sampler_fill_mesh": (C.c_int, [_P, C.c_char_p, C.c_double, _f, C.c_float, _i64p]),
+    "mpm_sampler_points": (C.c_int, [_P, _f]),
+    "mpm_debug_glibc_rand": (C.c_int, [C.c_uint32, C.c_int32, C.POINTER(C.c_int32)]),
     "mpm_diff_sizes": (C.c_int, [C.POINTER(mpm_params), _i64p, _i64p]),
     # device pointers travel as void*
     "mpm_diff_substep": (C.c_int, [C.POINTER(mpm_params), C.c_float, C.c_float, C.c_int64,

@@ -20,6 +20,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libmpm_hip.so"
+CLI = PKG / "bin" / "mpm"
 
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -38,7 +39,7 @@ COMMON = [
 ]
 
 SOURCES = ["capi.cpp", "engine.cpp", "batch.cpp", "substep.hip", "p2g_block.hip", "sort.hip",
-           "linalg.hip", "adjoint.hip"]
+           "linalg.hip", "adjoint.hip", "scene.cpp"]
 
 
 def _compile(src: Path) -> Path:
@@ -74,9 +75,25 @@ def build(verbose: bool = False) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    build_cli()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {CLI}")
     return LIB
+
+
+def build_cli() -> Path:
+    """The `mpm` program (host C++ over the C-ABI) next to the library."""
+    CLI.parent.mkdir(exist_ok=True)
+    tmp = CLI.with_suffix(".tmp")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", str(CSRC / "mpm_cli.cpp"), "-o", str(tmp),
+           f"-L{LIB.parent}", "-lmpm_hip", "-Wl,-rpath,$ORIGIN/.."]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"CLI build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    if res.stderr.strip():
+        sys.stderr.write(res.stderr)
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":

@@ -1,0 +1,116 @@
+// mpm_cli.cpp -- the `mpm` program: the reference's src/main.cu:20-116 on
+// the C-ABI. Loads a scene TOML (materials, objects, meshes under
+// <scene dir>/meshes/), fills the objects with particles, and advances the
+// simulation, syncing every --sync-every substeps (main.cu:99-101) and, with
+// --save-dir, writing the active particles 240 times per simulated second
+// (main.cu:7,93,102-113) to <save-dir>/particles/particles_<frame>.ply.
+//
+// Differences: the run stops after --steps substeps (the reference loops
+// until killed, rendering to a window); particles are written as binary PLY
+// (x, y, z, vx, vy, vz, id) because the reference's .bgeo writer needs the
+// Partio library (particle_writer.h:1-3, absent here); no OBJ surface meshes.
+#include <sys/stat.h>
+
+#include <chrono>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/mpm_hip.h"
+#include "../../include/mpm_simulation.hpp"
+
+namespace {
+
+bool write_ply(const std::string& path, const std::vector<mpm::Particle>& ps) {
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) return false;
+  std::fprintf(f,
+               "ply\nformat binary_little_endian 1.0\nelement vertex %zu\n"
+               "property float x\nproperty float y\nproperty float z\n"
+               "property float vx\nproperty float vy\nproperty float vz\n"
+               "property int id\nend_header\n",
+               ps.size());
+  for (size_t i = 0; i < ps.size(); ++i) {
+    float rec[6] = {ps[i].x[0], ps[i].x[1], ps[i].x[2], ps[i].v[0], ps[i].v[1], ps[i].v[2]};
+    const int32_t id = (int32_t)i;
+    std::fwrite(rec, sizeof(rec), 1, f);
+    std::fwrite(&id, sizeof(id), 1, f);
+  }
+  return std::fclose(f) == 0;
+}
+
+void make_dir(const std::string& d) { ::mkdir(d.c_str(), 0755); }
+
+}  // namespace
+
+int main(int argc, char* argv[]) {
+  mpm::CLIOptions flags(argc, argv);
+  std::printf("Loading scene file: %s\n", flags.scene.c_str());
+  mpm_scene sc = nullptr;
+  if (mpm_scene_load(flags.scene.c_str(), flags.particle_count, flags.seed, &sc) != MPM_OK) {
+    std::fprintf(stderr, "%s\n", mpm_scene_last_error());
+    return 1;
+  }
+  int32_t nm = 0, no = 0;
+  int64_t np = 0;
+  mpm_scene_info(sc, &nm, &no, &np);
+  std::vector<mpm_material> mats((size_t)nm);
+  mpm_scene_materials(sc, mats.data());
+  std::vector<float> x((size_t)np * 3), v((size_t)np * 3);
+  std::vector<uint8_t> mat((size_t)np);
+  mpm_scene_particles_soa(sc, x.data(), v.data(), mat.data());
+  try {
+    mpm::Simulation simulation(flags, mats);
+    for (int32_t i = 0; i < no; ++i) {
+      int32_t m;
+      int64_t first, count;
+      double lb, le;
+      mpm_scene_object(sc, i, &m, &first, &count, &lb, &le);
+      mpm::SimObject o(mats[(size_t)m]);
+      o.lifetime_begin = lb;
+      o.lifetime_end = le;
+      for (int64_t p = first; p < first + count; ++p)
+        o.particles.emplace_back((uint8_t)m, &x[(size_t)(3 * p)], &v[(size_t)(3 * p)]);
+      simulation.addObject(std::move(o));
+    }
+    mpm_scene_destroy(sc);
+    std::printf("loaded materials: %d materials, %d objects, %zu particles\n", nm, no,
+                simulation.getFullParticleCount());
+    simulation.initCuda();
+
+    const bool save = !flags.save_dir.empty();
+    if (save) {
+      make_dir(flags.save_dir);
+      make_dir(flags.save_dir + "/particles");
+    }
+    const uint32_t save_every = std::max<uint32_t>(1, (uint32_t)(1.0 / 240.0 / flags.dt));
+    uint32_t frame_id = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int64_t i = 0; i < flags.steps; ++i) {
+      if (i % 10 == 0) {
+        std::printf("Step %lld\r", (long long)i);
+        std::fflush(stdout);
+      }
+      simulation.advance();
+      if (i % flags.sync_every == 0) simulation.syncDevice();
+      if (save && i % save_every == 0) {
+        if (i % flags.sync_every != 0) simulation.syncDevice();
+        char name[64];
+        std::snprintf(name, sizeof(name), "/particles/particles_%05u.ply", frame_id++);
+        if (!write_ply(flags.save_dir + name, simulation.getActiveParticleList())) {
+          std::fprintf(stderr, "cannot write %s%s\n", flags.save_dir.c_str(), name);
+          return 1;
+        }
+      }
+    }
+    simulation.syncDevice();
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("\ndone: %lld substeps, t = %.6f s, %lld active particles, %.3f s wall, %u frames\n",
+                (long long)flags.steps, simulation.t, (long long)simulation.active_particle_count,
+                secs, frame_id);
+  } catch (const mpm::Error& e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}

@@ -1,0 +1,133 @@
+"""The `mpm` program and the C++ Simulation facade (include/mpm_simulation.hpp).
+
+CPU: flag handling, and a scene that loads but finds no GPU fails with the
+library's message. GPU: a scene whose second object appears mid-run
+(lifetime_begin) advanced by the program, against the same sequence done
+through the Python binding (same kernels, so the same bits): particles of
+the first object alone for the substeps before the object's start time,
+then both objects.
+"""
+import os
+import struct
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from test_scene_file import CUBE_F, CUBE_V, icosphere, write_obj
+
+ROOT = Path(__file__).resolve().parents[1]
+CLI = ROOT / "mpm_amd" / "bin" / "mpm"
+
+SCENE = """\
+[[material]]
+name = "rubber"
+density = 200
+E = 1.4e5
+Nu = 0.2
+hardening = 0
+plast_clamp_lower = 0.0
+plast_clamp_higher = 1.0e30
+
+[[material]]
+name = "snow"
+
+[[object]]
+material = "rubber"
+mesh = "ball.obj"
+size = 0.3
+position = [0.3, 0.2, 0.3]
+velocity = [0.5, 0.0, 0.0]
+
+[[object]]
+material = "snow"
+mesh = "cube.obj"
+size = 0.15
+position = [0.4, 0.6, 0.4]
+velocity = [0.0, -1.0, 0.0]
+lifetime_begin = 0.01
+"""
+
+
+@pytest.fixture()
+def scene(tmp_path):
+    (tmp_path / "meshes").mkdir()
+    V, F = icosphere(2)
+    write_obj(tmp_path / "meshes" / "ball.obj", V, F)
+    write_obj(tmp_path / "meshes" / "cube.obj", CUBE_V, CUBE_F)
+    p = tmp_path / "scene.toml"
+    p.write_text(SCENE)
+    return p
+
+
+def run(*args, timeout=120):
+    return subprocess.run([str(CLI), *map(str, args)], capture_output=True, text=True, timeout=timeout)
+
+
+def test_help_and_bad_flags():
+    assert CLI.exists(), "build the package first (python -m mpm_amd.build)"
+    r = run("--help")
+    assert r.returncode == 0 and "--particle-count" in r.stdout
+    r = run("--bogus", "1")
+    assert r.returncode == 1 and "unknown option 'bogus'" in r.stdout
+    r = run("--dt", "abc")
+    assert r.returncode == 1 and "bad number" in r.stdout
+
+
+def test_missing_scene_fails_cleanly(tmp_path):
+    r = run("--scene", tmp_path / "none.toml", "--steps", "1")
+    assert r.returncode == 1 and "cannot open" in r.stderr
+
+
+def read_ply(path):
+    data = Path(path).read_bytes()
+    head, _, body = data.partition(b"end_header\n")
+    n = int([l for l in head.decode().splitlines() if l.startswith("element vertex")][0].split()[-1])
+    rec = np.frombuffer(body, dtype=np.dtype([("xv", "<f4", 6), ("id", "<i4")]), count=n)
+    return rec["xv"][:, :3].copy(), rec["xv"][:, 3:].copy(), rec["id"].copy()
+
+
+@pytest.mark.gpu
+def test_lifetime_activation_matches_python_binding(scene, tmp_path, gpu_lib):
+    from mpm_amd import Engine, Params
+    from mpm_amd.scene_file import load_scene
+
+    out = tmp_path / "out"
+    pc, dt, N, steps = 60000, 1e-3, 32, 30
+    r = run("--scene", scene, "--particle-count", pc, "--dt", dt, "--N", N, "--steps", steps,
+            "--save-dir", out, "--sync-every", 20)
+    assert r.returncode == 0, r.stdout + r.stderr
+    frames = sorted((out / "particles").glob("particles_*.ply"))
+    save_every = int(1.0 / 240.0 / np.float32(dt))
+    assert len(frames) == (steps + save_every - 1) // save_every
+    sc = load_scene(str(scene), pc, seed=1)
+    a, b = sc.objects
+    # frame k is written after substep i = k * save_every, i.e. after i + 1 substeps
+    x0, _, _ = read_ply(frames[0])
+    assert x0.shape[0] == a.count  # the second object starts at t = 0.01
+    xl, vl, _ = read_ply(frames[-1])
+    assert xl.shape[0] == a.count + b.count
+    nsub = (len(frames) - 1) * save_every + 1
+    # the same sequence through the Python binding
+    t, k_before = 0.0, 0
+    while not t >= b.lifetime_begin:
+        t += float(np.float32(dt))
+        k_before += 1
+    eng = Engine(Params(dt=dt, N=N, gravity=-9.81))
+    eng.set_materials(sc.materials)
+    sl = slice(a.first, a.first + a.count)
+    eng.add_particles(sc.x[sl], sc.material[sl], sc.v[sl])
+    eng.upload()
+    eng.advance(k_before)
+    st = eng.download()
+    eng.clear_particles()
+    eng.add_particles(st["x"], st["material"], st["v"], st["C"], st["F"], st["Jp"])
+    sb = slice(b.first, b.first + b.count)
+    eng.add_particles(sc.x[sb], sc.material[sb], sc.v[sb])
+    eng.upload()
+    eng.advance(nsub - k_before)
+    ref = eng.download()
+    eng.close()
+    assert np.array_equal(xl, ref["x"])
+    assert np.array_equal(vl, ref["v"])
