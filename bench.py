@@ -36,6 +36,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+TIMING_EVERY = 16
 
 
 def alg_bytes(phase: str, P: int, N: int) -> int:
@@ -212,8 +213,10 @@ def main() -> int:
     eng.advance(args.warmup)
     torch.cuda.synchronize()
 
-    # timed region: events on the launch stream bracket each kernel phase
-    eng.set_option("timing", 1)
+    # timed region: events on the launch stream bracket the kernel phases of
+    # one substep in every 16, rotating through the 16 positions of the sort
+    # interval (every substep timed would add ~8 us of event packets to each)
+    eng.set_option("timing", TIMING_EVERY)
     eng.timing_reset()
     torch.cuda.synchronize()
     t0 = time.perf_counter()

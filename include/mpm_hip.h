@@ -170,14 +170,24 @@ int mpm_load_state(mpm_handle h, const char* path);
  *   "chunk"      particles per work item, 64..192 in steps of 64 (0 = 192)
  *   "radix_bits" digit width of the radix sort: 4, 6 or 8 (default 8)
  *   "block"      home block edge in cells: 4 (0 = default 4)
- *   "timing"     1: record HIP events around every kernel phase
+ *   "derive_f"   1 (default): when every material is WC_LIQUID and the
+ *                uploaded state has F = I, Jp = 1, the fused step carries F
+ *                implicitly as cbrt(Jp) I (the value it would store), saving
+ *                72 bytes of particle traffic per substep; downloads and the
+ *                other paths see the same F. 0: F planes read and written.
+ *   "timing"     k > 0: record HIP events around the kernel phases of one
+ *                substep in every k (k = 1: every substep); 0 (default): off
  * Results do not depend on the options beyond FP32 summation order. */
 int mpm_set_option(mpm_handle h, const char* name, int64_t value);
 int mpm_get_option(mpm_handle h, const char* name, int64_t* value);
 
 /* ---- per-kernel timing (no reference counterpart) ----
- * With option "timing" = 1 the engine records a HIP event pair on the launch
- * stream around each of its kernel phases. mpm_timing_read waits for the
+ * With option "timing" = k the engine records a HIP event pair on the launch
+ * stream around each kernel phase of one substep in every k; the timed
+ * substeps rotate through the positions of a k-substep cycle (substep i is
+ * timed when i % k == (i / k) % k, counting from the last reset). The events
+ * carry no system-scope fence, so they do not slow the kernels they bracket.
+ * mpm_timing_read waits for the
  * recorded events and returns the accumulated milliseconds and launch count
  * of one phase: "grid_update", "g2p2g", "p2g", "g2p", "sort" or "clear". */
 int mpm_timing_read(mpm_handle h, const char* phase, double* total_ms, int64_t* launches);

@@ -102,21 +102,29 @@ const char* phase_name(int p) {
   return (p >= 0 && p < NUM_PHASES) ? names[p] : "?";
 }
 
+// Timing events skip the system-scope fence: with it, every record writes
+// back and invalidates the caches, which slows the kernels it brackets
+// (measured: 14 us per substep at the dam break). Only the elapsed time
+// between the pair is read, after the stream has been synchronised.
+static hipError_t timing_event(hipEvent_t* ev) {
+  return hipEventCreateWithFlags(ev, hipEventDisableSystemFence);
+}
+
 int Sim::phase_begin(int ph, hipStream_t s) {
-  if (!opt_.timing) return MPM_OK;
+  if (!timing_now_) return MPM_OK;
   hipEvent_t ev;
   if (!ev_pool_.empty()) { ev = ev_pool_.back(); ev_pool_.pop_back(); }
-  else HIPCHK(hipEventCreate(&ev), "hipEventCreate");
+  else HIPCHK(timing_event(&ev), "hipEventCreate");
   HIPCHK(hipEventRecord(ev, s), "hipEventRecord");
   open_[ph] = ev;
   return MPM_OK;
 }
 
 int Sim::phase_end(int ph, hipStream_t s) {
-  if (!opt_.timing || !open_[ph]) return MPM_OK;
+  if (!timing_now_ || !open_[ph]) return MPM_OK;
   hipEvent_t ev;
   if (!ev_pool_.empty()) { ev = ev_pool_.back(); ev_pool_.pop_back(); }
-  else HIPCHK(hipEventCreate(&ev), "hipEventCreate");
+  else HIPCHK(timing_event(&ev), "hipEventCreate");
   HIPCHK(hipEventRecord(ev, s), "hipEventRecord");
   ev_.push_back({ph, open_[ph], ev});
   open_[ph] = nullptr;
@@ -150,6 +158,7 @@ int Sim::timing_reset() {
   int rc = timing_read("g2p2g", &d, &n);  // drain pending events
   if (rc) return rc;
   for (int ph = 0; ph < NUM_PHASES; ++ph) { phase_ms_[ph] = 0.0; phase_n_[ph] = 0; }
+  timed_subs_ = 0;
   return MPM_OK;
 }
 
@@ -386,6 +395,12 @@ int Sim::upload(hipStream_t s) {
   }
   std::vector<uint32_t> ids(n);
   std::iota(ids.begin(), ids.end(), 0u);
+  f_canonical_ = true;
+  for (int64_t i = 0; i < n && f_canonical_; ++i) {
+    f_canonical_ = hJp_[i] == 1.0f;
+    for (int a = 0; a < 9; ++a) f_canonical_ = f_canonical_ && hF_[9 * i + a] == ((a % 4 == 0) ? 1.0f : 0.0f);
+  }
+  f_stale_ = false;
   max_mat_ = 0;
   for (int64_t i = 0; i < n; ++i) max_mat_ = std::max<uint8_t>(max_mat_, hmat_[i]);
   cur_ = 0;
@@ -454,11 +469,34 @@ int Sim::sort_particles(hipStream_t s) {
   return MPM_OK;
 }
 
+bool Sim::derives_f() const {
+  // only the fused tiled path: the unfused one's G2P kernel reads the F planes
+  if (!opt_.derive_f || !f_canonical_ || opt_.p2g_mode != 1 || !opt_.fused || mats_.empty()) return false;
+  for (const auto& m : mats_)
+    if (m.kind != MPM_MATERIAL_WC_LIQUID) return false;
+  return true;
+}
+
+int Sim::ensure_f(hipStream_t s) {
+  if (!f_stale_) return MPM_OK;
+  HIPCHK(mpmk::liquid_f(particles(), s), "liquid F");
+  f_stale_ = false;
+  return MPM_OK;
+}
+
 int Sim::run_tiled(const float4* grid_v, int do_g2p, hipStream_t s) {
   // the liquid-only instantiation when every material is WC_LIQUID
   kinds_ = 1;
   for (const auto& m : mats_)
     if (m.kind != MPM_MATERIAL_WC_LIQUID) kinds_ = 0;
+  const bool derive = kinds_ == 1 && derives_f();
+  if (!derive) {
+    int rc = ensure_f(s);
+    if (rc) return rc;
+  }
+  int flags = do_g2p ? mpmk::TILED_G2P : 0;
+  if (derive) flags |= mpmk::TILED_DERIVE_F;
+  if (P_ > 0 && (uint64_t)(P_ - 1) > (uint64_t)k_.guard) flags |= mpmk::TILED_GUARD;
   const ParticlesDev P = particles();
   ParticlesDev D = P;
   const uint32_t* g = nullptr;
@@ -469,12 +507,13 @@ int Sim::run_tiled(const float4* grid_v, int do_g2p, hipStream_t s) {
     g = d_gather_;
   }
   HIPCHK(mpmk::g2p2g_block(P, D, g, d_mats_, (int)mats_.size(), k_, work_items(), grid_v, tile_sums(),
-                           do_g2p, kinds_, s),
+                           flags, kinds_, s),
          do_g2p ? "g2p2g" : "p2g");
   if (pending_gather_) {
     cur_ ^= 1;
     pending_gather_ = false;
   }
+  if (derive) f_stale_ = true;
   return MPM_OK;
 }
 
@@ -525,6 +564,16 @@ int Sim::tiled_p2g(hipStream_t s) {
 }
 
 int Sim::substep(hipStream_t s) {
+  // timing k: substep i is timed when i % k == (i / k) % k, so the timed
+  // substeps visit every position of a k-substep cycle (the sort interval)
+  const int64_t k = opt_.timing;
+  timing_now_ = k > 0 && timed_subs_ % k == (timed_subs_ / k) % k;
+  if (k > 0) ++timed_subs_;
+  // every path but the fused tiled one reads the F planes
+  if (!(opt_.p2g_mode == 1 && opt_.fused)) {
+    int rc = ensure_f(s);
+    if (rc) return rc;
+  }
   const ParticlesDev P0 = particles();
   if (opt_.p2g_mode == 1) {
     if (!opt_.fused) {
@@ -612,6 +661,8 @@ int Sim::fetch_state(std::vector<float>& planes, std::vector<uint8_t>& mat,
   mat.resize(P_);
   id.resize(P_);
   if (P_ == 0) return MPM_OK;
+  int rc = ensure_f(s);
+  if (rc) return rc;
   HIPCHK(hipMemcpyAsync(planes.data(), d_f_[cur_], sizeof(float) * planes.size(), hipMemcpyDeviceToHost, s), "download particles");
   HIPCHK(hipMemcpyAsync(mat.data(), d_mat_[cur_], P_, hipMemcpyDeviceToHost, s), "download material ids");
   HIPCHK(hipMemcpyAsync(id.data(), d_id_[cur_], sizeof(uint32_t) * P_, hipMemcpyDeviceToHost, s), "download ids");
@@ -685,6 +736,8 @@ int Sim::debug_p2g(float* grid, hipStream_t s) {
   int rc = check_materials_cover_particles("mpm_debug_p2g");
   if (rc) return rc;
   rc = upload_materials();
+  if (rc) return rc;
+  rc = ensure_f(s);
   if (rc) return rc;
   float4* g = nullptr;
   if (opt_.p2g_mode == 1) {
@@ -807,6 +860,7 @@ int Sim::set_option(const char* name, int64_t v) {
     return MPM_OK;
   }
   if (n == "sort_every") { if (v < 0) return fail(MPM_ERR_INVALID, "sort_every must be >= 0"); opt_.sort_every = v; return MPM_OK; }
+  if (n == "derive_f") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "derive_f must be 0 or 1"); opt_.derive_f = v; return MPM_OK; }
   if (n == "radix_bits") { if (v != 4 && v != 6 && v != 8) return fail(MPM_ERR_INVALID, "radix_bits must be 4, 6 or 8"); opt_.radix_bits = v; return MPM_OK; }
   if (n == "chunk") {
     if (v < 0 || v > mpmk::kMaxItemParticles || v % 64 != 0)
@@ -821,7 +875,12 @@ int Sim::set_option(const char* name, int64_t v) {
     opt_.block = v;
     return MPM_OK;
   }
-  if (n == "timing") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "timing must be 0 or 1"); opt_.timing = v; return MPM_OK; }
+  if (n == "timing") {
+    if (v < 0 || v > 4096) return fail(MPM_ERR_INVALID, "timing must be 0 (off) or 1..4096 (one substep in k)");
+    opt_.timing = v;
+    timed_subs_ = 0;
+    return MPM_OK;
+  }
   return fail(MPM_ERR_INVALID, "mpm_set_option: unknown option '%s'", name);
 }
 
@@ -835,6 +894,7 @@ int Sim::get_option(const char* name, int64_t* v) {
   else if (n == "chunk") *v = opt_.chunk > 0 ? opt_.chunk : chunk_;
   else if (n == "block") *v = 1 << lb_;
   else if (n == "timing") *v = opt_.timing;
+  else if (n == "derive_f") *v = opt_.derive_f;
   else return fail(MPM_ERR_INVALID, "mpm_get_option: unknown option '%s'", name);
   return MPM_OK;
 }

@@ -27,7 +27,9 @@ struct Options {
   int64_t radix_bits = 8;   // "radix_bits": digit width of the radix sort (4, 6 or 8)
   int64_t chunk = 0;        // "chunk": max particles per work item (64..192, multiple of 64; 0 = 192)
   int64_t block = 0;        // "block": edge of a home block in cells (4; 0 = default)
-  int64_t timing = 0;       // "timing": bracket every kernel with HIP events (per-phase time)
+  int64_t derive_f = 1;     // "derive_f": liquid-only scenes keep F = cbrt(J) I implicit
+  int64_t timing = 0;       // "timing": k > 0 brackets the kernels of one substep in every k
+                            // with HIP events (per-phase time); 0 off
 };
 
 // Phases whose kernels the timing option brackets with HIP events.
@@ -100,6 +102,15 @@ class Sim {
   // timing: events recorded around a phase when opt_.timing is set
   int phase_begin(int ph, hipStream_t s);
   int phase_end(int ph, hipStream_t s);
+  bool timing_now_ = false;  // this substep is timed
+  // F derived from J: f_canonical_ when every particle's F is exactly
+  // cbrt(Jp) I (at upload: F = I, Jp = 1), f_stale_ when the fused liquid
+  // kernel left the F planes of the current copy unwritten
+  bool f_canonical_ = false;
+  bool f_stale_ = false;
+  bool derives_f() const;
+  int ensure_f(hipStream_t s);
+  int64_t timed_subs_ = 0;   // substeps since timing_reset
 
   std::mutex mu_;
   std::string err_;

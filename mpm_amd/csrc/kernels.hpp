@@ -84,10 +84,16 @@ struct TileSums {
 // With `gather`, P is the particle order before the last sort: thread slot i
 // reads particle gather[i] of P and writes it to slot i of Pdst (the sort's
 // reorder folded into this pass). Without it Pdst must be P.
+// flags of the fused pass: G2P of the staged grid first (else P2G only);
+// F derived from J (liquid kinds only: F planes neither read nor written);
+// the original-index guard may bind (some particle id exceeds k.guard)
+enum TiledFlags : int { TILED_G2P = 1, TILED_DERIVE_F = 2, TILED_GUARD = 4 };
 hipError_t g2p2g_block(const ParticlesDev& P, const ParticlesDev& Pdst, const uint32_t* gather,
                        const MaterialDev* mats, int nmats, const StepConsts& k,
                        const WorkItems& w, const float4* grid_v, const TileSums& out,
-                       int do_g2p, int kinds, hipStream_t s);
+                       int flags, int kinds, hipStream_t s);
+// F = cbrt(Jp) I for every particle (writes the F planes a derived-F pass left)
+hipError_t liquid_f(const ParticlesDev& P, hipStream_t s);
 // Sums the tiles (and the dirty parts of the atomic grid, which it clears)
 // into every node of `grid`; update=1 applies the grid update (velocities),
 // update=0 leaves the raw P2G sums (momentum, mass).

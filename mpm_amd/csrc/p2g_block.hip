@@ -75,6 +75,13 @@ __device__ __forceinline__ void load_xfj(const PlaneIO& io, uint32_t i, PState& 
   for (int a = 0; a < 9; ++a) s.F[a] = io.ld(PL_F + a, o);
   s.Jp = io.ld(PL_JP, o);
 }
+// x, Jp: what G2P reads when F is derived from J (liquid, see k_g2p2g_tiled)
+__device__ __forceinline__ void load_xj(const PlaneIO& io, uint32_t i, PState& s) {
+  const uint32_t o = i * 4u;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) s.x[a] = io.ld(PL_X + a, o);
+  s.Jp = io.ld(PL_JP, o);
+}
 // v, C: needed only for particles G2P leaves as they are
 __device__ __forceinline__ void load_vc(const PlaneIO& io, uint32_t i, PState& s) {
   const uint32_t o = i * 4u;
@@ -84,7 +91,9 @@ __device__ __forceinline__ void load_vc(const PlaneIO& io, uint32_t i, PState& s
   for (int a = 0; a < 9; ++a) s.C[a] = io.ld(PL_C + a, o);
 }
 
-__device__ __forceinline__ void store_state(const PlaneIO& io, uint32_t i, const PState& s) {
+// with_f = false leaves the F planes unwritten (F derived from J)
+__device__ __forceinline__ void store_state(const PlaneIO& io, uint32_t i, const PState& s,
+                                            bool with_f = true) {
   const uint32_t o = i * 4u;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -92,9 +101,10 @@ __device__ __forceinline__ void store_state(const PlaneIO& io, uint32_t i, const
     io.st(PL_V + a, o, s.v[a]);
   }
 #pragma unroll
-  for (int a = 0; a < 9; ++a) {
-    io.st(PL_C + a, o, s.C[a]);
-    io.st(PL_F + a, o, s.F[a]);
+  for (int a = 0; a < 9; ++a) io.st(PL_C + a, o, s.C[a]);
+  if (with_f) {
+#pragma unroll
+    for (int a = 0; a < 9; ++a) io.st(PL_F + a, o, s.F[a]);
   }
   io.st(PL_JP, o, s.Jp);
 }
@@ -245,7 +255,7 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
                   const uint32_t* __restrict__ item_start, const uint32_t* __restrict__ item_block,
                   const uint32_t* __restrict__ item_waves, const uint32_t* __restrict__ item_count,
                   const float4* __restrict__ grid_v, float4* __restrict__ tiles,
-                  float* __restrict__ grid_atomic, uint32_t* __restrict__ dirty, int do_g2p) {
+                  float* __restrict__ grid_atomic, uint32_t* __restrict__ dirty, int flags) {
   constexpr int BE = 1 << mpmk::kBlockLog2;
   constexpr int W = mpmk::kTileW;        // BE + 2 (stencil reach) + 2 (drift margin)
   constexpr int TN = mpmk::kTileNodes;
@@ -272,6 +282,17 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
   const int o1 = (int)compact3(blk >> 1) * BE - 1;
   const int o2 = (int)compact3(blk) * BE - 1;
 
+  const bool do_g2p = (flags & mpmk::TILED_G2P) != 0;
+  // Liquid with F derived from J: every particle's F is exactly cbrt(Jp) I
+  // (the state F = I, Jp = 1 at upload, and what this kernel computes), so
+  // the F planes are neither read nor written; F_old = cbrt(Jp) I is rebuilt
+  // in registers with the same bits and the engine writes the planes out
+  // before anything else reads them.
+  const bool df = KS == KINDS_LIQUID && (flags & mpmk::TILED_DERIVE_F) != 0;
+  // the original-index guard of G2P (compat REFERENCE) can bind only when
+  // some particle index exceeds it; otherwise the ids are read only to move
+  // them with a reorder
+  const bool guard = (flags & mpmk::TILED_GUARD) != 0;
   const bool valid = lane < cnt_w;
   const uint32_t i = start + (uint32_t)(off_w + lane);
   PState s;
@@ -290,9 +311,10 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
   uint8_t mat = 0;
   if (valid) {
     src = gather ? gather[i] : i;
-    load_xfj(io, src, s);
+    if (df) load_xj(io, src, s);
+    else load_xfj(io, src, s);
     mat = P.mat[src];
-    pid = P.id[src];
+    if (gather || guard) pid = P.id[src];
     m = mats[mat];
   }
 
@@ -328,7 +350,7 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
       Pdst.id[i] = pid;
     }
     quad_weights(s.x, k.dx_inv, base, w);
-    const bool moved = do_g2p && pid <= k.guard && !outside_domain(base, N);
+    const bool moved = do_g2p && (!guard || pid <= k.guard) && !outside_domain(base, N);
     if (!moved) {
       load_vc(io, src, s);
       // complete these loads here: left in flight they would meet the
@@ -393,6 +415,11 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
       float A[9];
 #pragma unroll
       for (int a = 0; a < 9; ++a) A[a] = ((a % 4 == 0) ? 1.0f : 0.0f) + k.dt * s.C[a];
+      if (df) {
+        const float c0 = cbrtf(s.Jp);  // the stored F_old, bit for bit
+#pragma unroll
+        for (int a = 0; a < 9; ++a) s.F[a] = (a % 4 == 0) ? c0 : 0.0f;
+      }
       mul_ab(A, s.F, s.F);
       if (KS == KINDS_LIQUID) {
         // F <- J^(1/3) I, Jp <- J (the only kind this instantiation handles)
@@ -408,7 +435,7 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
       for (int a = 0; a < 3; ++a) s.x[a] = s.x[a] + k.dt * s.v[a];
       quad_weights(s.x, k.dx_inv, base, w);
     }
-    if (moved || gather) store_state(io_dst, i, s);
+    if (moved || gather) store_state(io_dst, i, s, !df);
     inside = !outside_domain(base, N);
   }
 
@@ -672,26 +699,42 @@ namespace mpmk {
 template <int KS>
 static void launch_tiled(const ParticlesDev& P, const ParticlesDev& Pdst, const uint32_t* gather,
                          const MaterialDev* mats, const StepConsts& k, const WorkItems& w,
-                         const float4* gv, const TileSums& o, int do_g2p, hipStream_t s) {
+                         const float4* gv, const TileSums& o, int flags, hipStream_t s) {
   hipLaunchKernelGGL((k_g2p2g_tiled<KS>), dim3((unsigned)w.launch), dim3(64 * kItemWaves), 0, s,
                      P, Pdst, gather, mats, k,
                      w.start, w.block, w.waves, w.count, gv, o.tiles, (float*)o.atomic, o.dirty,
-                     do_g2p);
+                     flags);
 }
 
 hipError_t g2p2g_block(const ParticlesDev& P, const ParticlesDev& Pdst, const uint32_t* gather,
                        const MaterialDev* mats, int nmats, const StepConsts& k,
                        const WorkItems& w, const float4* grid_v, const TileSums& out,
-                       int do_g2p, int kinds, hipStream_t s) {
+                       int flags, int kinds, hipStream_t s) {
   if (P.n <= 0 || w.max_items <= 0) return hipSuccess;
   if (nmats < 1 || nmats > MPM_MAX_MATERIALS) return hipErrorInvalidValue;
-  if (do_g2p && !grid_v) return hipErrorInvalidValue;
+  if ((flags & TILED_G2P) && !grid_v) return hipErrorInvalidValue;
   if (!out.tiles || !out.atomic || !out.dirty) return hipErrorInvalidValue;
   if (w.launch > 0x7fffffff || w.launch < 1) return hipErrorInvalidValue;
   if (gather && (Pdst.f == P.f || Pdst.n != P.n)) return hipErrorInvalidValue;
   if (!gather && Pdst.f != P.f) return hipErrorInvalidValue;
-  if (kinds == KINDS_LIQUID) launch_tiled<KINDS_LIQUID>(P, Pdst, gather, mats, k, w, grid_v, out, do_g2p, s);
-  else launch_tiled<KINDS_ALL>(P, Pdst, gather, mats, k, w, grid_v, out, do_g2p, s);
+  if (kinds == KINDS_LIQUID) launch_tiled<KINDS_LIQUID>(P, Pdst, gather, mats, k, w, grid_v, out, flags, s);
+  else launch_tiled<KINDS_ALL>(P, Pdst, gather, mats, k, w, grid_v, out, flags, s);
+  return hipGetLastError();
+}
+
+// F = cbrt(Jp) I for every particle: the F planes of a liquid state whose F
+// the fused kernel derived from J
+__global__ void __launch_bounds__(256) k_liquid_f(ParticlesDev P) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const float cj = cbrtf(P.f[PL_JP * P.stride + i]);
+#pragma unroll
+  for (int a = 0; a < 9; ++a) P.f[(PL_F + a) * P.stride + i] = (a % 4 == 0) ? cj : 0.0f;
+}
+
+hipError_t liquid_f(const ParticlesDev& P, hipStream_t s) {
+  if (P.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_liquid_f, dim3((unsigned)((P.n + 255) / 256)), dim3(256), 0, s, P);
   return hipGetLastError();
 }
 

@@ -257,6 +257,31 @@ def test_liquid_50_substeps(gpu_lib, opts):
     step_case(sc, 50, opts, 1e-4, fp32_noise=True)
 
 
+def test_liquid_derived_f_is_bit_identical(gpu_lib):
+    """The liquid step that keeps F = cbrt(J) I implicit gives the same bits as
+    the one that reads and writes the F planes, across a download in the
+    middle (which writes the planes out) and a switch to the unfused path."""
+    sc = scenes.dam_break(n=40_000, N=32)
+    outs = []
+    for derive in (1, 0):
+        eng = make_engine(sc, derive_f=derive)
+        eng.advance(37)
+        mid = eng.download()
+        eng.advance(20)
+        eng.set_option("fused", 0)
+        eng.advance(5)
+        eng.set_option("fused", 1)
+        eng.advance(9)
+        outs.append((mid, eng.download()))
+        eng.close()
+    for a, b in zip(outs[0], outs[1]):
+        for f in ("x", "v", "C", "F", "Jp"):
+            assert np.array_equal(a[f], b[f]), f
+    mid = outs[0][0]
+    cj = np.cbrt(mid["Jp"].astype(np.float64))
+    assert np.allclose(mid["F"], cj[:, None, None] * np.eye(3), rtol=3e-7, atol=0)
+
+
 @pytest.mark.parametrize("opts", ["direct-unfused", "tiled-b4", "tiled-b4-chunk64", "tiled-unfused", "tiled-default"])
 def test_snow_10_substeps(gpu_lib, opts):
     sc = scenes.snow_spheres(n=40_000, N=32)
