@@ -91,7 +91,8 @@ def batched_leg(warm: int, steps: int):
     """The batched workload (64 independent scenes, mpm_amd.scenes.batched_scenes):
     seconds for `steps` substeps of every scene advanced one scene after the
     other (each handle's own mpm_advance), against the same substeps through
-    one batch (HIP graph replay). Both sets are warmed by `warm` substeps."""
+    one batch (HIP graph replay of multi-scene launches: one per phase for all
+    scenes). Both sets are warmed by `warm` substeps."""
     import torch
     from mpm_amd import Batch, Engine, scenes
 
@@ -124,6 +125,7 @@ def batched_leg(warm: int, steps: int):
     torch.cuda.synchronize()
     t_bat = time.perf_counter() - t0
     launches = b.get_option("graph_launches")
+    fused = b.get_option("fused_captures")
     b.close()
     for e in bat:
         e.close()
@@ -134,6 +136,7 @@ def batched_leg(warm: int, steps: int):
         "sequential_s": t_seq, "batched_s": t_bat, "speedup": t_seq / t_bat,
         "batched_particle_substeps_per_s": P * steps / t_bat,
         "graph_launches": launches,
+        "multi_scene_cycles_recorded": fused,
     }
 
 

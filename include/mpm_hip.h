@@ -238,7 +238,10 @@ int mpm_batch_count(mpm_batch b, int32_t* out);
  * returns without waiting (later work on `stream` is ordered after it). */
 int mpm_batch_advance(mpm_batch b, int32_t substeps, void* stream);
 /* "graph" (1: record and replay, 0: launch kernel by kernel), "streams"
- * (1..16, default 4); read-only "graph_launches". */
+ * (1..16, default 4), "fuse" (1: when every scene is in the steady state of
+ * the tiled fused mode, record each phase of all scenes as one multi-scene
+ * launch; 0: per-scene kernel chains); read-only "graph_launches" and
+ * "fused_captures". */
 int mpm_batch_set_option(mpm_batch b, const char* name, int64_t value);
 int mpm_batch_get_option(mpm_batch b, const char* name, int64_t* value);
 

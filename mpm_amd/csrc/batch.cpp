@@ -11,9 +11,21 @@
 // the default fused tiled mode, timing on) run eagerly on the same streams.
 // A scene whose settings, materials, particles or current buffer changed since
 // the capture is captured again.
+//
+// Fused cycles (option "fuse", default on): when every scene is in the steady
+// state of the tiled fused mode, a cycle is recorded as one launch per phase
+// for all scenes together instead of per-scene kernel chains: per substep one
+// tile reduction over the node blocks of every scene, the due cell sorts
+// (per scene, on the forked streams), and one fused G2P/P2G pass over the
+// work items of every scene (two when liquid-only and other scenes mix). A
+// workgroup finds its scene in a table of per-scene arguments (the same
+// arguments the scene's own launch would take, so the results are the same
+// bits). The tables are written into pinned host memory while the cycle is
+// recorded and copied to the device by one copy at the head of the graph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <cstdarg>
 #include <cstdio>
 #include <mutex>
@@ -22,6 +34,7 @@
 
 #include "../../include/mpm_hip.h"
 #include "engine.hpp"
+#include "kernels.hpp"
 
 using mpm_engine::Sim;
 
@@ -38,7 +51,13 @@ struct mpm_batch_s {
   std::vector<uint64_t> signature;  // per scene, at capture
   int64_t nstreams = 4;
   int64_t use_graph = 1;
-  int64_t launches = 0;  // graph launches so far (for tests and reports)
+  int64_t fuse = 1;
+  int64_t launches = 0;        // graph launches so far (for tests and reports)
+  int64_t fused_captures = 0;  // cycles recorded as multi-scene launches
+  // argument tables of a fused cycle: pinned host copy and device copy
+  uint8_t* h_tables = nullptr;
+  uint8_t* d_tables = nullptr;
+  size_t tables_cap = 0;
 
   int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -61,6 +80,8 @@ struct mpm_batch_s {
   }
   ~mpm_batch_s() {
     drop_graph();
+    if (h_tables) (void)hipHostFree(h_tables);
+    if (d_tables) (void)hipFree(d_tables);
     for (auto st : streams) (void)hipStreamDestroy(st);
     for (auto ev : joins) (void)hipEventDestroy(ev);
     if (origin) (void)hipStreamDestroy(origin);
@@ -115,6 +136,130 @@ int run_scenes(mpm_batch_s* b, int32_t n) {
   return MPM_OK;
 }
 
+// ---- fused cycles ----------------------------------------------------------
+// Tables are laid out in the pinned buffer as they are produced; identical
+// tables (most substeps between two sorts) share one slot.
+struct TableWriter {
+  mpm_batch_s* b;
+  size_t used = 0;
+  std::vector<std::pair<size_t, size_t>> slots;  // (offset, bytes)
+  // returns the device address of a table holding `bytes` at `src`
+  const void* put(const void* src, size_t bytes) {
+    for (const auto& sl : slots)
+      if (sl.second == bytes && std::memcmp(b->h_tables + sl.first, src, bytes) == 0)
+        return b->d_tables + sl.first;
+    const size_t off = (used + 255) & ~(size_t)255;
+    if (off + bytes > b->tables_cap) return nullptr;
+    std::memcpy(b->h_tables + off, src, bytes);
+    slots.push_back({off, bytes});
+    used = off + bytes;
+    return b->d_tables + off;
+  }
+};
+
+bool fusable(const mpm_batch_s* b) {
+  if (!b->fuse) return false;
+  for (Sim* s : b->scenes) {
+    if (!s->fused_ready()) return false;
+    const int nm = s->material_count();
+    if (nm < 1 || nm > MPM_MAX_MATERIALS) return false;
+  }
+  return true;
+}
+
+int ensure_tables(mpm_batch_s* b, int32_t n) {
+  const size_t ns = b->scenes.size();
+  // per substep at most: one reduce table and two pass tables, each with its prefix
+  const size_t per = ns * (sizeof(mpmk::ReduceDesc) + 2 * sizeof(mpmk::ItemDesc)) + 3 * 4 * (ns + 1) + 6 * 256;
+  const size_t want = per * (size_t)n + 4096;
+  if (want <= b->tables_cap) return MPM_OK;
+  if (b->h_tables) (void)hipHostFree(b->h_tables);
+  if (b->d_tables) (void)hipFree(b->d_tables);
+  b->h_tables = nullptr;
+  b->d_tables = nullptr;
+  b->tables_cap = 0;
+  BCHK(hipHostMalloc((void**)&b->h_tables, want, 0), "hipHostMalloc(batch tables)");
+  BCHK(hipMalloc((void**)&b->d_tables, want), "hipMalloc(batch tables)");
+  b->tables_cap = want;
+  return MPM_OK;
+}
+
+// Record `n` substeps of every scene as multi-scene launches on the origin
+// stream (the stream is being captured).
+int run_scenes_fused(mpm_batch_s* b, int32_t n) {
+  const size_t nsc = b->scenes.size();
+  const size_t ns = std::min(b->streams.size(), nsc);
+  TableWriter tw{b};
+  // the tables reach the device before any kernel of the cycle runs
+  BCHK(hipMemcpyAsync(b->d_tables, b->h_tables, b->tables_cap, hipMemcpyHostToDevice, b->origin),
+       "hipMemcpyAsync(batch tables)");
+  std::vector<mpmk::ReduceDesc> rd(nsc);
+  std::vector<uint32_t> rp(nsc + 1);
+  std::vector<mpmk::ItemDesc> gd[2];
+  std::vector<uint32_t> gp[2];
+  std::vector<mpmk::ItemDesc> done(nsc);
+  for (int32_t step = 0; step < n; ++step) {
+    // 1. the tile reductions with the grid update
+    rp[0] = 0;
+    for (size_t j = 0; j < nsc; ++j) {
+      rd[j] = b->scenes[j]->reduce_desc();
+      rp[j + 1] = rp[j] + (uint32_t)b->scenes[j]->node_blocks();
+    }
+    const void* rdt = tw.put(rd.data(), sizeof(mpmk::ReduceDesc) * nsc);
+    const void* rpt = tw.put(rp.data(), sizeof(uint32_t) * (nsc + 1));
+    if (!rdt || !rpt) return b->fail(MPM_ERR_CAPACITY, "batch: argument tables overflow");
+    BCHK(mpmk::tile_reduce_multi((const mpmk::ReduceDesc*)rdt, (const uint32_t*)rpt, (int)nsc, rp[nsc],
+                                 b->origin),
+         "multi-scene grid update");
+    // 2. the cell sorts that are due, on the forked streams
+    bool any = false;
+    for (Sim* s : b->scenes) any = any || s->sort_due();
+    if (any) {
+      BCHK(hipEventRecord(b->fork, b->origin), "hipEventRecord");
+      for (size_t k = 0; k < ns; ++k) BCHK(hipStreamWaitEvent(b->streams[k], b->fork, 0), "hipStreamWaitEvent");
+      for (size_t j = 0; j < nsc; ++j) {
+        Sim* s = b->scenes[j];
+        if (!s->sort_due()) continue;
+        const int rc = s->sort_now(b->streams[j % ns]);
+        if (rc) return b->fail(rc, "scene %zu: %s", j, s->last_error().c_str());
+      }
+      for (size_t k = 0; k < ns; ++k) {
+        BCHK(hipEventRecord(b->joins[k], b->streams[k]), "hipEventRecord");
+        BCHK(hipStreamWaitEvent(b->origin, b->joins[k], 0), "hipStreamWaitEvent");
+      }
+    }
+    // 3. the fused passes, liquid-only scenes and the others apart
+    for (int g = 0; g < 2; ++g) {
+      gd[g].clear();
+      gp[g].assign(1, 0u);
+    }
+    for (size_t j = 0; j < nsc; ++j) {
+      Sim* s = b->scenes[j];
+      int kinds = 0;
+      int64_t launch = 0;
+      const int rc = s->fused_pass(s->velocity_grid(), 1, done[j], kinds, launch, b->origin);
+      if (rc) return b->fail(rc, "scene %zu: %s", j, s->last_error().c_str());
+      const int g = kinds ? 1 : 0;
+      gd[g].push_back(done[j]);
+      gp[g].push_back(gp[g].back() + (uint32_t)launch);
+    }
+    for (int g = 0; g < 2; ++g) {
+      if (gd[g].empty()) continue;
+      const void* dt = tw.put(gd[g].data(), sizeof(mpmk::ItemDesc) * gd[g].size());
+      const void* pt = tw.put(gp[g].data(), sizeof(uint32_t) * gp[g].size());
+      if (!dt || !pt) return b->fail(MPM_ERR_CAPACITY, "batch: argument tables overflow");
+      BCHK(mpmk::g2p2g_multi((const mpmk::ItemDesc*)dt, (const uint32_t*)pt, (int)gd[g].size(),
+                             gp[g].back(), g, b->origin),
+           "multi-scene G2P/P2G");
+    }
+    for (size_t j = 0; j < nsc; ++j) {
+      b->scenes[j]->fused_pass_done(done[j]);
+      b->scenes[j]->substep_done();
+    }
+  }
+  return MPM_OK;
+}
+
 bool all_at_cycle_start(const mpm_batch_s* b, int* cycle) {
   *cycle = 0;
   for (Sim* s : b->scenes) {
@@ -136,8 +281,20 @@ int capture_and_launch(mpm_batch_s* b, int32_t cycle) {
   b->drop_graph();
   std::vector<uint64_t> sig;
   for (Sim* s : b->scenes) sig.push_back(s->capture_signature());
+  const bool fused = fusable(b);
+  if (fused) {
+    int rc0 = ensure_tables(b, cycle);
+    if (rc0) return rc0;
+    // the previous graph's table copy may still be reading the pinned buffer
+    BCHK(hipStreamSynchronize(b->origin), "hipStreamSynchronize");
+  }
+  std::vector<std::unique_lock<std::mutex>> locks;
+  if (fused)
+    for (Sim* s : b->scenes) locks.emplace_back(s->mutex());
   BCHK(hipStreamBeginCapture(b->origin, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
-  const int rc = run_scenes(b, cycle);
+  const int rc = fused ? run_scenes_fused(b, cycle) : run_scenes(b, cycle);
+  locks.clear();
+  if (fused && !rc) ++b->fused_captures;
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(b->origin, &g);
   if (rc) {
@@ -172,7 +329,10 @@ extern "C" const char* mpm_batch_last_error(mpm_batch b) { return b ? b->err.c_s
 extern "C" int mpm_batch_add(mpm_batch b, mpm_handle h) {
   if (!b || !h) return MPM_ERR_INVALID;
   std::lock_guard<std::mutex> lock(b->mu);
-  b->scenes.push_back(reinterpret_cast<Sim*>(h));
+  Sim* sim = reinterpret_cast<Sim*>(h);
+  if (std::find(b->scenes.begin(), b->scenes.end(), sim) != b->scenes.end())
+    return b->fail(MPM_ERR_INVALID, "mpm_batch_add: the handle is in the batch already");
+  b->scenes.push_back(sim);
   b->drop_graph();
   return MPM_OK;
 }
@@ -199,6 +359,12 @@ extern "C" int mpm_batch_set_option(mpm_batch b, const char* name, int64_t v) {
     b->drop_graph();
     return MPM_OK;
   }
+  if (n == "fuse") {
+    if (v != 0 && v != 1) return b->fail(MPM_ERR_INVALID, "fuse must be 0 or 1");
+    b->fuse = v;
+    b->drop_graph();
+    return MPM_OK;
+  }
   return b->fail(MPM_ERR_INVALID, "mpm_batch_set_option: unknown option '%s'", name);
 }
 
@@ -209,6 +375,8 @@ extern "C" int mpm_batch_get_option(mpm_batch b, const char* name, int64_t* v) {
   if (n == "graph") *v = b->use_graph;
   else if (n == "streams") *v = b->nstreams;
   else if (n == "graph_launches") *v = b->launches;
+  else if (n == "fuse") *v = b->fuse;
+  else if (n == "fused_captures") *v = b->fused_captures;
   else return b->fail(MPM_ERR_INVALID, "mpm_batch_get_option: unknown option '%s'", name);
   return MPM_OK;
 }

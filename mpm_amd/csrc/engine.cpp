@@ -484,7 +484,8 @@ int Sim::ensure_f(hipStream_t s) {
   return MPM_OK;
 }
 
-int Sim::run_tiled(const float4* grid_v, int do_g2p, hipStream_t s) {
+int Sim::fused_pass(const float4* grid_v, int do_g2p, mpmk::ItemDesc& d, int& kinds, int64_t& launch,
+                    hipStream_t s) {
   // the liquid-only instantiation when every material is WC_LIQUID
   kinds_ = 1;
   for (const auto& m : mats_)
@@ -498,22 +499,69 @@ int Sim::run_tiled(const float4* grid_v, int do_g2p, hipStream_t s) {
   if (derive) flags |= mpmk::TILED_DERIVE_F;
   if (P_ > 0 && (uint64_t)(P_ - 1) > (uint64_t)k_.guard) flags |= mpmk::TILED_GUARD;
   const ParticlesDev P = particles();
-  ParticlesDev D = P;
-  const uint32_t* g = nullptr;
+  d.P = P;
+  d.Pdst = P;
+  d.gather = nullptr;
   if (pending_gather_) {
-    D.f = d_f_[cur_ ^ 1];
-    D.mat = d_mat_[cur_ ^ 1];
-    D.id = d_id_[cur_ ^ 1];
-    g = d_gather_;
+    d.Pdst.f = d_f_[cur_ ^ 1];
+    d.Pdst.mat = d_mat_[cur_ ^ 1];
+    d.Pdst.id = d_id_[cur_ ^ 1];
+    d.gather = d_gather_;
   }
-  HIPCHK(mpmk::g2p2g_block(P, D, g, d_mats_, (int)mats_.size(), k_, work_items(), grid_v, tile_sums(),
-                           flags, kinds_, s),
-         do_g2p ? "g2p2g" : "p2g");
+  const mpmk::WorkItems w = work_items();
+  const mpmk::TileSums t = tile_sums();
+  d.mats = d_mats_;
+  d.k = k_;
+  d.item_start = w.start;
+  d.item_block = w.block;
+  d.item_waves = w.waves;
+  d.item_count = w.count;
+  d.grid_v = grid_v;
+  d.tiles = t.tiles;
+  d.grid_atomic = (float*)t.atomic;
+  d.dirty = t.dirty;
+  d.flags = flags;
+  kinds = kinds_;
+  launch = w.launch;
+  return MPM_OK;
+}
+
+void Sim::fused_pass_done(const mpmk::ItemDesc& d) {
   if (pending_gather_) {
     cur_ ^= 1;
     pending_gather_ = false;
   }
-  if (derive) f_stale_ = true;
+  if (d.flags & mpmk::TILED_DERIVE_F) f_stale_ = true;
+}
+
+mpmk::ReduceDesc Sim::reduce_desc() const {
+  const mpmk::TileSums t = tile_sums();
+  mpmk::ReduceDesc r;
+  r.tiles = t.tiles;
+  r.block_items = t.block_items;
+  r.covered = t.covered;
+  r.grid_atomic = t.atomic;
+  r.dirty = t.dirty;
+  r.grid = d_grid_[0];
+  r.k = k_;
+  r.nbx = (k_.N + 3) / 4;
+  r.update = 1;
+  return r;
+}
+
+int64_t Sim::node_blocks() const {
+  const int64_t nbx = (k_.N + 3) / 4;
+  return nbx * nbx * nbx;
+}
+
+int Sim::run_tiled(const float4* grid_v, int do_g2p, hipStream_t s) {
+  mpmk::ItemDesc d;
+  int kinds = 0;
+  int64_t launch = 0;
+  int rc = fused_pass(grid_v, do_g2p, d, kinds, launch, s);
+  if (rc) return rc;
+  HIPCHK(mpmk::g2p2g_desc(d, (int)mats_.size(), launch, kinds, s), do_g2p ? "g2p2g" : "p2g");
+  fused_pass_done(d);
   return MPM_OK;
 }
 

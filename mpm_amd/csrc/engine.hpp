@@ -83,6 +83,36 @@ class Sim {
     for (int32_t i = 0; i < substeps; ++i) t_ += (double)prm_.dt;
   }
 
+  // ---- one tiled fused substep in parts, for batches that launch the parts
+  // of many scenes together (batch.cpp); Sim::substep runs the same parts ----
+  // the scene is in the steady state of the tiled fused mode (primed, items
+  // built, materials uploaded, timing off)
+  bool fused_ready() const {
+    return uploaded_ && P_ > 0 && !mats_dirty_ && primed_ && items_valid_ && opt_.p2g_mode == 1 &&
+           opt_.fused == 1 && opt_.sort_every > 0 && !opt_.timing;
+  }
+  // part 1: the tile reduction with the grid update into the velocity grid
+  mpmk::ReduceDesc reduce_desc() const;
+  int64_t node_blocks() const;
+  // part 2: the cell sort, when due
+  bool sort_due() const {
+    return !items_valid_ || (opt_.sort_every > 0 && steps_since_sort_ >= opt_.sort_every);
+  }
+  int sort_now(hipStream_t s) { return sort_particles(s); }
+  // part 3: the fused pass (writes F planes out first on `s` if needed);
+  // fused_pass_done() after it was launched
+  int fused_pass(const float4* grid_v, int do_g2p, mpmk::ItemDesc& d, int& kinds, int64_t& launch,
+                 hipStream_t s);
+  void fused_pass_done(const mpmk::ItemDesc& d);
+  // the end of a substep: velocity grid, sort counter, time
+  void substep_done() {
+    vel_grid_ = d_grid_[0];
+    ++steps_since_sort_;
+    t_ += (double)prm_.dt;
+  }
+  const float4* velocity_grid() const { return d_grid_[0]; }
+  int material_count() const { return (int)mats_.size(); }
+
   // device views for other entry points of the library
   ParticlesDev particles() const;
   mpmk::WorkItems work_items() const;

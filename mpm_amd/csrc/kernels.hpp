@@ -84,6 +84,35 @@ struct TileSums {
 // With `gather`, P is the particle order before the last sort: thread slot i
 // reads particle gather[i] of P and writes it to slot i of Pdst (the sort's
 // reorder folded into this pass). Without it Pdst must be P.
+// One scene's fused pass (the arguments of k_g2p2g_tiled) and one scene's
+// tile reduction (those of k_tile_reduce): the multi-scene launches read one
+// of these per scene from a device table.
+struct ItemDesc {
+  ParticlesDev P, Pdst;
+  const uint32_t* gather;
+  const MaterialDev* mats;
+  StepConsts k;
+  const uint32_t* item_start;
+  const uint32_t* item_block;
+  const uint32_t* item_waves;
+  const uint32_t* item_count;
+  const float4* grid_v;
+  float4* tiles;
+  float* grid_atomic;
+  uint32_t* dirty;
+  int flags;
+};
+struct ReduceDesc {
+  const float4* tiles;
+  const uint32_t* block_items;
+  const uint32_t* covered;
+  float4* grid_atomic;
+  uint32_t* dirty;
+  float4* grid;
+  StepConsts k;
+  int nbx;
+  int update;
+};
 // flags of the fused pass: G2P of the staged grid first (else P2G only);
 // F derived from J (liquid kinds only: F planes neither read nor written);
 // the original-index guard may bind (some particle id exceeds k.guard)
@@ -92,6 +121,15 @@ hipError_t g2p2g_block(const ParticlesDev& P, const ParticlesDev& Pdst, const ui
                        const MaterialDev* mats, int nmats, const StepConsts& k,
                        const WorkItems& w, const float4* grid_v, const TileSums& out,
                        int flags, int kinds, hipStream_t s);
+// The same passes over several scenes in one launch: descs[j] is scene j,
+// prefix[j] its first workgroup (prefix[n] = total; workgroup counts are the
+// scenes' max_items for g2p2g, their node-block counts for the reduction).
+// kinds: KINDS of every scene of the launch.
+hipError_t g2p2g_desc(const ItemDesc& d, int nmats, int64_t launch, int kinds, hipStream_t s);
+hipError_t g2p2g_multi(const ItemDesc* descs, const uint32_t* prefix, int nscenes, int64_t total,
+                       int kinds, hipStream_t s);
+hipError_t tile_reduce_multi(const ReduceDesc* descs, const uint32_t* prefix, int nscenes,
+                             int64_t total_blocks, hipStream_t s);
 // F = cbrt(Jp) I for every particle (writes the F planes a derived-F pass left)
 hipError_t liquid_f(const ParticlesDev& P, hipStream_t s);
 // Sums the tiles (and the dirty parts of the atomic grid, which it clears)

@@ -247,15 +247,21 @@ struct StencilSlots {
   }
 };
 
+// One work item of the fused pass (one workgroup of 64 kItemWaves threads).
 template <int KS>
-__global__ void __launch_bounds__(64 * mpmk::kItemWaves)
-    __attribute__((amdgpu_waves_per_eu(KS == KINDS_LIQUID ? 5 : 4)))
-    k_g2p2g_tiled(ParticlesDev P, ParticlesDev Pdst, const uint32_t* __restrict__ gather,
-                  const MaterialDev* __restrict__ mats, StepConsts k,
-                  const uint32_t* __restrict__ item_start, const uint32_t* __restrict__ item_block,
-                  const uint32_t* __restrict__ item_waves, const uint32_t* __restrict__ item_count,
-                  const float4* __restrict__ grid_v, float4* __restrict__ tiles,
-                  float* __restrict__ grid_atomic, uint32_t* __restrict__ dirty, int flags) {
+__device__ __forceinline__ void g2p2g_item(const mpmk::ItemDesc& d, const int item) {
+  const ParticlesDev P = d.P, Pdst = d.Pdst;
+  const uint32_t* __restrict__ const gather = d.gather;
+  const MaterialDev* __restrict__ const mats = d.mats;
+  const StepConsts k = d.k;
+  const uint32_t* __restrict__ const item_start = d.item_start;
+  const uint32_t* __restrict__ const item_block = d.item_block;
+  const uint32_t* __restrict__ const item_waves = d.item_waves;
+  const float4* __restrict__ const grid_v = d.grid_v;
+  float4* __restrict__ const tiles = d.tiles;
+  float* __restrict__ const grid_atomic = d.grid_atomic;
+  uint32_t* __restrict__ const dirty = d.dirty;
+  const int flags = d.flags;
   constexpr int BE = 1 << mpmk::kBlockLog2;
   constexpr int W = mpmk::kTileW;        // BE + 2 (stencil reach) + 2 (drift margin)
   constexpr int TN = mpmk::kTileNodes;
@@ -268,8 +274,7 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
   // it after the barrier that ends every G2P read.
   float4* const vt = acc[0];
 
-  const int item = blockIdx.x;
-  if ((uint32_t)item >= *item_count) return;  // uniform for the workgroup
+  if ((uint32_t)item >= *d.item_count) return;  // uniform for the workgroup
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const uint32_t start = item_start[item];
   const uint32_t wc = item_waves[item];
@@ -612,6 +617,34 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
   }
 }
 
+template <int KS>
+__global__ void __launch_bounds__(64 * mpmk::kItemWaves)
+    __attribute__((amdgpu_waves_per_eu(KS == KINDS_LIQUID ? 5 : 4)))
+    k_g2p2g_tiled(mpmk::ItemDesc d) {
+  g2p2g_item<KS>(d, (int)blockIdx.x);
+}
+
+// the scene of global workgroup or block g: the last j with prefix[j] <= g
+__device__ __forceinline__ int scene_of(const uint32_t* __restrict__ prefix, int nscenes, uint32_t g) {
+  int lo = 0, hi = nscenes;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <int KS>
+__global__ void __launch_bounds__(64 * mpmk::kItemWaves)
+    __attribute__((amdgpu_waves_per_eu(KS == KINDS_LIQUID ? 5 : 4)))
+    k_g2p2g_multi(const mpmk::ItemDesc* __restrict__ descs, const uint32_t* __restrict__ prefix,
+                  int nscenes) {
+  const uint32_t g = blockIdx.x;
+  const int j = scene_of(prefix, nscenes, g);
+  g2p2g_item<KS>(descs[j], (int)(g - prefix[j]));
+}
+
 // One wavefront per 4^3 node block: each lane sums its node over the tiles of
 // the items of the home blocks around it, adds the side grid where particles
 // left their tiles (and clears it), and writes the node: grid update
@@ -622,15 +655,16 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
 // picks the ranges of its 8 blocks with lane shuffles and reads the first
 // item of each in one batch of independent loads, further items (blocks
 // holding more than one item) after it. The order of the sum is fixed.
-__global__ void __launch_bounds__(256)
-    k_tile_reduce(const float4* __restrict__ tiles, const uint32_t* __restrict__ block_items,
-                  const uint32_t* __restrict__ covered, float4* __restrict__ grid_atomic,
-                  uint32_t* __restrict__ dirty, float4* __restrict__ grid, StepConsts k, int nbx,
-                  int update) {
+__device__ __forceinline__ void reduce_block(const mpmk::ReduceDesc& d, const int64_t nbk, const int lane) {
   constexpr int W = mpmk::kTileW, TN = mpmk::kTileNodes;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t nbk = (int64_t)blockIdx.x * 4 + wave;
-  if (nbk >= (int64_t)nbx * nbx * nbx) return;  // uniform for the wavefront
+  const float4* __restrict__ const tiles = d.tiles;
+  const uint32_t* __restrict__ const block_items = d.block_items;
+  const uint32_t* __restrict__ const covered = d.covered;
+  float4* __restrict__ const grid_atomic = d.grid_atomic;
+  uint32_t* __restrict__ const dirty = d.dirty;
+  float4* __restrict__ const grid = d.grid;
+  const StepConsts k = d.k;
+  const int nbx = d.nbx, update = d.update;
   const int bz = (int)(nbk % nbx), by = (int)((nbk / nbx) % nbx), bx = (int)(nbk / ((int64_t)nbx * nbx));
   const int u0 = lane >> 4, u1 = (lane >> 2) & 3, u2 = lane & 3;
   const int N = k.N;
@@ -671,15 +705,25 @@ __global__ void __launch_bounds__(256)
     for (int q = 0; q < 8; ++q) {
       sum.x += v[q].x; sum.y += v[q].y; sum.z += v[q].z; sum.w += v[q].w;
     }
-    // further items of blocks holding more than one
+    // further items of blocks holding more than one (dense scenes): round r
+    // reads item r of every block that has one, 8 independent loads per round
+    uint32_t first[8], rounds = 0;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int f0 = e0 + (q >> 2), f1 = e1 + ((q >> 1) & 1), f2 = e2 + (q & 1);
       const int j = 9 * (f0 + 1) + 3 * (f1 + 1) + (f2 + 1);
-      const uint32_t first = (uint32_t)__shfl((int)r0, j);
-      for (uint32_t it = first + 1; it < a1[q]; ++it) {
-        const float4 w4 = tiles[(size_t)it * TN + a0[q]];
-        sum.x += w4.x; sum.y += w4.y; sum.z += w4.z; sum.w += w4.w;
+      first[q] = (uint32_t)__shfl((int)r0, j);
+      rounds = max(rounds, a1[q] - first[q]);
+    }
+    for (uint32_t r = 1; r < rounds; ++r) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const uint32_t it = first[q] + r;
+        v[q] = it < a1[q] ? tiles[(size_t)it * TN + a0[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        sum.x += v[q].x; sum.y += v[q].y; sum.z += v[q].z; sum.w += v[q].w;
       }
     }
   }
@@ -694,16 +738,38 @@ __global__ void __launch_bounds__(256)
   if (in_grid) grid[node] = update ? grid_node_update(sum, x, y, z, k) : sum;
 }
 
+__global__ void __launch_bounds__(256) k_tile_reduce(mpmk::ReduceDesc d) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t nbk = (int64_t)blockIdx.x * 4 + wave;
+  if (nbk >= (int64_t)d.nbx * d.nbx * d.nbx) return;  // uniform for the wavefront
+  reduce_block(d, nbk, lane);
+}
+
+__global__ void __launch_bounds__(256)
+    k_tile_reduce_multi(const mpmk::ReduceDesc* __restrict__ descs, const uint32_t* __restrict__ prefix,
+                        int nscenes) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t g = blockIdx.x * 4u + (uint32_t)wave;
+  if (g >= prefix[nscenes]) return;  // uniform for the wavefront
+  const int j = scene_of(prefix, nscenes, g);
+  reduce_block(descs[j], (int64_t)(g - prefix[j]), lane);
+}
+
 namespace mpmk {
 
-template <int KS>
-static void launch_tiled(const ParticlesDev& P, const ParticlesDev& Pdst, const uint32_t* gather,
-                         const MaterialDev* mats, const StepConsts& k, const WorkItems& w,
-                         const float4* gv, const TileSums& o, int flags, hipStream_t s) {
-  hipLaunchKernelGGL((k_g2p2g_tiled<KS>), dim3((unsigned)w.launch), dim3(64 * kItemWaves), 0, s,
-                     P, Pdst, gather, mats, k,
-                     w.start, w.block, w.waves, w.count, gv, o.tiles, (float*)o.atomic, o.dirty,
-                     flags);
+hipError_t g2p2g_desc(const ItemDesc& d, int nmats, int64_t launch, int kinds, hipStream_t s) {
+  if (d.P.n <= 0 || launch <= 0) return hipSuccess;
+  if (nmats < 1 || nmats > MPM_MAX_MATERIALS) return hipErrorInvalidValue;
+  if ((d.flags & TILED_G2P) && !d.grid_v) return hipErrorInvalidValue;
+  if (!d.tiles || !d.grid_atomic || !d.dirty) return hipErrorInvalidValue;
+  if (launch > 0x7fffffff) return hipErrorInvalidValue;
+  if (d.gather && (d.Pdst.f == d.P.f || d.Pdst.n != d.P.n)) return hipErrorInvalidValue;
+  if (!d.gather && d.Pdst.f != d.P.f) return hipErrorInvalidValue;
+  if (kinds == KINDS_LIQUID)
+    hipLaunchKernelGGL((k_g2p2g_tiled<KINDS_LIQUID>), dim3((unsigned)launch), dim3(64 * kItemWaves), 0, s, d);
+  else
+    hipLaunchKernelGGL((k_g2p2g_tiled<KINDS_ALL>), dim3((unsigned)launch), dim3(64 * kItemWaves), 0, s, d);
+  return hipGetLastError();
 }
 
 hipError_t g2p2g_block(const ParticlesDev& P, const ParticlesDev& Pdst, const uint32_t* gather,
@@ -711,14 +777,44 @@ hipError_t g2p2g_block(const ParticlesDev& P, const ParticlesDev& Pdst, const ui
                        const WorkItems& w, const float4* grid_v, const TileSums& out,
                        int flags, int kinds, hipStream_t s) {
   if (P.n <= 0 || w.max_items <= 0) return hipSuccess;
-  if (nmats < 1 || nmats > MPM_MAX_MATERIALS) return hipErrorInvalidValue;
-  if ((flags & TILED_G2P) && !grid_v) return hipErrorInvalidValue;
-  if (!out.tiles || !out.atomic || !out.dirty) return hipErrorInvalidValue;
-  if (w.launch > 0x7fffffff || w.launch < 1) return hipErrorInvalidValue;
-  if (gather && (Pdst.f == P.f || Pdst.n != P.n)) return hipErrorInvalidValue;
-  if (!gather && Pdst.f != P.f) return hipErrorInvalidValue;
-  if (kinds == KINDS_LIQUID) launch_tiled<KINDS_LIQUID>(P, Pdst, gather, mats, k, w, grid_v, out, flags, s);
-  else launch_tiled<KINDS_ALL>(P, Pdst, gather, mats, k, w, grid_v, out, flags, s);
+  if (w.launch < 1) return hipErrorInvalidValue;
+  ItemDesc d;
+  d.P = P;
+  d.Pdst = Pdst;
+  d.gather = gather;
+  d.mats = mats;
+  d.k = k;
+  d.item_start = w.start;
+  d.item_block = w.block;
+  d.item_waves = w.waves;
+  d.item_count = w.count;
+  d.grid_v = grid_v;
+  d.tiles = out.tiles;
+  d.grid_atomic = (float*)out.atomic;
+  d.dirty = out.dirty;
+  d.flags = flags;
+  return g2p2g_desc(d, nmats, w.launch, kinds, s);
+}
+
+hipError_t g2p2g_multi(const ItemDesc* descs, const uint32_t* prefix, int nscenes, int64_t total,
+                       int kinds, hipStream_t s) {
+  if (nscenes <= 0 || total <= 0) return hipSuccess;
+  if (!descs || !prefix || total > 0x7fffffff) return hipErrorInvalidValue;
+  if (kinds == KINDS_LIQUID)
+    hipLaunchKernelGGL((k_g2p2g_multi<KINDS_LIQUID>), dim3((unsigned)total), dim3(64 * kItemWaves), 0, s,
+                       descs, prefix, nscenes);
+  else
+    hipLaunchKernelGGL((k_g2p2g_multi<KINDS_ALL>), dim3((unsigned)total), dim3(64 * kItemWaves), 0, s,
+                       descs, prefix, nscenes);
+  return hipGetLastError();
+}
+
+hipError_t tile_reduce_multi(const ReduceDesc* descs, const uint32_t* prefix, int nscenes,
+                             int64_t total_blocks, hipStream_t s) {
+  if (nscenes <= 0 || total_blocks <= 0) return hipSuccess;
+  if (!descs || !prefix || total_blocks > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_tile_reduce_multi, dim3((unsigned)((total_blocks + 3) / 4)), dim3(256), 0, s, descs,
+                     prefix, nscenes);
   return hipGetLastError();
 }
 
@@ -742,10 +838,18 @@ hipError_t tile_reduce(const TileSums& in, const StepConsts& k, float4* grid, in
                        hipStream_t s) {
   if (!in.tiles || !in.atomic || !in.dirty || !in.block_items || !in.covered || !grid)
     return hipErrorInvalidValue;
-  const int nbx = (k.N + 3) / 4;
-  const int64_t nblocks = (int64_t)nbx * nbx * nbx;
-  hipLaunchKernelGGL(k_tile_reduce, dim3((unsigned)((nblocks + 3) / 4)), dim3(256), 0, s, in.tiles,
-                     in.block_items, in.covered, in.atomic, in.dirty, grid, k, nbx, update);
+  ReduceDesc d;
+  d.tiles = in.tiles;
+  d.block_items = in.block_items;
+  d.covered = in.covered;
+  d.grid_atomic = in.atomic;
+  d.dirty = in.dirty;
+  d.grid = grid;
+  d.k = k;
+  d.nbx = (k.N + 3) / 4;
+  d.update = update;
+  const int64_t nblocks = (int64_t)d.nbx * d.nbx * d.nbx;
+  hipLaunchKernelGGL(k_tile_reduce, dim3((unsigned)((nblocks + 3) / 4)), dim3(256), 0, s, d);
   return hipGetLastError();
 }
 

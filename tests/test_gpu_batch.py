@@ -40,18 +40,21 @@ def run_alone(scs, k):
     return out
 
 
-@pytest.mark.parametrize("graph", [1, 0])
-def test_batch_equals_individual_runs(gpu_lib, graph):
+@pytest.mark.parametrize("graph,fuse", [(1, 1), (1, 0), (0, 1)])
+def test_batch_equals_individual_runs(gpu_lib, graph, fuse):
     scs = small_batch()
     k = 100  # 16 eager substeps, two recorded cycles of 32, 20 eager
     ref = run_alone(scs, k)
     engines = [make(sc) for sc in scs]
     b = Batch(engines)
     b.set_option("graph", graph)
+    b.set_option("fuse", fuse)
     b.advance(k)
     outs = [e.download() for e in engines]
     launches = b.get_option("graph_launches")
+    fused = b.get_option("fused_captures")
     b.close()
+    assert fused == (1 if graph and fuse else 0)
     for e, r in zip(engines, ref):
         assert e.time() == r["t"]
     for o, r in zip(outs, ref):
