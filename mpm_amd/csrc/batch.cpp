@@ -16,7 +16,7 @@
 // state of the tiled fused mode, a cycle is recorded as one launch per phase
 // for all scenes together instead of per-scene kernel chains: per substep one
 // tile reduction over the node blocks of every scene, the due cell sorts
-// (per scene, on the forked streams), and one fused G2P/P2G pass over the
+// (in one launch per sort kernel), and one fused G2P/P2G pass over the
 // work items of every scene (two when liquid-only and other scenes mix). A
 // workgroup finds its scene in a table of per-scene arguments (the same
 // arguments the scene's own launch would take, so the results are the same
@@ -52,6 +52,7 @@ struct mpm_batch_s {
   int64_t nstreams = 4;
   int64_t use_graph = 1;
   int64_t fuse = 1;
+  int64_t multi_sort = 1;  // fused cycles sort all due scenes in one launch per kernel
   int64_t launches = 0;        // graph launches so far (for tests and reports)
   int64_t fused_captures = 0;  // cycles recorded as multi-scene launches
   // argument tables of a fused cycle: pinned host copy and device copy
@@ -161,6 +162,7 @@ bool fusable(const mpm_batch_s* b) {
   if (!b->fuse) return false;
   for (Sim* s : b->scenes) {
     if (!s->fused_ready()) return false;
+    if (b->multi_sort && !s->multi_sort_ok()) return false;
     const int nm = s->material_count();
     if (nm < 1 || nm > MPM_MAX_MATERIALS) return false;
   }
@@ -169,8 +171,10 @@ bool fusable(const mpm_batch_s* b) {
 
 int ensure_tables(mpm_batch_s* b, int32_t n) {
   const size_t ns = b->scenes.size();
-  // per substep at most: one reduce table and two pass tables, each with its prefix
-  const size_t per = ns * (sizeof(mpmk::ReduceDesc) + 2 * sizeof(mpmk::ItemDesc)) + 3 * 4 * (ns + 1) + 6 * 256;
+  // per substep at most: one reduce table, two pass tables and a sort table,
+  // each launch with its prefix (the sort: about 20 launches)
+  const size_t per = ns * (sizeof(mpmk::ReduceDesc) + 2 * sizeof(mpmk::ItemDesc) + sizeof(mpmk::SortDesc)) +
+                     24 * (4 * (ns + 1) + 256) + 4 * 256;
   const size_t want = per * (size_t)n + 4096;
   if (want <= b->tables_cap) return MPM_OK;
   if (b->h_tables) (void)hipHostFree(b->h_tables);
@@ -194,27 +198,70 @@ int run_scenes_fused(mpm_batch_s* b, int32_t n) {
   BCHK(hipMemcpyAsync(b->d_tables, b->h_tables, b->tables_cap, hipMemcpyHostToDevice, b->origin),
        "hipMemcpyAsync(batch tables)");
   std::vector<mpmk::ReduceDesc> rd(nsc);
-  std::vector<uint32_t> rp(nsc + 1);
+  std::vector<uint32_t> rp(nsc + 1), cp(nsc + 1);
   std::vector<mpmk::ItemDesc> gd[2];
   std::vector<uint32_t> gp[2];
   std::vector<mpmk::ItemDesc> done(nsc);
   for (int32_t step = 0; step < n; ++step) {
-    // 1. the tile reductions with the grid update
+    // 1. the tile reductions with the grid update, after the tiles of home
+    //    blocks with several items were summed into one (small dense scenes
+    //    hold dozens of items per block)
     rp[0] = 0;
+    cp[0] = 0;
     for (size_t j = 0; j < nsc; ++j) {
       rd[j] = b->scenes[j]->reduce_desc();
+      rd[j].combined = 1;
       rp[j + 1] = rp[j] + (uint32_t)b->scenes[j]->node_blocks();
+      cp[j + 1] = cp[j] + (uint32_t)rd[j].nblk;
     }
     const void* rdt = tw.put(rd.data(), sizeof(mpmk::ReduceDesc) * nsc);
     const void* rpt = tw.put(rp.data(), sizeof(uint32_t) * (nsc + 1));
-    if (!rdt || !rpt) return b->fail(MPM_ERR_CAPACITY, "batch: argument tables overflow");
+    const void* cpt = tw.put(cp.data(), sizeof(uint32_t) * (nsc + 1));
+    if (!rdt || !rpt || !cpt) return b->fail(MPM_ERR_CAPACITY, "batch: argument tables overflow");
+    BCHK(mpmk::tile_combine_multi((const mpmk::ReduceDesc*)rdt, (const uint32_t*)cpt, (int)nsc, cp[nsc],
+                                  b->origin),
+         "multi-scene tile combine");
     BCHK(mpmk::tile_reduce_multi((const mpmk::ReduceDesc*)rdt, (const uint32_t*)rpt, (int)nsc, rp[nsc],
                                  b->origin),
          "multi-scene grid update");
-    // 2. the cell sorts that are due, on the forked streams
+    // 2. the cell sorts that are due: one launch per sort kernel for all
+    //    of them (or per scene on the forked streams)
     bool any = false;
     for (Sim* s : b->scenes) any = any || s->sort_due();
-    if (any) {
+    if (any && b->multi_sort) {
+      std::vector<mpmk::SortDesc> sd;
+      std::vector<Sim*> due;
+      for (size_t j = 0; j < nsc; ++j) {
+        Sim* s = b->scenes[j];
+        if (!s->sort_due()) continue;
+        mpmk::SortDesc d;
+        const int rc = s->sort_desc(d);
+        if (rc) return b->fail(rc, "scene %zu: %s", j, s->last_error().c_str());
+        sd.push_back(d);
+        due.push_back(s);
+      }
+      const void* sdt = tw.put(sd.data(), sizeof(mpmk::SortDesc) * sd.size());
+      if (!sdt) return b->fail(MPM_ERR_CAPACITY, "batch: argument tables overflow");
+      int maxp = 0;
+      for (const auto& d : sd) maxp = std::max(maxp, d.npasses);
+      std::vector<std::pair<int, int>> seq = {{mpmk::SORT_FILL, 0}, {mpmk::SORT_KEYS, 0}};
+      for (int p = 0; p < maxp; ++p)
+        for (int ph : {(int)mpmk::SORT_HIST, (int)mpmk::SORT_RSCAN, (int)mpmk::SORT_SCATTER}) seq.push_back({ph, p});
+      for (int ph = mpmk::SORT_HEADS; ph < mpmk::SORT_NUM_PHASES; ++ph) seq.push_back({ph, 0});
+      std::vector<uint32_t> pre(sd.size() + 1);
+      for (const auto& pp : seq) {
+        pre[0] = 0;
+        for (size_t j = 0; j < sd.size(); ++j)
+          pre[j + 1] = pre[j] + (uint32_t)mpmk::sort_blocks(sd[j], pp.first, pp.second);
+        if (pre.back() == 0) continue;
+        const void* pt = tw.put(pre.data(), sizeof(uint32_t) * pre.size());
+        if (!pt) return b->fail(MPM_ERR_CAPACITY, "batch: argument tables overflow");
+        BCHK(mpmk::sort_multi(pp.first, pp.second, (const mpmk::SortDesc*)sdt, (const uint32_t*)pt,
+                              (int)sd.size(), pre.back(), b->origin),
+             "multi-scene sort");
+      }
+      for (Sim* s : due) s->sort_done();
+    } else if (any) {
       BCHK(hipEventRecord(b->fork, b->origin), "hipEventRecord");
       for (size_t k = 0; k < ns; ++k) BCHK(hipStreamWaitEvent(b->streams[k], b->fork, 0), "hipStreamWaitEvent");
       for (size_t j = 0; j < nsc; ++j) {
@@ -365,6 +412,12 @@ extern "C" int mpm_batch_set_option(mpm_batch b, const char* name, int64_t v) {
     b->drop_graph();
     return MPM_OK;
   }
+  if (n == "multi_sort") {
+    if (v != 0 && v != 1) return b->fail(MPM_ERR_INVALID, "multi_sort must be 0 or 1");
+    b->multi_sort = v;
+    b->drop_graph();
+    return MPM_OK;
+  }
   return b->fail(MPM_ERR_INVALID, "mpm_batch_set_option: unknown option '%s'", name);
 }
 
@@ -376,6 +429,7 @@ extern "C" int mpm_batch_get_option(mpm_batch b, const char* name, int64_t* v) {
   else if (n == "streams") *v = b->nstreams;
   else if (n == "graph_launches") *v = b->launches;
   else if (n == "fuse") *v = b->fuse;
+  else if (n == "multi_sort") *v = b->multi_sort;
   else if (n == "fused_captures") *v = b->fused_captures;
   else return b->fail(MPM_ERR_INVALID, "mpm_batch_get_option: unknown option '%s'", name);
   return MPM_OK;

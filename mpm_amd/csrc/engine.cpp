@@ -469,6 +469,40 @@ int Sim::sort_particles(hipStream_t s) {
   return MPM_OK;
 }
 
+int Sim::sort_desc(mpmk::SortDesc& d) {
+  chunk_ = opt_.chunk > 0 ? (int)opt_.chunk : mpmk::kMaxItemParticles;
+  const int64_t items = std::max<int64_t>(1, max_items());
+  if (P_ <= 0 || items > tiles_cap_ || opt_.radix_bits != 8)
+    return fail(MPM_ERR_STATE, "sort_desc: needs particles, allocated tiles and 8-bit radix digits");
+  d.P = particles();
+  d.k = k_;
+  d.keys[0] = d_keys_[0];
+  d.keys[1] = d_keys_[1];
+  d.vals[0] = d_vals_[0];
+  d.vals[1] = d_vals_[1];
+  d.scratch = d_hist_;
+  d.n = P_;
+  d.lb = lb_;
+  d.start_bit = 3 * lb_;
+  d.key_bits = mpmk::cell_key_bits(k_.N, lb_);
+  d.npasses = (P_ > 1 && d.key_bits > d.start_bit) ? (d.key_bits - d.start_bit + 7) / 8 : 0;
+  d.block_range = d_block_start_;
+  d.block_items = d_block_items_;
+  d.covered = d_covered_;
+  d.nblk = nblk_;
+  d.nbx = (k_.N + 3) / 4;
+  d.flag = d_flag_;
+  d.pos = d_pos_;
+  d.item_count = d_item_count_;
+  d.item_start = d_item_start_;
+  d.item_block = d_item_block_;
+  d.item_waves = d_item_waves_;
+  d.gather = d_gather_;
+  d.chunk = chunk_;
+  d.max_items = items;
+  return MPM_OK;
+}
+
 bool Sim::derives_f() const {
   // only the fused tiled path: the unfused one's G2P kernel reads the F planes
   if (!opt_.derive_f || !f_canonical_ || opt_.p2g_mode != 1 || !opt_.fused || mats_.empty()) return false;
@@ -546,6 +580,8 @@ mpmk::ReduceDesc Sim::reduce_desc() const {
   r.k = k_;
   r.nbx = (k_.N + 3) / 4;
   r.update = 1;
+  r.combined = 0;
+  r.nblk = nblk_;
   return r;
 }
 

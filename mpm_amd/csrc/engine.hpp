@@ -99,6 +99,15 @@ class Sim {
     return !items_valid_ || (opt_.sort_every > 0 && steps_since_sort_ >= opt_.sort_every);
   }
   int sort_now(hipStream_t s) { return sort_particles(s); }
+  // the same sort as arguments of the multi-scene sort kernels; sort_done()
+  // after they were launched
+  bool multi_sort_ok() const { return opt_.radix_bits == 8 && tiles_cap_ >= max_items(); }
+  int sort_desc(mpmk::SortDesc& d);
+  void sort_done() {
+    pending_gather_ = true;
+    steps_since_sort_ = 0;
+    items_valid_ = true;
+  }
   // part 3: the fused pass (writes F planes out first on `s` if needed);
   // fused_pass_done() after it was launched
   int fused_pass(const float4* grid_v, int do_g2p, mpmk::ItemDesc& d, int& kinds, int64_t& launch,

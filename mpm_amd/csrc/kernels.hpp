@@ -112,6 +112,8 @@ struct ReduceDesc {
   StepConsts k;
   int nbx;
   int update;
+  int combined;   // the items of each home block were summed into its first tile
+  int64_t nblk;   // home blocks (Morton space): the combine's workgroups
 };
 // flags of the fused pass: G2P of the staged grid first (else P2G only);
 // F derived from J (liquid kinds only: F planes neither read nor written);
@@ -130,6 +132,46 @@ hipError_t g2p2g_multi(const ItemDesc* descs, const uint32_t* prefix, int nscene
                        int kinds, hipStream_t s);
 hipError_t tile_reduce_multi(const ReduceDesc* descs, const uint32_t* prefix, int nscenes,
                              int64_t total_blocks, hipStream_t s);
+// sums the tiles of every home block holding several items into its first
+// item's tile, in item order (prefix over the scenes' nblk); a reduction with
+// `combined` set then reads one tile per home block
+hipError_t tile_combine_multi(const ReduceDesc* descs, const uint32_t* prefix, int nscenes,
+                              int64_t total_blocks, hipStream_t s);
+// One scene's cell sort and work-item build (the arguments of the sort
+// kernels), for sorts of many scenes in one launch per kernel.
+struct SortDesc {
+  ParticlesDev P;
+  StepConsts k;
+  uint32_t* keys[2];
+  uint32_t* vals[2];
+  uint32_t* scratch;  // radix histograms and totals; tile sums of the item scan
+  int64_t n;
+  int lb, start_bit, key_bits, npasses;
+  uint32_t* block_range;
+  uint32_t* block_items;
+  uint32_t* covered;
+  int64_t nblk;
+  int nbx;
+  uint32_t* flag;
+  uint32_t* pos;
+  uint32_t* item_count;
+  uint32_t* item_start;
+  uint32_t* item_block;
+  uint32_t* item_waves;
+  uint32_t* gather;
+  int chunk;
+  int64_t max_items;
+};
+// the sort as a sequence of phases; SORT_HIST/RSCAN/SCATTER run once per
+// 8-bit pass (pass = 0, 1, ...)
+enum SortPhase : int {
+  SORT_FILL = 0, SORT_KEYS, SORT_HIST, SORT_RSCAN, SORT_SCATTER, SORT_HEADS, SORT_FLAGS,
+  SORT_SCAN_TILES, SORT_SCAN_SUMS, SORT_SCAN_ADD, SORT_EMIT, SORT_ARRANGE, SORT_NUM_PHASES
+};
+// workgroups of one scene in a phase (0: the scene sits the phase out)
+int64_t sort_blocks(const SortDesc& d, int phase, int pass);
+hipError_t sort_multi(int phase, int pass, const SortDesc* descs, const uint32_t* prefix, int nscenes,
+                      int64_t total_blocks, hipStream_t s);
 // F = cbrt(Jp) I for every particle (writes the F planes a derived-F pass left)
 hipError_t liquid_f(const ParticlesDev& P, hipStream_t s);
 // Sums the tiles (and the dirty parts of the atomic grid, which it clears)

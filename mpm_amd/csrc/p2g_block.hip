@@ -701,10 +701,11 @@ __device__ __forceinline__ void reduce_block(const mpmk::ReduceDesc& d, const in
       v[q] = a0[q] < a1[q] ? tiles[(size_t)a0[q] * TN + t] : make_float4(0.f, 0.f, 0.f, 0.f);
       a0[q] = (uint32_t)t;  // tile node from here on
     }
+    // per home block q the sum of its items in item order (what the combine
+    // pass leaves in the block's first tile), then the 8 block sums in order
+    float4 T[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      sum.x += v[q].x; sum.y += v[q].y; sum.z += v[q].z; sum.w += v[q].w;
-    }
+    for (int q = 0; q < 8; ++q) T[q] = v[q];
     // further items of blocks holding more than one (dense scenes): round r
     // reads item r of every block that has one, 8 independent loads per round
     uint32_t first[8], rounds = 0;
@@ -715,6 +716,7 @@ __device__ __forceinline__ void reduce_block(const mpmk::ReduceDesc& d, const in
       first[q] = (uint32_t)__shfl((int)r0, j);
       rounds = max(rounds, a1[q] - first[q]);
     }
+    if (d.combined) rounds = 1;  // the first tile holds the block's sum
     for (uint32_t r = 1; r < rounds; ++r) {
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
@@ -723,8 +725,12 @@ __device__ __forceinline__ void reduce_block(const mpmk::ReduceDesc& d, const in
       }
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        sum.x += v[q].x; sum.y += v[q].y; sum.z += v[q].z; sum.w += v[q].w;
+        if (first[q] + r < a1[q]) { T[q].x += v[q].x; T[q].y += v[q].y; T[q].z += v[q].z; T[q].w += v[q].w; }
       }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      sum.x += T[q].x; sum.y += T[q].y; sum.z += T[q].z; sum.w += T[q].w;
     }
   }
   if (is_dirty) {
@@ -743,6 +749,38 @@ __global__ void __launch_bounds__(256) k_tile_reduce(mpmk::ReduceDesc d) {
   const int64_t nbk = (int64_t)blockIdx.x * 4 + wave;
   if (nbk >= (int64_t)d.nbx * d.nbx * d.nbx) return;  // uniform for the wavefront
   reduce_block(d, nbk, lane);
+}
+
+// one workgroup per home block: tiles[first] += tiles[first + 1 .. end), in order
+__device__ __forceinline__ void combine_block(const mpmk::ReduceDesc& d, const int64_t b) {
+  constexpr int TN = mpmk::kTileNodes;
+  const uint32_t r0 = d.block_items[2 * (size_t)b], r1 = d.block_items[2 * (size_t)b + 1];
+  if (r1 <= r0 + 1) return;  // uniform for the workgroup
+  float4* const t0 = const_cast<float4*>(d.tiles) + (size_t)r0 * TN;  // the tiles are the engine's scratch
+  for (int n = threadIdx.x; n < TN; n += 256) {
+    float4 acc = t0[n];
+    uint32_t it = r0 + 1;
+    for (; it + 4 <= r1; it += 4) {
+      float4 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = d.tiles[(size_t)(it + u) * TN + n];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) { acc.x += v[u].x; acc.y += v[u].y; acc.z += v[u].z; acc.w += v[u].w; }
+    }
+    for (; it < r1; ++it) {
+      const float4 v = d.tiles[(size_t)it * TN + n];
+      acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+    }
+    t0[n] = acc;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_tile_combine_multi(const mpmk::ReduceDesc* __restrict__ descs, const uint32_t* __restrict__ prefix,
+                         int nscenes) {
+  const uint32_t g = blockIdx.x;
+  const int j = scene_of(prefix, nscenes, g);
+  combine_block(descs[j], (int64_t)(g - prefix[j]));
 }
 
 __global__ void __launch_bounds__(256)
@@ -809,6 +847,15 @@ hipError_t g2p2g_multi(const ItemDesc* descs, const uint32_t* prefix, int nscene
   return hipGetLastError();
 }
 
+hipError_t tile_combine_multi(const ReduceDesc* descs, const uint32_t* prefix, int nscenes,
+                              int64_t total_blocks, hipStream_t s) {
+  if (nscenes <= 0 || total_blocks <= 0) return hipSuccess;
+  if (!descs || !prefix || total_blocks > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_tile_combine_multi, dim3((unsigned)total_blocks), dim3(256), 0, s, descs, prefix,
+                     nscenes);
+  return hipGetLastError();
+}
+
 hipError_t tile_reduce_multi(const ReduceDesc* descs, const uint32_t* prefix, int nscenes,
                              int64_t total_blocks, hipStream_t s) {
   if (nscenes <= 0 || total_blocks <= 0) return hipSuccess;
@@ -848,6 +895,8 @@ hipError_t tile_reduce(const TileSums& in, const StepConsts& k, float4* grid, in
   d.k = k;
   d.nbx = (k.N + 3) / 4;
   d.update = update;
+  d.combined = 0;
+  d.nblk = 0;
   const int64_t nblocks = (int64_t)d.nbx * d.nbx * d.nbx;
   hipLaunchKernelGGL(k_tile_reduce, dim3((unsigned)((nblocks + 3) / 4)), dim3(256), 0, s, d);
   return hipGetLastError();

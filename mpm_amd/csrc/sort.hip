@@ -50,10 +50,10 @@ __device__ __forceinline__ uint32_t cell_key_of(const float x[3], const StepCons
   return (m << (3 * lb)) | local;
 }
 
-__global__ void __launch_bounds__(256)
-    k_cell_keys(ParticlesDev P, StepConsts k, int lb, uint32_t* __restrict__ keys,
-                uint32_t* __restrict__ vals) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void cell_keys_body(const ParticlesDev& P, const StepConsts& k, int lb,
+                                               uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                               uint32_t blk) {
+  const int i = (int)(blk * 256u + threadIdx.x);
   if (i >= P.n) return;
   float x[3];
 #pragma unroll
@@ -62,25 +62,36 @@ __global__ void __launch_bounds__(256)
   vals[i] = (uint32_t)i;
 }
 
+__global__ void __launch_bounds__(256)
+    k_cell_keys(ParticlesDev P, StepConsts k, int lb, uint32_t* __restrict__ keys,
+                uint32_t* __restrict__ vals) {
+  cell_keys_body(P, k, lb, keys, vals, blockIdx.x);
+}
+
 // ---- work items: runs of at most `chunk` particles of one home block -------
 // block_range[2b], block_range[2b+1] = first and one-past-last sorted index of
 // block b (written for occupied blocks only)
-__global__ void __launch_bounds__(256)
-    k_block_heads(const uint32_t* __restrict__ keys, int64_t n, int shift,
-                  uint32_t* __restrict__ block_range) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void block_heads_body(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                                                 uint32_t* __restrict__ block_range, uint32_t blk) {
+  const int64_t i = (int64_t)blk * 256 + threadIdx.x;
   if (i >= n) return;
   const uint32_t b = keys[i] >> shift;
   if (i == 0 || (keys[i - 1] >> shift) != b) block_range[2 * (size_t)b] = (uint32_t)i;
   if (i == n - 1 || (keys[i + 1] >> shift) != b) block_range[2 * (size_t)b + 1] = (uint32_t)(i + 1);
 }
 
+__global__ void __launch_bounds__(256)
+    k_block_heads(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                  uint32_t* __restrict__ block_range) {
+  block_heads_body(keys, n, shift, block_range, blockIdx.x);
+}
+
 // flag[i] = 1 where a work item starts. A block of c particles is cut into
 // ceil(c / chunk) items of near-equal size (at most chunk each).
-__global__ void __launch_bounds__(256)
-    k_item_flags(const uint32_t* __restrict__ keys, int64_t n, int shift, int chunk,
-                 const uint32_t* __restrict__ block_range, uint32_t* __restrict__ flag) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void item_flags_body(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                                                int chunk, const uint32_t* __restrict__ block_range,
+                                                uint32_t* __restrict__ flag, uint32_t blk) {
+  const int64_t i = (int64_t)blk * 256 + threadIdx.x;
   if (i >= n) return;
   const size_t b = keys[i] >> shift;
   const uint32_t s = block_range[2 * b], c = block_range[2 * b + 1] - s;
@@ -89,15 +100,23 @@ __global__ void __launch_bounds__(256)
   flag[i] = (((uint32_t)i - s) % size == 0u) ? 1u : 0u;
 }
 
+__global__ void __launch_bounds__(256)
+    k_item_flags(const uint32_t* __restrict__ keys, int64_t n, int shift, int chunk,
+                 const uint32_t* __restrict__ block_range, uint32_t* __restrict__ flag) {
+  item_flags_body(keys, n, shift, chunk, block_range, flag, blockIdx.x);
+}
+
 // Items in sorted order: item_start/item_block for every flagged particle, and
 // for every occupied block its items [block_items[2b], block_items[2b+1]).
-__global__ void __launch_bounds__(256)
-    k_item_emit(const uint32_t* __restrict__ keys, int64_t n, int shift,
-                const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
-                const uint32_t* __restrict__ total, uint32_t* __restrict__ item_start,
-                uint32_t* __restrict__ item_block, uint32_t* __restrict__ block_items,
-                uint32_t* __restrict__ covered, int nbx) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void item_emit_body(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                                               const uint32_t* __restrict__ flag,
+                                               const uint32_t* __restrict__ pos,
+                                               const uint32_t* __restrict__ total,
+                                               uint32_t* __restrict__ item_start,
+                                               uint32_t* __restrict__ item_block,
+                                               uint32_t* __restrict__ block_items,
+                                               uint32_t* __restrict__ covered, int nbx, uint32_t blk) {
+  const int64_t i = (int64_t)blk * 256 + threadIdx.x;
   if (i < n) {
     const uint32_t b = keys[i] >> shift;
     if (flag[i]) {
@@ -122,6 +141,16 @@ __global__ void __launch_bounds__(256)
   if (i == 0) item_start[*total] = (uint32_t)n;  // sentinel: end of the last item
 }
 
+__global__ void __launch_bounds__(256)
+    k_item_emit(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                const uint32_t* __restrict__ total, uint32_t* __restrict__ item_start,
+                uint32_t* __restrict__ item_block, uint32_t* __restrict__ block_items,
+                uint32_t* __restrict__ covered, int nbx) {
+  item_emit_body(keys, n, shift, flag, pos, total, item_start, item_block, block_items, covered, nbx,
+                 blockIdx.x);
+}
+
 __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -138,12 +167,12 @@ __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, int6
 // permutation inside the item); item_waves packs the per-wavefront counts
 // (8 bits each). The layout is a function of the sorted
 // keys only. One workgroup of 256 threads per item (n <= kMaxItemParticles).
-__global__ void __launch_bounds__(256)
-    k_item_arrange(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ perm,
-                   int cell_bits, const uint32_t* __restrict__ item_start,
-                   const uint32_t* __restrict__ count, uint32_t* __restrict__ gather,
-                   uint32_t* __restrict__ item_waves) {
-  const uint32_t item = blockIdx.x;
+__device__ __forceinline__ void item_arrange_body(const uint32_t* __restrict__ keys,
+                                                  const uint32_t* __restrict__ perm, int cell_bits,
+                                                  const uint32_t* __restrict__ item_start,
+                                                  const uint32_t* __restrict__ count,
+                                                  uint32_t* __restrict__ gather,
+                                                  uint32_t* __restrict__ item_waves, const uint32_t item) {
   if (item >= *count) return;  // uniform for the workgroup
   __shared__ uint32_t sk[256];
   __shared__ uint8_t orig[256];
@@ -258,6 +287,14 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+__global__ void __launch_bounds__(256)
+    k_item_arrange(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ perm,
+                   int cell_bits, const uint32_t* __restrict__ item_start,
+                   const uint32_t* __restrict__ count, uint32_t* __restrict__ gather,
+                   uint32_t* __restrict__ item_waves) {
+  item_arrange_body(keys, perm, cell_bits, item_start, count, gather, item_waves, blockIdx.x);
+}
+
 // ---- exclusive scan of u32 (tiles of kTile, then tile sums in one workgroup)
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
   // 256 threads; returns the exclusive prefix of v over the workgroup
@@ -276,12 +313,11 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
   return incl - v;
 }
 
-__global__ void __launch_bounds__(256)
-    k_scan_tiles(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int64_t n,
-                 uint32_t* __restrict__ tile_sums) {
+__device__ __forceinline__ void scan_tiles_body(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                int64_t n, uint32_t* __restrict__ tile_sums, uint32_t blk) {
   __shared__ uint32_t sh[256];
   __shared__ uint32_t tot;
-  const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kSortItems;
+  const int64_t base = (int64_t)blk * kTile + (int64_t)threadIdx.x * kSortItems;
   uint32_t v[kSortItems], s = 0;
 #pragma unroll
   for (int t = 0; t < kSortItems; ++t) {
@@ -296,11 +332,17 @@ __global__ void __launch_bounds__(256)
     run += v[t];
   }
   __syncthreads();
-  if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+  if (threadIdx.x == 0) tile_sums[blk] = tot;
 }
 
 __global__ void __launch_bounds__(256)
-    k_scan_sums(uint32_t* __restrict__ sums, int ntiles, uint32_t* __restrict__ total) {
+    k_scan_tiles(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int64_t n,
+                 uint32_t* __restrict__ tile_sums) {
+  scan_tiles_body(in, out, n, tile_sums, blockIdx.x);
+}
+
+__device__ __forceinline__ void scan_sums_body(uint32_t* __restrict__ sums, int ntiles,
+                                               uint32_t* __restrict__ total) {
   __shared__ uint32_t sh[256];
   __shared__ uint32_t tot;
   uint32_t carry = 0;
@@ -317,38 +359,54 @@ __global__ void __launch_bounds__(256)
 }
 
 __global__ void __launch_bounds__(256)
-    k_scan_add(uint32_t* __restrict__ out, int64_t n, const uint32_t* __restrict__ sums) {
-  const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kSortItems;
-  const uint32_t add = sums[blockIdx.x];
+    k_scan_sums(uint32_t* __restrict__ sums, int ntiles, uint32_t* __restrict__ total) {
+  scan_sums_body(sums, ntiles, total);
+}
+
+__device__ __forceinline__ void scan_add_body(uint32_t* __restrict__ out, int64_t n,
+                                              const uint32_t* __restrict__ sums, uint32_t blk) {
+  const int64_t base = (int64_t)blk * kTile + (int64_t)threadIdx.x * kSortItems;
+  const uint32_t add = sums[blk];
 #pragma unroll
   for (int t = 0; t < kSortItems; ++t)
     if (base + t < n) out[base + t] += add;
 }
 
+__global__ void __launch_bounds__(256)
+    k_scan_add(uint32_t* __restrict__ out, int64_t n, const uint32_t* __restrict__ sums) {
+  scan_add_body(out, n, sums, blockIdx.x);
+}
+
 template <int BITS>
-__global__ void __launch_bounds__(kSortThreads)
-    k_radix_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
-                 uint32_t* __restrict__ hist, int ntiles) {
+__device__ __forceinline__ void radix_hist_body(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                                                uint32_t* __restrict__ hist, int ntiles, uint32_t blk) {
   constexpr int R = 1 << BITS;
   __shared__ uint32_t cnt[R];
   for (int d = threadIdx.x; d < R; d += kSortThreads) cnt[d] = 0;
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * kTile;
+  const int64_t base = (int64_t)blk * kTile;
 #pragma unroll
   for (int it = 0; it < kSortItems; ++it) {
     const int64_t i = base + it * kSortThreads + threadIdx.x;
     if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & (R - 1)], 1u);
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < R; d += kSortThreads) hist[(size_t)d * ntiles + blockIdx.x] = cnt[d];
+  for (int d = threadIdx.x; d < R; d += kSortThreads) hist[(size_t)d * ntiles + blk] = cnt[d];
+}
+
+template <int BITS>
+__global__ void __launch_bounds__(kSortThreads)
+    k_radix_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                 uint32_t* __restrict__ hist, int ntiles) {
+  radix_hist_body<BITS>(keys, n, shift, hist, ntiles, blockIdx.x);
 }
 
 // One workgroup per digit: exclusive scan of hist[d][0..ntiles) in place;
 // total[d] receives the digit's count.
-__global__ void __launch_bounds__(256)
-    k_radix_scan(uint32_t* __restrict__ hist, int ntiles, uint32_t* __restrict__ total) {
+__device__ __forceinline__ void radix_scan_body(uint32_t* __restrict__ hist, int ntiles,
+                                                uint32_t* __restrict__ total, uint32_t digit) {
   __shared__ uint32_t part[256];
-  uint32_t* row = hist + (size_t)blockIdx.x * ntiles;
+  uint32_t* row = hist + (size_t)digit * ntiles;
   uint32_t carry = 0;
   for (int c0 = 0; c0 < ntiles; c0 += 256) {
     const int i = c0 + threadIdx.x;
@@ -366,15 +424,21 @@ __global__ void __launch_bounds__(256)
     carry += part[255];
     __syncthreads();
   }
-  if (threadIdx.x == 0) total[blockIdx.x] = carry;
+  if (threadIdx.x == 0) total[digit] = carry;
+}
+
+__global__ void __launch_bounds__(256)
+    k_radix_scan(uint32_t* __restrict__ hist, int ntiles, uint32_t* __restrict__ total) {
+  radix_scan_body(hist, ntiles, total, blockIdx.x);
 }
 
 template <int BITS>
-__global__ void __launch_bounds__(kSortThreads)
-    k_radix_scatter(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
-                    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n,
-                    int shift, const uint32_t* __restrict__ hist, int ntiles,
-                    const uint32_t* __restrict__ total) {
+__device__ __forceinline__ void radix_scatter_body(const uint32_t* __restrict__ keys_in,
+                                                   const uint32_t* __restrict__ vals_in,
+                                                   uint32_t* __restrict__ keys_out,
+                                                   uint32_t* __restrict__ vals_out, int64_t n, int shift,
+                                                   const uint32_t* __restrict__ hist, int ntiles,
+                                                   const uint32_t* __restrict__ total, uint32_t blk) {
   constexpr int R = 1 << BITS;
   __shared__ uint32_t dbase[R];             // global start of each digit for this tile
   __shared__ uint32_t wcnt[kWaves][R];      // per-wave digit counts -> offsets
@@ -395,12 +459,12 @@ __global__ void __launch_bounds__(kSortThreads)
   }
   __syncthreads();
   for (int d = tid; d < R; d += kSortThreads) {
-    dbase[d] = run[d] + hist[(size_t)d * ntiles + blockIdx.x];
+    dbase[d] = run[d] + hist[(size_t)d * ntiles + blk];
   }
   __syncthreads();
   for (int d = tid; d < R; d += kSortThreads) run[d] = 0;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int64_t base = (int64_t)blockIdx.x * kTile;
+  const int64_t base = (int64_t)blk * kTile;
   for (int it = 0; it < kSortItems; ++it) {
     for (int d = tid; d < kWaves * R; d += kSortThreads) (&wcnt[0][0])[d] = 0;
     __syncthreads();
@@ -441,6 +505,15 @@ __global__ void __launch_bounds__(kSortThreads)
   }
 }
 
+template <int BITS>
+__global__ void __launch_bounds__(kSortThreads)
+    k_radix_scatter(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+                    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n,
+                    int shift, const uint32_t* __restrict__ hist, int ntiles,
+                    const uint32_t* __restrict__ total) {
+  radix_scatter_body<BITS>(keys_in, vals_in, keys_out, vals_out, n, shift, hist, ntiles, total, blockIdx.x);
+}
+
 // dst[i] = src[perm[i]] for every plane, material and id
 __global__ void __launch_bounds__(256)
     k_reorder(ParticlesDev src, ParticlesDev dst, const uint32_t* __restrict__ perm) {
@@ -473,9 +546,108 @@ hipError_t radix_sort_t(uint32_t* keys[2], uint32_t* vals[2], int64_t n, int sta
   return hipGetLastError();
 }
 
+// ---- many scenes per launch: workgroup g of the launch is workgroup
+// g - prefix[j] of scene j (prefix[j] <= g < prefix[j + 1])
+__device__ __forceinline__ int sort_scene_of(const uint32_t* __restrict__ prefix, int nscenes, uint32_t g) {
+  int lo = 0, hi = nscenes;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <int PH>
+__global__ void __launch_bounds__(256)
+    k_sort_multi(const mpmk::SortDesc* __restrict__ descs, const uint32_t* __restrict__ prefix, int nscenes,
+                 int pass) {
+  const uint32_t g = blockIdx.x;
+  const int j = sort_scene_of(prefix, nscenes, g);
+  const mpmk::SortDesc& d = descs[j];
+  const uint32_t blk = g - prefix[j];
+  const int out = d.npasses & 1;
+  const int shift = 3 * d.lb;
+  if constexpr (PH == mpmk::SORT_FILL) {
+    const int64_t i = (int64_t)blk * 256 + threadIdx.x;
+    const int64_t nnb = (int64_t)d.nbx * d.nbx * d.nbx;
+    if (i < 2 * d.nblk) d.block_items[i] = 0u;
+    else if (i < 2 * d.nblk + nnb) d.covered[i - 2 * d.nblk] = 0u;
+  } else if constexpr (PH == mpmk::SORT_KEYS) {
+    cell_keys_body(d.P, d.k, d.lb, d.keys[0], d.vals[0], blk);
+  } else if constexpr (PH == mpmk::SORT_HIST) {
+    const int ntiles = (int)((d.n + kTile - 1) / kTile);
+    radix_hist_body<8>(d.keys[pass & 1], d.n, d.start_bit + 8 * pass, d.scratch, ntiles, blk);
+  } else if constexpr (PH == mpmk::SORT_RSCAN) {
+    const int ntiles = (int)((d.n + kTile - 1) / kTile);
+    radix_scan_body(d.scratch, ntiles, d.scratch + (size_t)256 * ntiles, blk);
+  } else if constexpr (PH == mpmk::SORT_SCATTER) {
+    const int ntiles = (int)((d.n + kTile - 1) / kTile);
+    radix_scatter_body<8>(d.keys[pass & 1], d.vals[pass & 1], d.keys[(pass + 1) & 1], d.vals[(pass + 1) & 1],
+                          d.n, d.start_bit + 8 * pass, d.scratch, ntiles, d.scratch + (size_t)256 * ntiles,
+                          blk);
+  } else if constexpr (PH == mpmk::SORT_HEADS) {
+    block_heads_body(d.keys[out], d.n, shift, d.block_range, blk);
+  } else if constexpr (PH == mpmk::SORT_FLAGS) {
+    item_flags_body(d.keys[out], d.n, shift, d.chunk, d.block_range, d.flag, blk);
+  } else if constexpr (PH == mpmk::SORT_SCAN_TILES) {
+    scan_tiles_body(d.flag, d.pos, d.n, d.scratch, blk);
+  } else if constexpr (PH == mpmk::SORT_SCAN_SUMS) {
+    scan_sums_body(d.scratch, (int)((d.n + kTile - 1) / kTile), d.item_count);
+  } else if constexpr (PH == mpmk::SORT_SCAN_ADD) {
+    scan_add_body(d.pos, d.n, d.scratch, blk);
+  } else if constexpr (PH == mpmk::SORT_EMIT) {
+    item_emit_body(d.keys[out], d.n, shift, d.flag, d.pos, d.item_count, d.item_start, d.item_block,
+                   d.block_items, d.covered, d.nbx, blk);
+  } else {
+    item_arrange_body(d.keys[out], d.vals[out], shift, d.item_start, d.item_count, d.gather,
+                      d.item_waves, blk);
+  }
+}
+
 }  // namespace
 
 namespace mpmk {
+
+int64_t sort_blocks(const SortDesc& d, int phase, int pass) {
+  const int64_t per256 = (d.n + 255) / 256, tiles = (d.n + kTile - 1) / kTile;
+  switch (phase) {
+    case SORT_FILL: return (2 * d.nblk + (int64_t)d.nbx * d.nbx * d.nbx + 255) / 256;
+    case SORT_KEYS: case SORT_HEADS: case SORT_FLAGS: case SORT_EMIT: return per256;
+    case SORT_HIST: case SORT_SCATTER: return pass < d.npasses ? tiles : 0;
+    case SORT_RSCAN: return pass < d.npasses ? 256 : 0;
+    case SORT_SCAN_TILES: case SORT_SCAN_ADD: return tiles;
+    case SORT_SCAN_SUMS: return 1;
+    case SORT_ARRANGE: return d.max_items;
+    default: return 0;
+  }
+}
+
+hipError_t sort_multi(int phase, int pass, const SortDesc* descs, const uint32_t* prefix, int nscenes,
+                      int64_t total_blocks, hipStream_t s) {
+  if (nscenes <= 0 || total_blocks <= 0) return hipSuccess;
+  if (!descs || !prefix || total_blocks > 0x7fffffff) return hipErrorInvalidValue;
+  const dim3 g((unsigned)total_blocks), b(256);
+  switch (phase) {
+#define MPM_SORT_CASE(PH) \
+    case PH: hipLaunchKernelGGL(k_sort_multi<PH>, g, b, 0, s, descs, prefix, nscenes, pass); break;
+    MPM_SORT_CASE(SORT_FILL)
+    MPM_SORT_CASE(SORT_KEYS)
+    MPM_SORT_CASE(SORT_HIST)
+    MPM_SORT_CASE(SORT_RSCAN)
+    MPM_SORT_CASE(SORT_SCATTER)
+    MPM_SORT_CASE(SORT_HEADS)
+    MPM_SORT_CASE(SORT_FLAGS)
+    MPM_SORT_CASE(SORT_SCAN_TILES)
+    MPM_SORT_CASE(SORT_SCAN_SUMS)
+    MPM_SORT_CASE(SORT_SCAN_ADD)
+    MPM_SORT_CASE(SORT_EMIT)
+    MPM_SORT_CASE(SORT_ARRANGE)
+#undef MPM_SORT_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
 
 int cell_key_bits(int N, int lb) {
   const int nb = (N + (1 << lb) - 1) >> lb;
