@@ -194,6 +194,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=128)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -214,12 +215,21 @@ def main() -> int:
     eng.add_particles(st["x"], st["material"], st["v"], st["C"], st["F"], st["Jp"])
     eng.upload()
     eng.advance(args.warmup)
+    tuned = None
+    if not args.no_autotune:
+        # the library's autotuner picks the sort interval and work-item size
+        # for this scene (from a snapshot; the state is put back)
+        tuned = eng.autotune(trial_substeps=64)
+        # back to the steady state under the chosen options (the tuner
+        # restored the state as it was, with the items to rebuild)
+        eng.advance(max(args.warmup, 2 * eng.get_option("sort_every")))
     torch.cuda.synchronize()
 
     # timed region: events on the launch stream bracket the kernel phases of
-    # one substep in every 16, rotating through the 16 positions of the sort
-    # interval (every substep timed would add ~8 us of event packets to each)
-    eng.set_option("timing", TIMING_EVERY)
+    # one substep in every k (k = the sort interval), rotating through the k
+    # positions of the interval (every substep timed would add ~8 us of event
+    # packets to each)
+    eng.set_option("timing", eng.get_option("sort_every") or TIMING_EVERY)
     eng.timing_reset()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -232,6 +242,7 @@ def main() -> int:
         ms, n = eng.timing(ph)
         if n:
             phases[ph] = {"mean_us": 1e3 * ms / n, "launches": n}
+    eng_opts = {k: eng.get_option(k) for k in ("sort_every", "chunk")}
     out = eng.download()
     eng.close()
     finite = bool(torch.isfinite(torch.from_numpy(out["x"])).all())
@@ -261,6 +272,8 @@ def main() -> int:
                          "snow_spheres": "snow E=1.4e5 nu=0.2 rho=400 xi=10",
                          "elastic_sphere": "elastic (snow model, clamps [0,1e30]) E=1e4 nu=0.3"}[args.workload],
             "substep_kernels": phases,
+            "options": {"sort_every": eng_opts["sort_every"], "chunk": eng_opts["chunk"],
+                        "autotuned": tuned is not None},
             "algorithmic_bytes_per_substep": scenes.algorithmic_bytes(P, N),
             "substep_fraction_of_hbm_roofline":
                 scenes.algorithmic_bytes(P, N) / sec_per_step / (HBM_PEAK_GBS * 1e9),

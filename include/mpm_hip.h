@@ -193,6 +193,16 @@ int mpm_get_option(mpm_handle h, const char* name, int64_t* value);
 int mpm_timing_read(mpm_handle h, const char* phase, double* total_ms, int64_t* launches);
 int mpm_timing_reset(mpm_handle h);
 
+/* ---- autotuner (no reference counterpart) ----
+ * Times `trial_substeps` substeps (at least two sort intervals) of the
+ * uploaded scene under each candidate sort interval (8, 12, 16, 24, 32, 48
+ * substeps) and work-item size (192 and 128 particles), each from the same
+ * snapshot of the current state, after one interval of warm-up; keeps the
+ * fastest as the "sort_every" and "chunk" options (tiled fused mode) and
+ * puts the state and simulated time back as they were. The report (JSON,
+ * truncated to report_len) lists the trials. Blocking. */
+int mpm_autotune(mpm_handle h, int32_t trial_substeps, char* report, int32_t report_len, void* stream);
+
 /* ---- kernel-level entry points used by the parity tests ---- */
 /* Run only the particle-to-grid scatter of the current state into a cleared
  * grid and copy the raw grid (momentum, mass: no normalisation) to host

@@ -109,6 +109,7 @@ SIGNATURES = {
     "mpm_get_option": (C.c_int, [_P, C.c_char_p, _i64p]),
     "mpm_timing_read": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), _i64p]),
     "mpm_timing_reset": (C.c_int, [_P]),
+    "mpm_autotune": (C.c_int, [_P, C.c_int32, C.c_char_p, C.c_int32, _P]),
     "mpm_debug_p2g": (C.c_int, [_P, _f, _P]),
     "mpm_debug_sort": (C.c_int, [_P, _u32, _u32, _P]),
     "mpm_debug_svd": (C.c_int, [C.c_int64, _f, _f, _f, _f]),

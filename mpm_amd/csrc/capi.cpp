@@ -201,6 +201,16 @@ extern "C" int mpm_timing_read(mpm_handle h, const char* phase, double* total_ms
   return s->timing_read(phase, total_ms, launches);
 }
 
+extern "C" int mpm_autotune(mpm_handle h, int32_t trial_substeps, char* report, int32_t report_len,
+                            void* stream) {
+  MPM_GET(h);
+  std::string r;
+  const int rc = s->autotune(trial_substeps, &r, (hipStream_t)stream);
+  if (rc) return rc;
+  if (report && report_len > 0) std::snprintf(report, (size_t)report_len, "%s", r.c_str());
+  return MPM_OK;
+}
+
 extern "C" int mpm_timing_reset(mpm_handle h) {
   MPM_GET(h);
   return s->timing_reset();

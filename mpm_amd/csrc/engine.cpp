@@ -932,6 +932,109 @@ int Sim::load_state(const char* path) {
   return MPM_OK;
 }
 
+int Sim::autotune(int32_t trial_substeps, std::string* report, hipStream_t s) {
+  ++gen_;
+  if (!uploaded_ || P_ <= 0) return fail(MPM_ERR_STATE, "mpm_autotune: nothing uploaded");
+  if (trial_substeps < 1 || trial_substeps > 100000)
+    return fail(MPM_ERR_INVALID, "mpm_autotune: trial_substeps must be 1..100000");
+  int rc = check_materials_cover_particles("mpm_autotune");
+  if (rc) return rc;
+  rc = upload_materials();
+  if (rc) return rc;
+  rc = ensure_f(s);
+  if (rc) return rc;
+  // snapshot of the current particle copy
+  const size_t plane_bytes = sizeof(float) * NUM_PLANES * stride_;
+  float* snap_f = nullptr;
+  uint8_t* snap_m = nullptr;
+  uint32_t* snap_i = nullptr;
+  HIPCHK(hipMalloc(&snap_f, plane_bytes), "hipMalloc(autotune snapshot)");
+  hipError_t e = hipMalloc(&snap_m, (size_t)P_);
+  if (e == hipSuccess) e = hipMalloc(&snap_i, sizeof(uint32_t) * (size_t)P_);
+  if (e == hipSuccess) e = hipMemcpyAsync(snap_f, d_f_[cur_], plane_bytes, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(snap_m, d_mat_[cur_], (size_t)P_, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(snap_i, d_id_[cur_], sizeof(uint32_t) * (size_t)P_, hipMemcpyDeviceToDevice, s);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  const Options saved = opt_;
+  const double t0 = t_;
+  const bool canon = f_canonical_;
+  auto restore = [&]() -> hipError_t {
+    hipError_t r = hipMemcpyAsync(d_f_[0], snap_f, plane_bytes, hipMemcpyDeviceToDevice, s);
+    if (r == hipSuccess) r = hipMemcpyAsync(d_mat_[0], snap_m, (size_t)P_, hipMemcpyDeviceToDevice, s);
+    if (r == hipSuccess) r = hipMemcpyAsync(d_id_[0], snap_i, sizeof(uint32_t) * (size_t)P_, hipMemcpyDeviceToDevice, s);
+    cur_ = 0;
+    items_valid_ = false;
+    primed_ = false;
+    pending_gather_ = false;
+    steps_since_sort_ = 1 << 30;
+    side_clean_ = false;
+    vel_grid_ = nullptr;
+    f_stale_ = false;
+    f_canonical_ = canon;
+    t_ = t0;
+    return r;
+  };
+  struct Trial { int64_t sort_every, chunk; double us; };
+  std::vector<Trial> trials;
+  const int64_t sorts[] = {8, 12, 16, 24, 32, 48};
+  const int64_t chunks[] = {mpmk::kMaxItemParticles, 128};
+  for (int64_t ch : chunks) {
+    for (int64_t se : sorts) {
+      if (e != hipSuccess) break;
+      e = restore();
+      if (e != hipSuccess) break;
+      opt_ = saved;
+      opt_.p2g_mode = 1;
+      opt_.fused = 1;
+      opt_.timing = 0;
+      opt_.sort_every = se;
+      opt_.chunk = ch;
+      const int32_t n = std::max<int32_t>(trial_substeps, (int32_t)(2 * se));
+      rc = advance((int32_t)se, s);  // one sort interval to the steady state
+      if (!rc) e = hipEventRecord(e0, s);
+      if (!rc && e == hipSuccess) rc = advance(n, s);
+      if (!rc && e == hipSuccess) e = hipEventRecord(e1, s);
+      if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (rc) break;
+      trials.push_back({se, ch, 1e3 * (double)ms / n});
+    }
+  }
+  const hipError_t er = restore();
+  if (e == hipSuccess) e = er;
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(snap_f);
+  (void)hipFree(snap_m);
+  (void)hipFree(snap_i);
+  opt_ = saved;
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "mpm_autotune");
+  size_t best = 0;
+  for (size_t i = 1; i < trials.size(); ++i)
+    if (trials[i].us < trials[best].us) best = i;
+  opt_.p2g_mode = 1;
+  opt_.fused = 1;
+  opt_.sort_every = trials[best].sort_every;
+  opt_.chunk = trials[best].chunk == mpmk::kMaxItemParticles ? 0 : trials[best].chunk;
+  if (report) {
+    char buf[160];
+    *report = "{\"sort_every\": " + std::to_string(opt_.sort_every) + ", \"chunk\": " +
+              std::to_string(trials[best].chunk) + ", \"trials\": [";
+    for (size_t i = 0; i < trials.size(); ++i) {
+      std::snprintf(buf, sizeof(buf), "%s{\"sort_every\": %lld, \"chunk\": %lld, \"us_per_substep\": %.3f}",
+                    i ? ", " : "", (long long)trials[i].sort_every, (long long)trials[i].chunk, trials[i].us);
+      *report += buf;
+    }
+    *report += "]}";
+  }
+  return MPM_OK;
+}
+
 int Sim::set_option(const char* name, int64_t v) {
   ++gen_;
   if (!name) return fail(MPM_ERR_INVALID, "mpm_set_option: null name");
@@ -975,7 +1078,7 @@ int Sim::get_option(const char* name, int64_t* v) {
   else if (n == "p2g_mode") *v = opt_.p2g_mode;
   else if (n == "sort_every") *v = opt_.sort_every;
   else if (n == "radix_bits") *v = opt_.radix_bits;
-  else if (n == "chunk") *v = opt_.chunk > 0 ? opt_.chunk : chunk_;
+  else if (n == "chunk") *v = opt_.chunk > 0 ? opt_.chunk : mpmk::kMaxItemParticles;
   else if (n == "block") *v = 1 << lb_;
   else if (n == "timing") *v = opt_.timing;
   else if (n == "derive_f") *v = opt_.derive_f;

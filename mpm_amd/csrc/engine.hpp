@@ -66,6 +66,10 @@ class Sim {
   int debug_sort(uint32_t* keys, uint32_t* perm, hipStream_t s);
   int timing_read(const char* phase, double* total_ms, int64_t* launches);
   int timing_reset();
+  // measures the substep under candidate sort intervals and item sizes from
+  // a snapshot of the current state, keeps the fastest, restores the state;
+  // *report receives the trials as JSON
+  int autotune(int32_t trial_substeps, std::string* report, hipStream_t s);
 
   // ---- replay of captured substeps (batch.cpp) ----
   // A capture cycle is 2 sort intervals: it starts when the next substep sorts

@@ -179,6 +179,35 @@ class Engine:
     def timing_reset(self) -> None:
         self._chk(lib().mpm_timing_reset(self._h))
 
+    def autotune(self, trial_substeps: int = 64, cache: Optional[str] = None,
+                 key: Optional[str] = None, stream=None) -> dict:
+        """Pick the sort interval and work-item size by timing them on this
+        scene (mpm_autotune; the state is restored). With `cache` (a JSON
+        file) and `key`, a stored choice is applied without timing, and a new
+        one is stored. Returns the choice (and the trials when timed)."""
+        import json
+        import os
+        if cache and key and os.path.exists(cache):
+            with open(cache) as f:
+                stored = json.load(f)
+            if key in stored:
+                choice = stored[key]
+                self.set_option("sort_every", int(choice["sort_every"]))
+                self.set_option("chunk", int(choice["chunk"]))
+                return choice
+        buf = C.create_string_buffer(8192)
+        self._chk(lib().mpm_autotune(self._h, int(trial_substeps), buf, len(buf), _stream(stream)))
+        choice = json.loads(buf.value.decode())
+        if cache and key:
+            stored = {}
+            if os.path.exists(cache):
+                with open(cache) as f:
+                    stored = json.load(f)
+            stored[key] = {"sort_every": choice["sort_every"], "chunk": choice["chunk"]}
+            with open(cache, "w") as f:
+                json.dump(stored, f, indent=1, sort_keys=True)
+        return choice
+
     # -- results
     def download(self, stream=None) -> dict:
         n = self._n
