@@ -89,7 +89,7 @@ Sim::~Sim() {
   if (d_hist_) (void)hipFree(d_hist_);
   if (d_mats_) (void)hipFree(d_mats_);
   for (uint32_t* p : {d_block_start_, d_block_items_, d_flag_, d_pos_, d_gather_, d_item_start_,
-                      d_item_block_, d_item_waves_, d_item_count_, d_dirty_, d_covered_})
+                      d_item_block_, d_item_waves_, d_item_count_, d_dirty_, d_covered_, d_live_})
     if (p) (void)hipFree(p);
   if (d_tiles_) (void)hipFree(d_tiles_);
   for (auto& p : ev_) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -197,7 +197,7 @@ int Sim::ensure_grids() {
   }
   grid_nodes_ = nodes;
   const size_t nbx = ((size_t)prm_.N + 3) / 4;
-  for (uint32_t** p : {&d_dirty_, &d_covered_}) {
+  for (uint32_t** p : {&d_dirty_, &d_covered_, &d_live_}) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
     HIPCHK(hipMalloc(p, nbx * nbx * nbx * sizeof(uint32_t)), "hipMalloc(node block flags)");
@@ -582,6 +582,8 @@ mpmk::ReduceDesc Sim::reduce_desc() const {
   r.update = 1;
   r.combined = 0;
   r.nblk = nblk_;
+  r.live = d_live_;
+  r.skip_dead = 1;
   return r;
 }
 
@@ -625,6 +627,7 @@ mpmk::TileSums Sim::tile_sums() const {
   t.dirty = d_dirty_;
   t.block_items = d_block_items_;
   t.covered = d_covered_;
+  t.live = d_live_;
   return t;
 }
 
@@ -667,6 +670,7 @@ int Sim::substep(hipStream_t s) {
       PHASE(PH_GRID_UPDATE, HIPCHK(mpmk::tile_reduce(tile_sums(), k_, d_grid_[0], 1, s), "grid update"));
       PHASE(PH_G2P, HIPCHK(mpmk::g2p(particles(), d_mats_, k_, d_grid_[0], s), "g2p"));
       vel_grid_ = d_grid_[0];
+      grid_dead_ = true;
       primed_ = false;
       ++steps_since_sort_;
       return MPM_OK;
@@ -686,11 +690,13 @@ int Sim::substep(hipStream_t s) {
     PHASE(PH_G2P2G, rc = run_tiled(d_grid_[0], 1, s));
     if (rc) return rc;
     vel_grid_ = d_grid_[0];
+    grid_dead_ = true;
     ++steps_since_sort_;
     return MPM_OK;
   }
   // direct P2G (global atomics), two grids
   side_clean_ = false;
+  grid_dead_ = false;
   float4* gA = d_grid_[gcur_];
   float4* gB = d_grid_[gcur_ ^ 1];
   if (!opt_.fused) {
@@ -808,6 +814,11 @@ int Sim::download_aos104(void* out, hipStream_t s) {
 int Sim::download_grid(float* grid, hipStream_t s) {
   if (!grid) return fail(MPM_ERR_INVALID, "mpm_download_grid: null output");
   if (!vel_grid_) return fail(MPM_ERR_STATE, "mpm_download_grid: no grid update has run yet");
+  if (grid_dead_ && vel_grid_ == d_grid_[0]) {
+    // node blocks the tiled grid update skipped hold old values: zero them
+    HIPCHK(mpmk::zero_dead_blocks(d_grid_[0], d_live_, k_.N, s), "zero dead node blocks");
+    grid_dead_ = false;
+  }
   HIPCHK(hipMemcpyAsync(grid, vel_grid_, grid_nodes_ * sizeof(float4), hipMemcpyDeviceToHost, s), "download grid");
   HIPCHK(hipStreamSynchronize(s), "download grid sync");
   return MPM_OK;

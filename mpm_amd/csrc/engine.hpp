@@ -120,6 +120,7 @@ class Sim {
   // the end of a substep: velocity grid, sort counter, time
   void substep_done() {
     vel_grid_ = d_grid_[0];
+    grid_dead_ = true;
     ++steps_since_sort_;
     t_ += (double)prm_.dt;
   }
@@ -192,6 +193,8 @@ class Sim {
   bool side_clean_ = false;
   uint32_t* d_dirty_ = nullptr;      // one flag per 4^3 node block
   uint32_t* d_covered_ = nullptr;    // one flag per 4^3 node block (set at the sort)
+  uint32_t* d_live_ = nullptr;       // one flag per 4^3 node block (set by the grid update)
+  bool grid_dead_ = false;           // the velocity grid holds stale values in dead blocks
   size_t dirty_cap_ = 0;
 
   // work items of the tiled kernel (valid after a sort)

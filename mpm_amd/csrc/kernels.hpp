@@ -80,6 +80,7 @@ struct TileSums {
   uint32_t* dirty;
   const uint32_t* block_items;  // 2 entries per home block (Morton space)
   const uint32_t* covered;      // per node block: some tile reaches it
+  uint32_t* live;               // per node block: written by the last grid update
 };
 // With `gather`, P is the particle order before the last sort: thread slot i
 // reads particle gather[i] of P and writes it to slot i of Pdst (the sort's
@@ -114,6 +115,12 @@ struct ReduceDesc {
   int update;
   int combined;   // the items of each home block were summed into its first tile
   int64_t nblk;   // home blocks (Morton space): the combine's workgroups
+  // live[nb] = some tile or side-grid scatter reaches node block nb; with
+  // skip_dead the other blocks keep their old values instead of zeros (no
+  // particle stencil reads them: a particle's G2P reads the nodes its own
+  // P2G reached)
+  uint32_t* live;
+  int skip_dead;
 };
 // flags of the fused pass: G2P of the staged grid first (else P2G only);
 // F derived from J (liquid kinds only: F planes neither read nor written);
@@ -132,6 +139,8 @@ hipError_t g2p2g_multi(const ItemDesc* descs, const uint32_t* prefix, int nscene
                        int kinds, hipStream_t s);
 hipError_t tile_reduce_multi(const ReduceDesc* descs, const uint32_t* prefix, int nscenes,
                              int64_t total_blocks, hipStream_t s);
+// zero the node blocks that the last grid update left out (before a download)
+hipError_t zero_dead_blocks(float4* grid, const uint32_t* live, int N, hipStream_t s);
 // sums the tiles of every home block holding several items into its first
 // item's tile, in item order (prefix over the scenes' nblk); a reduction with
 // `combined` set then reads one tile per home block

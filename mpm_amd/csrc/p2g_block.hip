@@ -672,8 +672,10 @@ __device__ __forceinline__ void reduce_block(const mpmk::ReduceDesc& d, const in
   const bool in_grid = x < N && y < N && z < N;
   const size_t node = ((size_t)x * N + (size_t)y) * N + (size_t)z;
   const bool is_dirty = dirty[nbk] != 0u;
-  if (!covered[nbk] && !is_dirty) {  // no tile and no side-grid scatter reaches it
-    if (in_grid) grid[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+  const bool live = covered[nbk] || is_dirty;
+  if (d.live && lane == 0) d.live[nbk] = live ? 1u : 0u;
+  if (!live) {  // no tile and no side-grid scatter reaches it
+    if (!d.skip_dead && in_grid) grid[node] = make_float4(0.f, 0.f, 0.f, 0.f);
     return;
   }
   uint32_t r0 = 0, r1 = 0;
@@ -881,6 +883,22 @@ hipError_t liquid_f(const ParticlesDev& P, hipStream_t s) {
   return hipGetLastError();
 }
 
+// node block of each node: zero where the last grid update skipped it
+__global__ void __launch_bounds__(256) k_zero_dead(float4* __restrict__ grid, const uint32_t* __restrict__ live, int N) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t nodes = (size_t)N * N * N;
+  if (t >= nodes) return;
+  const int z = (int)(t % N), y = (int)((t / N) % N), x = (int)(t / ((size_t)N * N));
+  const int nbx = (N + 3) / 4;
+  if (!live[((size_t)(x >> 2) * nbx + (y >> 2)) * nbx + (z >> 2)]) grid[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+hipError_t zero_dead_blocks(float4* grid, const uint32_t* live, int N, hipStream_t s) {
+  const size_t nodes = (size_t)N * N * N;
+  hipLaunchKernelGGL(k_zero_dead, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, s, grid, live, N);
+  return hipGetLastError();
+}
+
 hipError_t tile_reduce(const TileSums& in, const StepConsts& k, float4* grid, int update,
                        hipStream_t s) {
   if (!in.tiles || !in.atomic || !in.dirty || !in.block_items || !in.covered || !grid)
@@ -897,6 +915,8 @@ hipError_t tile_reduce(const TileSums& in, const StepConsts& k, float4* grid, in
   d.update = update;
   d.combined = 0;
   d.nblk = 0;
+  d.live = in.live;
+  d.skip_dead = update && in.live;
   const int64_t nblocks = (int64_t)d.nbx * d.nbx * d.nbx;
   hipLaunchKernelGGL(k_tile_reduce, dim3((unsigned)((nblocks + 3) / 4)), dim3(256), 0, s, d);
   return hipGetLastError();
