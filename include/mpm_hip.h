@@ -175,7 +175,10 @@ int mpm_load_state(mpm_handle h, const char* path);
  *                implicitly as cbrt(Jp) I (the value it would store), saving
  *                72 bytes of particle traffic per substep; downloads and the
  *                other paths see the same F. 0: F planes read and written.
- *   "timing"     k > 0: record HIP events around the kernel phases of one
+ *   "xcd_map"    1 (default): the fused kernel hands each of the 8 XCDs a
+ *                contiguous run of work items (neighbouring home blocks share
+ *                its L2); 0: the hardware's round-robin order
+ *   "timing"    k > 0: record HIP events around the kernel phases of one
  *                substep in every k (k = 1: every substep); 0 (default): off
  * Results do not depend on the options beyond FP32 summation order. */
 int mpm_set_option(mpm_handle h, const char* name, int64_t value);

@@ -532,6 +532,7 @@ int Sim::fused_pass(const float4* grid_v, int do_g2p, mpmk::ItemDesc& d, int& ki
   int flags = do_g2p ? mpmk::TILED_G2P : 0;
   if (derive) flags |= mpmk::TILED_DERIVE_F;
   if (P_ > 0 && (uint64_t)(P_ - 1) > (uint64_t)k_.guard) flags |= mpmk::TILED_GUARD;
+  if (opt_.xcd_map) flags |= mpmk::TILED_XCD;
   const ParticlesDev P = particles();
   d.P = P;
   d.Pdst = P;
@@ -1059,6 +1060,7 @@ int Sim::set_option(const char* name, int64_t v) {
   }
   if (n == "sort_every") { if (v < 0) return fail(MPM_ERR_INVALID, "sort_every must be >= 0"); opt_.sort_every = v; return MPM_OK; }
   if (n == "derive_f") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "derive_f must be 0 or 1"); opt_.derive_f = v; return MPM_OK; }
+  if (n == "xcd_map") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "xcd_map must be 0 or 1"); opt_.xcd_map = v; return MPM_OK; }
   if (n == "radix_bits") { if (v != 4 && v != 6 && v != 8) return fail(MPM_ERR_INVALID, "radix_bits must be 4, 6 or 8"); opt_.radix_bits = v; return MPM_OK; }
   if (n == "chunk") {
     if (v < 0 || v > mpmk::kMaxItemParticles || v % 64 != 0)
@@ -1093,6 +1095,7 @@ int Sim::get_option(const char* name, int64_t* v) {
   else if (n == "block") *v = 1 << lb_;
   else if (n == "timing") *v = opt_.timing;
   else if (n == "derive_f") *v = opt_.derive_f;
+  else if (n == "xcd_map") *v = opt_.xcd_map;
   else return fail(MPM_ERR_INVALID, "mpm_get_option: unknown option '%s'", name);
   return MPM_OK;
 }

@@ -28,6 +28,7 @@ struct Options {
   int64_t chunk = 0;        // "chunk": max particles per work item (64..192, multiple of 64; 0 = 192)
   int64_t block = 0;        // "block": edge of a home block in cells (4; 0 = default)
   int64_t derive_f = 1;     // "derive_f": liquid-only scenes keep F = cbrt(J) I implicit
+  int64_t xcd_map = 1;      // "xcd_map": XCD-contiguous order of the fused kernel's items
   int64_t timing = 0;       // "timing": k > 0 brackets the kernels of one substep in every k
                             // with HIP events (per-phase time); 0 off
 };

@@ -124,8 +124,13 @@ struct ReduceDesc {
 };
 // flags of the fused pass: G2P of the staged grid first (else P2G only);
 // F derived from J (liquid kinds only: F planes neither read nor written);
-// the original-index guard may bind (some particle id exceeds k.guard)
-enum TiledFlags : int { TILED_G2P = 1, TILED_DERIVE_F = 2, TILED_GUARD = 4 };
+// the original-index guard may bind (some particle id exceeds k.guard);
+// XCD-contiguous item order: the hardware hands workgroup b to XCD b % 8, so
+// workgroup b runs item (b % 8) * ceil(n / 8) + b / 8 and each XCD's L2 serves
+// a contiguous run of items (neighbouring home blocks share staged nodes)
+enum TiledFlags : int { TILED_G2P = 1, TILED_DERIVE_F = 2, TILED_GUARD = 4, TILED_XCD = 8 };
+// launches of the XCD-ordered kernels round their grid up to a multiple of this
+constexpr int kXcds = 8;
 hipError_t g2p2g_block(const ParticlesDev& P, const ParticlesDev& Pdst, const uint32_t* gather,
                        const MaterialDev* mats, int nmats, const StepConsts& k,
                        const WorkItems& w, const float4* grid_v, const TileSums& out,

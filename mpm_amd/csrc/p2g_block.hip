@@ -506,9 +506,12 @@ __device__ __forceinline__ void g2p2g_item(const mpmk::ItemDesc& d, const int it
   //    wavefront with DPP lane shifts leaves each run's total in its last
   //    lane: shifts of 1, 2, 4, 8 within rows of 16 and the two
   //    row-broadcast steps that carry sums across rows. Done only when some
-  //    run is longer than 4; shorter runs go through step 3 lane by lane.
-  // 3. the writers (run tails after a reduction, else every lane with a tile
-  //    particle) add their 9 nodes with plain read-modify-writes, in rounds:
+  //    run is longer than 4. Shorter runs merge pairs of lanes with one
+  //    whole-wavefront shift (odd ranks add their left neighbour) and go
+  //    through step 3 with at most two writers per run.
+  // 3. the writers (run tails after a reduction, odd ranks and even tails
+  //    after a pair merge, else every lane with a tile particle) add their
+  //    9 nodes with plain read-modify-writes, in rounds:
   //    every round a writer claims its cell (one byte per cell) and only the
   //    lane whose claim stands writes (particles that moved since the last
   //    sort can also put one cell in two runs). Within
@@ -524,18 +527,21 @@ __device__ __forceinline__ void g2p2g_item(const mpmk::ItemDesc& d, const int it
   const bool tail = next != cid1;
   const int maxrun = __builtin_amdgcn_readlane(wave_max_scan(rank + 1), 63);
   const bool reduce = maxrun > 4;
+  // runs of 2 to 4: one wavefront shift merges pairs, the odd ranks taking
+  // their left neighbour's sums; a run then has at most 2 writers
+  const bool pair = !reduce && maxrun > 1;
+  const bool odd = (rank & 1) != 0;
   const int lr = lane & 15;
 
   typedef __attribute__((address_space(3))) volatile uint8_t lds_byte;
   lds_byte* const claim = (lds_byte*)(&claims[wave][0]);
   float4* const my = acc[wave];
   const StencilSlots slot(l0, l1, l2);
-  // Writers (run tails after a reduction, else every lane with a tile
-  // particle) claim their cell; the lanes whose claim stands add their nodes,
+  // Writers (see step 3 above) claim their cell; the lanes whose claim stands add their nodes,
   // one slab of 9 nodes (offset a along x) at a time so that 36 sums sit in
   // registers. Writers that lost (a cell with more than one writer in the
   // wavefront) retry slab by slab in further claim rounds.
-  const bool writer = pending && (!reduce || tail);
+  const bool writer = pending && (reduce ? tail : (!pair || odd || tail));
   if (writer) claim[cell] = (uint8_t)lane;
   const bool mine = writer && claim[cell] == (uint8_t)lane;
   const bool lost = writer && !mine;
@@ -555,6 +561,14 @@ __device__ __forceinline__ void g2p2g_item(const mpmk::ItemDesc& d, const int it
         cv[o][1] = wt * (s1 + q2[1][c]);
         cv[o][2] = wt * (s2 + q2[2][c]);
         cv[o][3] = wt * m.mass;
+        if (pair) {  // odd ranks add their left neighbour (whole-wavefront shift)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const float t_ = __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+                0, __builtin_bit_cast(int, cv[o][q]), 0x138 /* wave_shr:1 */, 0xf, 0xf, true));
+            cv[o][q] = odd ? cv[o][q] + t_ : cv[o][q];
+          }
+        }
       }
     }
 #define MPM_SEG_STEP(CTRL, ROWS, COND)                                                      \
@@ -617,11 +631,24 @@ __device__ __forceinline__ void g2p2g_item(const mpmk::ItemDesc& d, const int it
   }
 }
 
+// Workgroup b of a launch of g workgroups over n units, XCD-contiguous: the
+// units are cut into kXcds runs of ceil(n / kXcds) and XCD b % kXcds takes
+// its run in order. Returns n (no unit) for the workgroups beyond, and b
+// itself when the launch is too small to remap.
+__device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t g, uint32_t n) {
+  constexpr uint32_t X = mpmk::kXcds;
+  const uint32_t per = (n + X - 1) / X;
+  if (X * per > g) return b;
+  const uint32_t j = b / X;
+  return j < per ? (b % X) * per + j : n;
+}
+
 template <int KS>
 __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
     __attribute__((amdgpu_waves_per_eu(KS == KINDS_LIQUID ? 5 : 4)))
     k_g2p2g_tiled(mpmk::ItemDesc d) {
-  g2p2g_item<KS>(d, (int)blockIdx.x);
+  const uint32_t b = blockIdx.x;
+  g2p2g_item<KS>(d, (int)((d.flags & mpmk::TILED_XCD) ? xcd_order(b, gridDim.x, *d.item_count) : b));
 }
 
 // the scene of global workgroup or block g: the last j with prefix[j] <= g
@@ -802,9 +829,10 @@ hipError_t g2p2g_desc(const ItemDesc& d, int nmats, int64_t launch, int kinds, h
   if (nmats < 1 || nmats > MPM_MAX_MATERIALS) return hipErrorInvalidValue;
   if ((d.flags & TILED_G2P) && !d.grid_v) return hipErrorInvalidValue;
   if (!d.tiles || !d.grid_atomic || !d.dirty) return hipErrorInvalidValue;
-  if (launch > 0x7fffffff) return hipErrorInvalidValue;
+  if (launch > 0x7fffffff - kXcds) return hipErrorInvalidValue;
   if (d.gather && (d.Pdst.f == d.P.f || d.Pdst.n != d.P.n)) return hipErrorInvalidValue;
   if (!d.gather && d.Pdst.f != d.P.f) return hipErrorInvalidValue;
+  if (d.flags & TILED_XCD) launch = (launch + kXcds - 1) / kXcds * kXcds;
   if (kinds == KINDS_LIQUID)
     hipLaunchKernelGGL((k_g2p2g_tiled<KINDS_LIQUID>), dim3((unsigned)launch), dim3(64 * kItemWaves), 0, s, d);
   else
