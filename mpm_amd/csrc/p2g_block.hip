@@ -166,6 +166,22 @@ __device__ __forceinline__ void scatter_global(const float q0[3][3], const float
     end[a] = min(3, (k.compat == MPM_COMPAT_FIXED) ? N - base[a] : N + base[a]);
   }
   const uint64_t n3 = (uint64_t)N * N * N;
+  // A stencil inside the grid writes node (base + offset) itself: its node
+  // blocks are flagged once, at most 2 per axis. Elsewhere (the reference's
+  // aliasing of j, k >= N) each node's block comes from its linear index.
+  const uint32_t nbx = ((uint32_t)N + 3u) >> 2;
+  const bool plain = base[0] >= 0 && base[1] >= 0 && base[2] >= 0 && base[0] + 2 < N &&
+                     base[1] + 2 < N && base[2] + 2 < N;
+  if (dirty && plain) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk)
+          dirty[(((uint32_t)(base[0] + 2 * i) >> 2) * nbx + ((uint32_t)(base[1] + 2 * j) >> 2)) * nbx +
+                ((uint32_t)(base[2] + 2 * kk) >> 2)] = 1u;
+  }
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     if (i < beg[0] || i >= end[0]) continue;
@@ -181,8 +197,7 @@ __device__ __forceinline__ void scatter_global(const float q0[3][3], const float
         if ((uint64_t)idx >= n3) continue;
         const float wt = wij * w[2][kk];
         float* cell = grid + 4 * (size_t)idx;
-        if (dirty) {
-          const uint32_t nbx = ((uint32_t)N + 3u) >> 2;
+        if (dirty && !plain) {
           const uint32_t gx = idx / ((uint32_t)N * (uint32_t)N), gy = (idx / (uint32_t)N) % (uint32_t)N,
                          gz = idx % (uint32_t)N;
           dirty[((gx >> 2) * nbx + (gy >> 2)) * nbx + (gz >> 2)] = 1u;
@@ -531,6 +546,7 @@ __device__ __forceinline__ void g2p2g_item(const mpmk::ItemDesc& d, const int it
   // their left neighbour's sums; a run then has at most 2 writers
   const bool pair = !reduce && maxrun > 1;
   const bool odd = (rank & 1) != 0;
+  const float take = odd ? 1.0f : 0.0f;
   const int lr = lane & 15;
 
   typedef __attribute__((address_space(3))) volatile uint8_t lds_byte;
@@ -562,12 +578,19 @@ __device__ __forceinline__ void g2p2g_item(const mpmk::ItemDesc& d, const int it
         cv[o][2] = wt * (s2 + q2[2][c]);
         cv[o][3] = wt * m.mass;
         if (pair) {  // odd ranks add their left neighbour (whole-wavefront shift)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const float t_ = __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
-                0, __builtin_bit_cast(int, cv[o][q]), 0x138 /* wave_shr:1 */, 0xf, 0xf, true));
-            cv[o][q] = odd ? cv[o][q] + t_ : cv[o][q];
-          }
+          // as cv += shifted cv * (0 or 1): one multiply-add reading its
+          // first operand through the lane shift (exact: t * 1 = t, t * 0 =
+          // 0). Every lane of the wavefront is active here. The s_nop gives
+          // the two wait states a DPP read needs after the VALU writes of
+          // these registers (the compiler does not pad inside inline asm).
+          __asm__(
+              "s_nop 1\n\t"
+              "v_fmac_f32_dpp %0, %0, %4 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+              "v_fmac_f32_dpp %1, %1, %4 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+              "v_fmac_f32_dpp %2, %2, %4 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+              "v_fmac_f32_dpp %3, %3, %4 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+              : "+v"(cv[o][0]), "+v"(cv[o][1]), "+v"(cv[o][2]), "+v"(cv[o][3])
+              : "v"(take));
         }
       }
     }
