@@ -740,49 +740,38 @@ __device__ __forceinline__ void reduce_block(const mpmk::ReduceDesc& d, const in
   float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
   if (__ballot(r0 < r1) != 0ull) {  // some home block around holds items
     const int e0 = (u0 == 3) ? 0 : -1, e1 = (u1 == 3) ? 0 : -1, e2 = (u2 == 3) ? 0 : -1;
-    float4 v[8];
-    uint32_t a0[8], a1[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int f0 = e0 + (q >> 2), f1 = e1 + ((q >> 1) & 1), f2 = e2 + (q & 1);
-      const int j = 9 * (f0 + 1) + 3 * (f1 + 1) + (f2 + 1);
-      a0[q] = (uint32_t)__shfl((int)r0, j);
-      a1[q] = (uint32_t)__shfl((int)r1, j);
-      const int t = ((u0 + 1 - 4 * f0) * W + (u1 + 1 - 4 * f1)) * W + (u2 + 1 - 4 * f2);
-      a1[q] = a0[q] < a1[q] ? a1[q] : a0[q];
-      v[q] = a0[q] < a1[q] ? tiles[(size_t)a0[q] * TN + t] : make_float4(0.f, 0.f, 0.f, 0.f);
-      a0[q] = (uint32_t)t;  // tile node from here on
-    }
     // per home block q the sum of its items in item order (what the combine
-    // pass leaves in the block's first tile), then the 8 block sums in order
-    float4 T[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) T[q] = v[q];
-    // further items of blocks holding more than one (dense scenes): round r
-    // reads item r of every block that has one, 8 independent loads per round
-    uint32_t first[8], rounds = 0;
+    // pass leaves in the block's first tile), then the 8 block sums in order.
+    // The first items of the 8 blocks are read in one batch of independent
+    // loads; blocks holding more than one item (dense scenes) add the rest
+    // after it, block by block.
+    float4 v[8];
+    bool more = false;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int f0 = e0 + (q >> 2), f1 = e1 + ((q >> 1) & 1), f2 = e2 + (q & 1);
       const int j = 9 * (f0 + 1) + 3 * (f1 + 1) + (f2 + 1);
-      first[q] = (uint32_t)__shfl((int)r0, j);
-      rounds = max(rounds, a1[q] - first[q]);
+      const uint32_t a0 = (uint32_t)__shfl((int)r0, j), a1 = (uint32_t)__shfl((int)r1, j);
+      const int t = ((u0 + 1 - 4 * f0) * W + (u1 + 1 - 4 * f1)) * W + (u2 + 1 - 4 * f2);
+      v[q] = a0 < a1 ? tiles[(size_t)a0 * TN + t] : make_float4(0.f, 0.f, 0.f, 0.f);
+      more = more || a1 > a0 + 1;
     }
-    if (d.combined) rounds = 1;  // the first tile holds the block's sum
-    for (uint32_t r = 1; r < rounds; ++r) {
+    if (!d.combined && __ballot(more) != 0ull) {  // (combined: the first tile holds the block's sum)
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const uint32_t it = first[q] + r;
-        v[q] = it < a1[q] ? tiles[(size_t)it * TN + a0[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
-      }
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        if (first[q] + r < a1[q]) { T[q].x += v[q].x; T[q].y += v[q].y; T[q].z += v[q].z; T[q].w += v[q].w; }
+        const int f0 = e0 + (q >> 2), f1 = e1 + ((q >> 1) & 1), f2 = e2 + (q & 1);
+        const int j = 9 * (f0 + 1) + 3 * (f1 + 1) + (f2 + 1);
+        const uint32_t a0 = (uint32_t)__shfl((int)r0, j), a1 = (uint32_t)__shfl((int)r1, j);
+        const int t = ((u0 + 1 - 4 * f0) * W + (u1 + 1 - 4 * f1)) * W + (u2 + 1 - 4 * f2);
+        for (uint32_t it = a0 + 1; it < a1; ++it) {
+          const float4 u = tiles[(size_t)it * TN + t];
+          v[q].x += u.x; v[q].y += u.y; v[q].z += u.z; v[q].w += u.w;
+        }
       }
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      sum.x += T[q].x; sum.y += T[q].y; sum.z += T[q].z; sum.w += T[q].w;
+      sum.x += v[q].x; sum.y += v[q].y; sum.z += v[q].z; sum.w += v[q].w;
     }
   }
   if (is_dirty) {
