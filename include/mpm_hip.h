@@ -178,6 +178,10 @@ int mpm_load_state(mpm_handle h, const char* path);
  *   "xcd_map"    1 (default): the fused kernel hands each of the 8 XCDs a
  *                contiguous run of work items (neighbouring home blocks share
  *                its L2); 0: the hardware's round-robin order
+ *   "reduce_mlp" -1 (default): chosen at upload from how crowded the home
+ *                blocks are; 1: the grid update reads the further work items
+ *                of crowded blocks 8 tiles at a time (more registers); 0: one
+ *                at a time (more wavefronts resident). Same result either way.
  *   "timing"    k > 0: record HIP events around the kernel phases of one
  *                substep in every k (k = 1: every substep); 0 (default): off
  * Results do not depend on the options beyond FP32 summation order. */

@@ -401,6 +401,7 @@ int Sim::upload(hipStream_t s) {
     for (int a = 0; a < 9; ++a) f_canonical_ = f_canonical_ && hF_[9 * i + a] == ((a % 4 == 0) ? 1.0f : 0.0f);
   }
   f_stale_ = false;
+  dense_ = dense_blocks();
   max_mat_ = 0;
   for (int64_t i = 0; i < n; ++i) max_mat_ = std::max<uint8_t>(max_mat_, hmat_[i]);
   cur_ = 0;
@@ -502,6 +503,38 @@ int Sim::sort_desc(mpmk::SortDesc& d) {
   d.max_items = items;
   return MPM_OK;
 }
+
+// Whether the staged particles crowd their home blocks: more than 1 in 20
+// occupied blocks needs a further work item (more particles than one item
+// takes). The grid update then reads several tiles per node and block,
+// which the mlp variant of the tile reduction batches.
+bool Sim::dense_blocks() const {
+  const int64_t n = (int64_t)hJp_.size();
+  if (n == 0) return false;
+  const int N = (int)k_.N, nb = (N + 3) / 4;
+  const int64_t per = opt_.chunk > 0 ? opt_.chunk : mpmk::kMaxItemParticles;
+  std::vector<uint64_t> keys((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    uint64_t key = 0;
+    for (int a = 0; a < 3; ++a) {
+      const int c = std::min(N - 1, std::max(0, (int)std::floor(hx_[3 * i + a] * (float)N)));
+      key = key * (uint64_t)nb + (uint64_t)(c >> 2);
+    }
+    keys[(size_t)i] = key;
+  }
+  std::sort(keys.begin(), keys.end());
+  int64_t occupied = 0, extra = 0;
+  for (size_t i = 0; i < keys.size();) {
+    size_t j = i;
+    while (j < keys.size() && keys[j] == keys[i]) ++j;
+    ++occupied;
+    extra += ((int64_t)(j - i) + per - 1) / per - 1;
+    i = j;
+  }
+  return extra * 20 > occupied;
+}
+
+bool Sim::use_mlp() const { return opt_.reduce_mlp < 0 ? dense_ : opt_.reduce_mlp != 0; }
 
 bool Sim::derives_f() const {
   // only the fused tiled path: the unfused one's G2P kernel reads the F planes
@@ -668,7 +701,7 @@ int Sim::substep(hipStream_t s) {
       // reference order: P2G, grid update, G2P
       int rc = tiled_p2g(s);
       if (rc) return rc;
-      PHASE(PH_GRID_UPDATE, HIPCHK(mpmk::tile_reduce(tile_sums(), k_, d_grid_[0], 1, s), "grid update"));
+      PHASE(PH_GRID_UPDATE, HIPCHK(mpmk::tile_reduce(tile_sums(), k_, d_grid_[0], 1, use_mlp(), s), "grid update"));
       PHASE(PH_G2P, HIPCHK(mpmk::g2p(particles(), d_mats_, k_, d_grid_[0], s), "g2p"));
       vel_grid_ = d_grid_[0];
       grid_dead_ = true;
@@ -682,7 +715,7 @@ int Sim::substep(hipStream_t s) {
       primed_ = true;
     }
     // the tiles hold the P2G of this substep (items of the current order)
-    PHASE(PH_GRID_UPDATE, HIPCHK(mpmk::tile_reduce(tile_sums(), k_, d_grid_[0], 1, s), "grid update"));
+    PHASE(PH_GRID_UPDATE, HIPCHK(mpmk::tile_reduce(tile_sums(), k_, d_grid_[0], 1, use_mlp(), s), "grid update"));
     if (!items_valid_ || (opt_.sort_every > 0 && steps_since_sort_ >= opt_.sort_every)) {
       int rc = sort_particles(s);
       if (rc) return rc;
@@ -842,7 +875,7 @@ int Sim::debug_p2g(float* grid, hipStream_t s) {
     rc = tiled_p2g(s);
     if (rc) return rc;
     g = d_grid_[0];
-    HIPCHK(mpmk::tile_reduce(tile_sums(), k_, g, 0, s), "tile sums");
+    HIPCHK(mpmk::tile_reduce(tile_sums(), k_, g, 0, use_mlp(), s), "tile sums");
   } else {
     // the non-current grid, so a primed direct-path grid survives
     g = d_grid_[gcur_ ^ 1];
@@ -1060,6 +1093,7 @@ int Sim::set_option(const char* name, int64_t v) {
   }
   if (n == "sort_every") { if (v < 0) return fail(MPM_ERR_INVALID, "sort_every must be >= 0"); opt_.sort_every = v; return MPM_OK; }
   if (n == "derive_f") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "derive_f must be 0 or 1"); opt_.derive_f = v; return MPM_OK; }
+  if (n == "reduce_mlp") { if (v < -1 || v > 1) return fail(MPM_ERR_INVALID, "reduce_mlp must be -1, 0 or 1"); opt_.reduce_mlp = v; return MPM_OK; }
   if (n == "xcd_map") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "xcd_map must be 0 or 1"); opt_.xcd_map = v; return MPM_OK; }
   if (n == "radix_bits") { if (v != 4 && v != 6 && v != 8) return fail(MPM_ERR_INVALID, "radix_bits must be 4, 6 or 8"); opt_.radix_bits = v; return MPM_OK; }
   if (n == "chunk") {
@@ -1096,6 +1130,7 @@ int Sim::get_option(const char* name, int64_t* v) {
   else if (n == "timing") *v = opt_.timing;
   else if (n == "derive_f") *v = opt_.derive_f;
   else if (n == "xcd_map") *v = opt_.xcd_map;
+  else if (n == "reduce_mlp") *v = opt_.reduce_mlp;
   else return fail(MPM_ERR_INVALID, "mpm_get_option: unknown option '%s'", name);
   return MPM_OK;
 }

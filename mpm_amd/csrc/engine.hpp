@@ -29,6 +29,8 @@ struct Options {
   int64_t block = 0;        // "block": edge of a home block in cells (4; 0 = default)
   int64_t derive_f = 1;     // "derive_f": liquid-only scenes keep F = cbrt(J) I implicit
   int64_t xcd_map = 1;      // "xcd_map": XCD-contiguous order of the fused kernel's items
+  int64_t reduce_mlp = -1;  // "reduce_mlp": tile reduction batching further items of a block
+                            // (1), one at a time (0), or chosen from the uploaded density (-1)
   int64_t timing = 0;       // "timing": k > 0 brackets the kernels of one substep in every k
                             // with HIP events (per-phase time); 0 off
 };
@@ -154,6 +156,9 @@ class Sim {
   bool f_canonical_ = false;
   bool f_stale_ = false;
   bool derives_f() const;
+  bool dense_blocks() const;  // from the staged particles
+  bool use_mlp() const;
+  bool dense_ = false;
   int ensure_f(hipStream_t s);
   int64_t timed_subs_ = 0;   // substeps since timing_reset
 

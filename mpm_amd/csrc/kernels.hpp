@@ -190,8 +190,10 @@ hipError_t sort_multi(int phase, int pass, const SortDesc* descs, const uint32_t
 hipError_t liquid_f(const ParticlesDev& P, hipStream_t s);
 // Sums the tiles (and the dirty parts of the atomic grid, which it clears)
 // into every node of `grid`; update=1 applies the grid update (velocities),
-// update=0 leaves the raw P2G sums (momentum, mass).
-hipError_t tile_reduce(const TileSums& in, const StepConsts& k, float4* grid, int update,
+// update=0 leaves the raw P2G sums (momentum, mass). mlp=1 suits dense scenes
+// (home blocks of several items: their further tiles are read 8 at a time,
+// at the cost of registers); the result does not depend on it.
+hipError_t tile_reduce(const TileSums& in, const StepConsts& k, float4* grid, int update, int mlp,
                        hipStream_t s);
 
 // adjoint.hip: the differentiable substep (24 state planes of n floats) and

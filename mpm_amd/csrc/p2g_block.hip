@@ -705,6 +705,7 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
 // picks the ranges of its 8 blocks with lane shuffles and reads the first
 // item of each in one batch of independent loads, further items (blocks
 // holding more than one item) after it. The order of the sum is fixed.
+template <bool MLP>
 __device__ __forceinline__ void reduce_block(const mpmk::ReduceDesc& d, const int64_t nbk, const int lane) {
   constexpr int W = mpmk::kTileW, TN = mpmk::kTileNodes;
   const float4* __restrict__ const tiles = d.tiles;
@@ -744,28 +745,52 @@ __device__ __forceinline__ void reduce_block(const mpmk::ReduceDesc& d, const in
     // pass leaves in the block's first tile), then the 8 block sums in order.
     // The first items of the 8 blocks are read in one batch of independent
     // loads; blocks holding more than one item (dense scenes) add the rest
-    // after it, block by block.
+    // after it: MLP = false block by block, one load in flight (fewer
+    // registers, more waves resident), MLP = true round by round, item r of
+    // every block in one batch of 8 loads. The sums are the same bits.
     float4 v[8];
-    bool more = false;
+    uint32_t a0[8], a1[8];
+    uint32_t more = 0;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int f0 = e0 + (q >> 2), f1 = e1 + ((q >> 1) & 1), f2 = e2 + (q & 1);
       const int j = 9 * (f0 + 1) + 3 * (f1 + 1) + (f2 + 1);
-      const uint32_t a0 = (uint32_t)__shfl((int)r0, j), a1 = (uint32_t)__shfl((int)r1, j);
+      const uint32_t b0 = (uint32_t)__shfl((int)r0, j), b1 = (uint32_t)__shfl((int)r1, j);
       const int t = ((u0 + 1 - 4 * f0) * W + (u1 + 1 - 4 * f1)) * W + (u2 + 1 - 4 * f2);
-      v[q] = a0 < a1 ? tiles[(size_t)a0 * TN + t] : make_float4(0.f, 0.f, 0.f, 0.f);
-      more = more || a1 > a0 + 1;
+      v[q] = b0 < b1 ? tiles[(size_t)b0 * TN + t] : make_float4(0.f, 0.f, 0.f, 0.f);
+      more = max(more, b1 > b0 ? b1 - b0 : 0u);
+      if (MLP) {
+        a0[q] = b0 + 1;  // next item
+        a1[q] = b1;
+      }
     }
-    if (!d.combined && __ballot(more) != 0ull) {  // (combined: the first tile holds the block's sum)
+    if (!d.combined && __ballot(more > 1) != 0ull) {  // (combined: the first tile holds the block's sum)
+      if (MLP) {
+        for (uint32_t r = 1; r < more; ++r) {
+          float4 u[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int f0 = e0 + (q >> 2), f1 = e1 + ((q >> 1) & 1), f2 = e2 + (q & 1);
-        const int j = 9 * (f0 + 1) + 3 * (f1 + 1) + (f2 + 1);
-        const uint32_t a0 = (uint32_t)__shfl((int)r0, j), a1 = (uint32_t)__shfl((int)r1, j);
-        const int t = ((u0 + 1 - 4 * f0) * W + (u1 + 1 - 4 * f1)) * W + (u2 + 1 - 4 * f2);
-        for (uint32_t it = a0 + 1; it < a1; ++it) {
-          const float4 u = tiles[(size_t)it * TN + t];
-          v[q].x += u.x; v[q].y += u.y; v[q].z += u.z; v[q].w += u.w;
+          for (int q = 0; q < 8; ++q) {
+            const int f0 = e0 + (q >> 2), f1 = e1 + ((q >> 1) & 1), f2 = e2 + (q & 1);
+            const int t = ((u0 + 1 - 4 * f0) * W + (u1 + 1 - 4 * f1)) * W + (u2 + 1 - 4 * f2);
+            u[q] = a0[q] < a1[q] ? tiles[(size_t)a0[q] * TN + t] : make_float4(0.f, 0.f, 0.f, 0.f);
+          }
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            if (a0[q] < a1[q]) { v[q].x += u[q].x; v[q].y += u[q].y; v[q].z += u[q].z; v[q].w += u[q].w; }
+            ++a0[q];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int f0 = e0 + (q >> 2), f1 = e1 + ((q >> 1) & 1), f2 = e2 + (q & 1);
+          const int j = 9 * (f0 + 1) + 3 * (f1 + 1) + (f2 + 1);
+          const uint32_t b0 = (uint32_t)__shfl((int)r0, j), b1 = (uint32_t)__shfl((int)r1, j);
+          const int t = ((u0 + 1 - 4 * f0) * W + (u1 + 1 - 4 * f1)) * W + (u2 + 1 - 4 * f2);
+          for (uint32_t it = b0 + 1; it < b1; ++it) {
+            const float4 u = tiles[(size_t)it * TN + t];
+            v[q].x += u.x; v[q].y += u.y; v[q].z += u.z; v[q].w += u.w;
+          }
         }
       }
     }
@@ -785,11 +810,12 @@ __device__ __forceinline__ void reduce_block(const mpmk::ReduceDesc& d, const in
   if (in_grid) grid[node] = update ? grid_node_update(sum, x, y, z, k) : sum;
 }
 
+template <bool MLP>
 __global__ void __launch_bounds__(256) k_tile_reduce(mpmk::ReduceDesc d) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t nbk = (int64_t)blockIdx.x * 4 + wave;
   if (nbk >= (int64_t)d.nbx * d.nbx * d.nbx) return;  // uniform for the wavefront
-  reduce_block(d, nbk, lane);
+  reduce_block<MLP>(d, nbk, lane);
 }
 
 // one workgroup per home block: tiles[first] += tiles[first + 1 .. end), in order
@@ -831,7 +857,7 @@ __global__ void __launch_bounds__(256)
   const uint32_t g = blockIdx.x * 4u + (uint32_t)wave;
   if (g >= prefix[nscenes]) return;  // uniform for the wavefront
   const int j = scene_of(prefix, nscenes, g);
-  reduce_block(descs[j], (int64_t)(g - prefix[j]), lane);
+  reduce_block<false>(descs[j], (int64_t)(g - prefix[j]), lane);
 }
 
 namespace mpmk {
@@ -939,7 +965,7 @@ hipError_t zero_dead_blocks(float4* grid, const uint32_t* live, int N, hipStream
   return hipGetLastError();
 }
 
-hipError_t tile_reduce(const TileSums& in, const StepConsts& k, float4* grid, int update,
+hipError_t tile_reduce(const TileSums& in, const StepConsts& k, float4* grid, int update, int mlp,
                        hipStream_t s) {
   if (!in.tiles || !in.atomic || !in.dirty || !in.block_items || !in.covered || !grid)
     return hipErrorInvalidValue;
@@ -958,7 +984,10 @@ hipError_t tile_reduce(const TileSums& in, const StepConsts& k, float4* grid, in
   d.live = in.live;
   d.skip_dead = update && in.live;
   const int64_t nblocks = (int64_t)d.nbx * d.nbx * d.nbx;
-  hipLaunchKernelGGL(k_tile_reduce, dim3((unsigned)((nblocks + 3) / 4)), dim3(256), 0, s, d);
+  if (mlp)
+    hipLaunchKernelGGL(k_tile_reduce<true>, dim3((unsigned)((nblocks + 3) / 4)), dim3(256), 0, s, d);
+  else
+    hipLaunchKernelGGL(k_tile_reduce<false>, dim3((unsigned)((nblocks + 3) / 4)), dim3(256), 0, s, d);
   return hipGetLastError();
 }
 

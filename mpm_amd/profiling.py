@@ -62,9 +62,21 @@ def cmd_copy(a) -> int:
     return 0
 
 
-def summarize_db(path: Path) -> str:
+def summarize_db(path: Path, tail_of: str = "") -> str:
+    """Per-kernel durations. tail_of = "KERNEL:N" keeps only the launches from
+    the N-th last launch of KERNEL on (the timed region of a run whose warm-up
+    launches other configurations, e.g. autotuning trials)."""
     con = sqlite3.connect(str(path))
     rows = con.execute("select name, start, end from kernels").fetchall()
+    head = ""
+    if tail_of:
+        kname, n = tail_of.rsplit(":", 1)
+        starts = sorted(s for nm, s, _ in rows if _clean(nm) == kname)
+        if len(starts) < int(n):
+            raise SystemExit(f"{kname}: {len(starts)} launches, fewer than {n}")
+        t0 = starts[-int(n)]
+        rows = [r for r in rows if r[1] >= t0]
+        head = f"(launches from the {n}-th last {kname} on)\n"
     per = defaultdict(list)
     for n, s, e in rows:
         per[_clean(n)].append((e - s) / 1000.0)
@@ -74,7 +86,7 @@ def summarize_db(path: Path) -> str:
     for n, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
         out.append(f"{n[:44]:44s} {len(v):6d} {sum(v):11.1f} {sum(v) / len(v):9.2f} {min(v):9.2f} "
                    f"{max(v):9.2f} {100 * sum(v) / tot:6.2f}")
-    return "\n".join(out)
+    return head + "\n".join(out)
 
 
 def summarize_csv(paths) -> str:
@@ -96,7 +108,7 @@ def cmd_summarize(a) -> int:
     dbs = [p for p in paths if p.suffix == ".db"]
     csvs = [p for p in paths if p.suffix == ".csv"]
     for p in dbs:
-        print(summarize_db(p))
+        print(summarize_db(p, a.tail_of))
     if csvs:
         print(summarize_csv(csvs))
     return 0
@@ -116,6 +128,7 @@ def main(argv=None) -> int:
     c.add_argument("--reps", type=int, default=3)
     s = sub.add_parser("summarize")
     s.add_argument("paths", nargs="+")
+    s.add_argument("--tail-of", default="", help="KERNEL:N -- only launches from the N-th last KERNEL on")
     a = ap.parse_args(argv)
     return {"run": cmd_run, "copy": cmd_copy, "summarize": cmd_summarize}[a.cmd](a)
 

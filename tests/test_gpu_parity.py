@@ -282,6 +282,24 @@ def test_liquid_derived_f_is_bit_identical(gpu_lib):
     assert np.allclose(mid["F"], cj[:, None, None] * np.eye(3), rtol=3e-7, atol=0)
 
 
+def test_reduce_strategies_are_bit_identical(gpu_lib):
+    """The two ways the grid update reads the further work items of crowded
+    home blocks (one at a time, or 8 blocks' items per batch) sum in the same
+    order: a dense scene (several items per block) gives the same bits
+    with either, and with the choice left to the engine."""
+    sc = scenes.batched_scenes()[1]  # 41k particles in a N=32 grid, ~800 per occupied block
+    outs = []
+    for mlp in (0, 1, -1):
+        eng = make_engine(sc, reduce_mlp=mlp)
+        assert eng.get_option("reduce_mlp") == mlp
+        eng.advance(40)
+        outs.append(eng.download())
+        eng.close()
+    for o in outs[1:]:
+        for f in ("x", "v", "C", "F", "Jp"):
+            assert np.array_equal(outs[0][f], o[f]), f
+
+
 @pytest.mark.parametrize("opts", ["direct-unfused", "tiled-b4", "tiled-b4-chunk64", "tiled-unfused", "tiled-default"])
 def test_snow_10_substeps(gpu_lib, opts):
     sc = scenes.snow_spheres(n=40_000, N=32)
