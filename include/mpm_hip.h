@@ -202,8 +202,8 @@ int mpm_timing_reset(mpm_handle h);
 
 /* ---- autotuner (no reference counterpart) ----
  * Times `trial_substeps` substeps (at least two sort intervals) of the
- * uploaded scene under each candidate sort interval (8, 12, 16, 24, 32, 48
- * substeps) and work-item size (192 and 128 particles), each from the same
+ * uploaded scene under each candidate sort interval (8, 12, 16, 24, 32, 48,
+ * 64, 96 substeps) and work-item size (192 and 128 particles), each from the same
  * snapshot of the current state, after one interval of warm-up; keeps the
  * fastest as the "sort_every" and "chunk" options (tiled fused mode) and
  * puts the state and simulated time back as they were. The report (JSON,

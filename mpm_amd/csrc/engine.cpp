@@ -1023,7 +1023,7 @@ int Sim::autotune(int32_t trial_substeps, std::string* report, hipStream_t s) {
   };
   struct Trial { int64_t sort_every, chunk; double us; };
   std::vector<Trial> trials;
-  const int64_t sorts[] = {8, 12, 16, 24, 32, 48};
+  const int64_t sorts[] = {8, 12, 16, 24, 32, 48, 64, 96};
   const int64_t chunks[] = {mpmk::kMaxItemParticles, 128};
   for (int64_t ch : chunks) {
     for (int64_t se : sorts) {

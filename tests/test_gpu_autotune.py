@@ -35,8 +35,8 @@ def test_autotune_restores_state_and_sets_options(gpu_lib, tmp_path):
     assert eng.time() == t
     for f in ("x", "v", "C", "F", "Jp", "material"):
         assert np.array_equal(before[f], after[f]), f
-    assert len(choice["trials"]) == 12
-    assert choice["sort_every"] in (8, 12, 16, 24, 32, 48) and choice["chunk"] in (128, 192)
+    assert len(choice["trials"]) == 16
+    assert choice["sort_every"] in (8, 12, 16, 24, 32, 48, 64, 96) and choice["chunk"] in (128, 192)
     best = min(choice["trials"], key=lambda r: r["us_per_substep"])
     assert (best["sort_every"], best["chunk"]) == (choice["sort_every"], choice["chunk"])
     assert eng.get_option("sort_every") == choice["sort_every"]
