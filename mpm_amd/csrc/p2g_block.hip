@@ -338,14 +338,20 @@ __device__ __forceinline__ void g2p2g_item(const mpmk::ItemDesc& d, const int it
     m = mats[mat];
   }
 
+  // Tile node n = (a W + tb) W + tc: thread tid takes the nodes with
+  // n mod 64 = tid mod 64 (its tb, tc) and a = tid / 64 + 3 j.
+  static_assert(NT % 64 == 0 && W * W == 64, "tile rows per wavefront");
+  const int tb = (lane >> 3) & 7, tc = lane & 7;
+  const int slot_even = tb * W + (tc ^ ((tb & 1) << 2)), slot_odd = tb * W + (tc ^ (((tb & 1) ^ 1) << 2));
   if (do_g2p) {
-    for (int n = tid; n < TN; n += NT) {
-      const int a = n / (W * W), r = n - a * W * W, b = r / W, c = r - b * W;
-      const int gi = o0 + a, gj = o1 + b, gk = o2 + c;
+    const int gj = o1 + tb, gk = o2 + tc;
+    const bool jk_in = gj >= 0 && gj < N && gk >= 0 && gk < N;
+    const uint32_t jk = (uint32_t)gj * (uint32_t)N + (uint32_t)gk;
+    for (int a = wave; a < W; a += NT / 64) {
+      const int gi = o0 + a;
       float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (gi >= 0 && gi < N && gj >= 0 && gj < N && gk >= 0 && gk < N)
-        v = grid_v[((size_t)gi * N + (size_t)gj) * N + (size_t)gk];
-      vt[tile_slot(a, b, c)] = v;
+      if (jk_in && gi >= 0 && gi < N) v = grid_v[(size_t)(uint32_t)gi * ((size_t)N * N) + jk];
+      vt[a * W * W + ((a & 1) ? slot_odd : slot_even)] = v;
     }
   }
   {
@@ -436,11 +442,15 @@ __device__ __forceinline__ void g2p2g_item(const mpmk::ItemDesc& d, const int it
 #pragma unroll
       for (int a = 0; a < 9; ++a) A[a] = ((a % 4 == 0) ? 1.0f : 0.0f) + k.dt * s.C[a];
       if (df) {
-        const float c0 = cbrtf(s.Jp);  // the stored F_old, bit for bit
+        // F_old = c0 I with c0 = cbrt(Jp) (the stored F_old, bit for bit), so
+        // F_new = A F_old is c0 A elementwise: the full product's bits, its
+        // other terms being products with zero
+        const float c0 = cbrtf(s.Jp);
 #pragma unroll
-        for (int a = 0; a < 9; ++a) s.F[a] = (a % 4 == 0) ? c0 : 0.0f;
+        for (int a = 0; a < 9; ++a) s.F[a] = A[a] * c0;
+      } else {
+        mul_ab(A, s.F, s.F);
       }
-      mul_ab(A, s.F, s.F);
       if (KS == KINDS_LIQUID) {
         // F <- J^(1/3) I, Jp <- J (the only kind this instantiation handles)
         const float J = det3(s.F);
@@ -478,7 +488,9 @@ __device__ __forceinline__ void g2p2g_item(const mpmk::ItemDesc& d, const int it
     }
     float aff[9];
 #pragma unroll
-    for (int a = 0; a < 9; ++a) aff[a] = m.stress_scale * tau[a] + m.mass * s.C[a];
+    for (int a = 0; a < 9; ++a)  // (the liquid's tau is diagonal: its zeros are left out)
+      aff[a] = (KS == KINDS_LIQUID && a % 4 != 0) ? m.mass * s.C[a]
+                                                  : m.stress_scale * tau[a] + m.mass * s.C[a];
     float d[3][3];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
@@ -642,8 +654,9 @@ __device__ __forceinline__ void g2p2g_item(const mpmk::ItemDesc& d, const int it
 
   // ---- the item's tile: sum of the wavefront tiles, fixed order
   float4* const out = tiles + (size_t)item * TN;
-  for (int n = tid; n < TN; n += NT) {
-    const int sl = tile_slot(n / (W * W), (n / W) % W, n % W);
+  for (int a = wave; a < W; a += NT / 64) {
+    const int n = a * W * W + lane;
+    const int sl = a * W * W + ((a & 1) ? slot_odd : slot_even);
     float4 t = acc[0][sl];
 #pragma unroll
     for (int q = 1; q < NW; ++q) {
