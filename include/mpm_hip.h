@@ -134,6 +134,13 @@ int mpm_clear_particles(mpm_handle h);
 /* Copy the staging set to the device (allocating as needed) and reset the
  * simulated time to 0 (Simulation::initCuda, mpm.cu:197-207). */
 int mpm_upload(mpm_handle h, void* stream);
+/* Copy the particles staged since the last mpm_upload / mpm_append_particles
+ * (or mpm_clear_particles) to the device behind the ones there, which stay
+ * on the device untouched; the new particles take the next original indices
+ * (downloads list them last) and the simulated time goes on. Replaces the
+ * reference's host round trip when an object's lifetime begins
+ * (Simulation::particlesToHost + reallocateParticles, mpm.cu:288-314). */
+int mpm_append_particles(mpm_handle h, void* stream);
 /* Number of particles on the device (Simulation::getFullParticleCount,
  * mpm.cu:253-259, for the uploaded set). */
 int mpm_count(mpm_handle h, int64_t* out);

@@ -8,7 +8,10 @@
 // ignores CUDA errors); objects whose lifetime starts or ends are brought onto
 // or taken off the device at the first advance() where t crosses the bound
 // (the reference adds them at the next syncDevice, mpm.cu:288-314, and never
-// removes them); the material list is copied, not referenced.
+// removes them). An object that starts is appended on the device
+// (mpm_append_particles): the particles already there are not copied to the
+// host and back; one that ends takes a round trip. The material list is
+// copied, not referenced.
 #pragma once
 
 #include <cstdint>
@@ -229,9 +232,16 @@ class Simulation {
 
   // Simulation::advance (mpm.cu:323-329): one substep, then t += dt
   void advance() {
-    if (active_mask(t) != uploaded_) {
-      syncDevice();
-      upload_active();
+    const std::vector<char> now = active_mask(t);
+    if (now != uploaded_) {
+      bool ended = false;
+      for (size_t k = 0; k < objects.size(); ++k) ended = ended || (uploaded_[k] && !now[k]);
+      if (ended || active_particle_count == 0) {
+        syncDevice();
+        upload_active();
+      } else {
+        append_started(now);
+      }
     }
     if (active_particle_count > 0) check(mpm_advance(h_, 1, nullptr), h_, "mpm_advance");
     t += par.dt;
@@ -244,10 +254,8 @@ class Simulation {
     std::vector<Particle> buf((size_t)active_particle_count);
     check(mpm_download_aos104(h_, buf.data(), nullptr), h_, "mpm_download_aos104");
     size_t i = 0;
-    for (size_t k = 0; k < objects.size(); ++k) {
-      if (!uploaded_[k]) continue;
+    for (size_t k : on_device_)
       for (Particle& p : objects[k].particles) p = buf[i++];
-    }
   }
 
   size_t getFullParticleCount() const {
@@ -275,10 +283,13 @@ class Simulation {
   }
   void upload_active() {
     uploaded_ = active_mask(t);
+    on_device_.clear();
     check(mpm_clear_particles(h_), h_, "mpm_clear_particles");
     int64_t n = 0;
     for (size_t k = 0; k < objects.size(); ++k) {
-      if (!uploaded_[k] || objects[k].particles.empty()) continue;
+      if (!uploaded_[k]) continue;
+      on_device_.push_back(k);
+      if (objects[k].particles.empty()) continue;
       check(mpm_add_particles_aos104(h_, objects[k].particles.data(), (int64_t)objects[k].particles.size()),
             h_, "mpm_add_particles_aos104");
       n += (int64_t)objects[k].particles.size();
@@ -286,11 +297,28 @@ class Simulation {
     active_particle_count = n;
     if (n > 0) check(mpm_upload(h_, nullptr), h_, "mpm_upload");
   }
+  // objects that became active, in object order, behind the device's particles
+  void append_started(const std::vector<char>& now) {
+    check(mpm_clear_particles(h_), h_, "mpm_clear_particles");
+    int64_t n = 0;
+    for (size_t k = 0; k < objects.size(); ++k) {
+      if (uploaded_[k] || !now[k]) continue;
+      on_device_.push_back(k);
+      if (objects[k].particles.empty()) continue;
+      check(mpm_add_particles_aos104(h_, objects[k].particles.data(), (int64_t)objects[k].particles.size()),
+            h_, "mpm_add_particles_aos104");
+      n += (int64_t)objects[k].particles.size();
+    }
+    uploaded_ = now;
+    if (n > 0) check(mpm_append_particles(h_, nullptr), h_, "mpm_append_particles");
+    active_particle_count += n;
+  }
 
   mpm_handle h_ = nullptr;
   mpm_sampler sampler_ = nullptr;
   uint32_t seed_ = 1;
   std::vector<char> uploaded_;
+  std::vector<size_t> on_device_;  // objects on the device, in the order of their particles there
   std::vector<Particle> all_;
 };
 

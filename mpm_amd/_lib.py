@@ -97,6 +97,7 @@ SIGNATURES = {
     "mpm_add_particles_aos104": (C.c_int, [_P, C.c_void_p, C.c_int64]),
     "mpm_clear_particles": (C.c_int, [_P]),
     "mpm_upload": (C.c_int, [_P, _P]),
+    "mpm_append_particles": (C.c_int, [_P, _P]),
     "mpm_count": (C.c_int, [_P, _i64p]),
     "mpm_advance": (C.c_int, [_P, C.c_int32, _P]),
     "mpm_time": (C.c_int, [_P, C.POINTER(C.c_double)]),

@@ -141,6 +141,11 @@ extern "C" int mpm_upload(mpm_handle h, void* stream) {
   return s->upload((hipStream_t)stream);
 }
 
+extern "C" int mpm_append_particles(mpm_handle h, void* stream) {
+  MPM_GET(h);
+  return s->append((hipStream_t)stream);
+}
+
 extern "C" int mpm_count(mpm_handle h, int64_t* out) {
   MPM_GET(h);
   if (!out) return s->fail(MPM_ERR_INVALID, "mpm_count: null output");

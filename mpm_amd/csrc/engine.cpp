@@ -309,6 +309,7 @@ int Sim::add_particles_aos104(const void* p, int64_t n) {
 
 int Sim::clear_staging() {
   hx_.clear(); hv_.clear(); hC_.clear(); hF_.clear(); hJp_.clear(); hmat_.clear();
+  staged_on_device_ = 0;
   return MPM_OK;
 }
 
@@ -412,6 +413,7 @@ int Sim::upload(hipStream_t s) {
   }
   HIPCHK(hipStreamSynchronize(s), "upload sync");
   P_ = n;
+  staged_on_device_ = (size_t)n;
   uploaded_ = true;
   items_valid_ = false;
   pending_gather_ = false;
@@ -419,6 +421,105 @@ int Sim::upload(hipStream_t s) {
   vel_grid_ = nullptr;
   steps_since_sort_ = 1 << 30;
   t_ = 0.0;
+  return MPM_OK;
+}
+
+// Adds the particles staged since the last upload / append (or clear) after
+// the ones on the device, which stay as they are (no copy back to the
+// host); the new ones take the next original indices. Simulated time goes
+// on. The items are rebuilt at the next substep (a sort, then the P2G of
+// the whole set).
+int Sim::append(hipStream_t s) {
+  ++gen_;
+  if (!uploaded_) return fail(MPM_ERR_STATE, "mpm_append_particles: nothing uploaded");
+  const size_t total = hJp_.size();
+  if (total < staged_on_device_) staged_on_device_ = 0;
+  const int64_t n = (int64_t)(total - staged_on_device_), o = (int64_t)staged_on_device_;
+  if (n == 0) return MPM_OK;
+  for (int64_t i = o; i < (int64_t)total; ++i)
+    if (hmat_[i] >= mats_.size())
+      return fail(MPM_ERR_INVALID, "mpm_append_particles: particle %lld uses material %d but only %zu are set",
+                  (long long)(i - o), (int)hmat_[i], mats_.size());
+  if (P_ + n > (int64_t)0x7fffffff - 64)
+    return fail(MPM_ERR_CAPACITY, "mpm_append_particles: more than 2^31 particles");
+  bool canon = true;
+  for (int64_t i = o; i < (int64_t)total && canon; ++i) {
+    canon = hJp_[i] == 1.0f;
+    for (int a = 0; a < 9; ++a) canon = canon && hF_[9 * i + a] == ((a % 4 == 0) ? 1.0f : 0.0f);
+  }
+  if (!canon) {  // F stops being implicit: write out the device particles' F first
+    int rc = ensure_f(s);
+    if (rc) return rc;
+  }
+  int rc = upload_materials();
+  if (rc) return rc;
+  if (P_ + n > cap_) {
+    // grow: keep the device particles in a compact copy across the reallocation
+    float* tf = nullptr;
+    uint8_t* tm = nullptr;
+    uint32_t* ti = nullptr;
+    const size_t P = (size_t)P_;
+    hipError_t e = hipSuccess;
+    if (P > 0) {
+      e = hipMalloc(&tf, sizeof(float) * NUM_PLANES * P);
+      if (e == hipSuccess) e = hipMalloc(&tm, P);
+      if (e == hipSuccess) e = hipMalloc(&ti, sizeof(uint32_t) * P);
+      if (e == hipSuccess)
+        e = hipMemcpy2DAsync(tf, sizeof(float) * P, d_f_[cur_], sizeof(float) * stride_, sizeof(float) * P,
+                             NUM_PLANES, hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(tm, d_mat_[cur_], P, hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(ti, d_id_[cur_], sizeof(uint32_t) * P, hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    if (e == hipSuccess) {
+      rc = ensure_capacity(P_ + n);
+      cur_ = 0;
+      if (!rc && P > 0) {
+        e = hipMemcpy2DAsync(d_f_[0], sizeof(float) * stride_, tf, sizeof(float) * P, sizeof(float) * P,
+                             NUM_PLANES, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_mat_[0], tm, P, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_id_[0], ti, sizeof(uint32_t) * P, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+      }
+    }
+    if (tf) (void)hipFree(tf);
+    if (tm) (void)hipFree(tm);
+    if (ti) (void)hipFree(ti);
+    if (rc) return rc;
+    HIPCHK(e, "mpm_append_particles: keeping the device particles");
+  }
+  // the new particles as planes of n floats, copied behind the P_ on the device
+  std::vector<float> planes((size_t)NUM_PLANES * n, 0.0f);
+  for (int64_t j = 0; j < n; ++j) {
+    const int64_t i = o + j;
+    for (int a = 0; a < 3; ++a) {
+      planes[(PL_X + a) * n + j] = hx_[3 * i + a];
+      planes[(PL_V + a) * n + j] = hv_[3 * i + a];
+    }
+    for (int a = 0; a < 9; ++a) {
+      planes[(PL_C + a) * n + j] = hC_[9 * i + a];
+      planes[(PL_F + a) * n + j] = hF_[9 * i + a];
+    }
+    planes[PL_JP * n + j] = hJp_[i];
+  }
+  std::vector<uint32_t> ids((size_t)n);
+  std::iota(ids.begin(), ids.end(), (uint32_t)P_);
+  HIPCHK(hipMemcpy2DAsync(d_f_[cur_] + P_, sizeof(float) * stride_, planes.data(), sizeof(float) * n,
+                          sizeof(float) * n, NUM_PLANES, hipMemcpyHostToDevice, s),
+         "append particles");
+  HIPCHK(hipMemcpyAsync(d_mat_[cur_] + P_, &hmat_[o], n, hipMemcpyHostToDevice, s), "append material ids");
+  HIPCHK(hipMemcpyAsync(d_id_[cur_] + P_, ids.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s),
+         "append ids");
+  HIPCHK(hipStreamSynchronize(s), "append sync");
+  for (int64_t i = o; i < (int64_t)total; ++i) max_mat_ = std::max<uint8_t>(max_mat_, hmat_[i]);
+  f_canonical_ = f_canonical_ && canon;
+  P_ += n;
+  staged_on_device_ = total;
+  items_valid_ = false;
+  pending_gather_ = false;  // the next sort starts from the current order
+  primed_ = false;
+  side_clean_ = false;      // the last pass's side-grid scatters are dropped with its tiles
+  steps_since_sort_ = 1 << 30;
   return MPM_OK;
 }
 

@@ -55,6 +55,7 @@ class Sim {
   int add_particles_aos104(const void* p, int64_t n);
   int clear_staging();
   int upload(hipStream_t s);
+  int append(hipStream_t s);
   int64_t count() const { return P_; }
   double time() const { return t_; }
   int advance(int32_t substeps, hipStream_t s);
@@ -159,6 +160,7 @@ class Sim {
   bool dense_blocks() const;  // from the staged particles
   bool use_mlp() const;
   bool dense_ = false;
+  size_t staged_on_device_ = 0;  // staged particles the last upload / append copied
   int ensure_f(hipStream_t s);
   int64_t timed_subs_ = 0;   // substeps since timing_reset
 

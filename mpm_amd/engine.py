@@ -154,6 +154,13 @@ class Engine:
         self._chk(lib().mpm_upload(self._h, _stream(stream)))
         self._n = self.count()
 
+    def append(self, stream=None) -> None:
+        """Add the particles staged since the last upload / append (or
+        clear_particles) behind the ones on the device, without copying those
+        back; the simulated time goes on (mpm_append_particles)."""
+        self._chk(lib().mpm_append_particles(self._h, _stream(stream)))
+        self._n = self.count()
+
     def count(self) -> int:
         v = C.c_int64()
         self._chk(lib().mpm_count(self._h, C.byref(v)))

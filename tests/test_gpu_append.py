@@ -1,0 +1,101 @@
+"""mpm_append_particles (Engine.append): particles added behind the ones on
+the device, without copying those back to the host.
+
+Against the host round trip (download, add, upload) on the same sequence:
+the same physics to the step's parity tolerance (the two runs hold the
+particles in different device orders, so the P2G sums add in a different
+order), the simulated time goes on instead of restarting, downloads list the
+appended particles last, and the device arrays grow past their first
+capacity. A liquid whose F was implicit (cbrt(J) I) keeps the first
+particles' F when particles with another F arrive.
+"""
+import numpy as np
+import pytest
+
+from conftest import state_errors
+from mpm_amd import Engine, scenes
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "v", "C", "F", "Jp", "material")
+
+
+def engine(sc, part):
+    st = sc.state()
+    e = Engine(sc.params)
+    e.set_materials(sc.materials)
+    e.add_particles(*(st[f][part] for f in ("x", "material", "v", "C", "F", "Jp")))
+    e.upload()
+    return e
+
+
+def split_run(sc, k1, k2, F_b=None):
+    st = sc.state()
+    n = sc.n
+    a, b = slice(0, n // 2), slice(n // 2, n)
+    Fb = st["F"][b] if F_b is None else np.broadcast_to(F_b, st["F"][b].shape).copy()
+    # append
+    e = engine(sc, a)
+    e.advance(k1)
+    t1 = e.time()
+    e.clear_particles()
+    e.add_particles(st["x"][b], st["material"][b], st["v"][b], st["C"][b], Fb, st["Jp"][b])
+    e.append()
+    assert e.count() == n and e.time() == t1
+    e.advance(k2)
+    app = e.download()
+    t_app = e.time()
+    e.close()
+    # round trip
+    r = engine(sc, a)
+    r.advance(k1)
+    s1 = r.download()
+    r.clear_particles()
+    r.add_particles(s1["x"], s1["material"], s1["v"], s1["C"], s1["F"], s1["Jp"])
+    r.add_particles(st["x"][b], st["material"][b], st["v"][b], st["C"][b], Fb, st["Jp"][b])
+    r.upload()
+    r.advance(k2)
+    ref = r.download()
+    r.close()
+    return app, ref, t_app
+
+
+def test_append_matches_round_trip(gpu_lib):
+    sc = scenes.snow_spheres(n=40_000, N=32)
+    app, ref, t = split_run(sc, 25, 30)
+    assert t == pytest.approx(55 * float(np.float32(sc.params.dt)), rel=1e-6)
+    assert np.array_equal(app["material"], ref["material"])
+    errs = state_errors(app, ref, sc.params.N)
+    assert all(e < 1e-4 for e in errs.values()), errs
+
+
+def test_append_to_implicit_liquid_F(gpu_lib):
+    sc = scenes.dam_break(n=40_000, N=32)
+    F_b = (np.eye(3, dtype=np.float32) * np.float32(1.01))
+    app, ref, _ = split_run(sc, 20, 15, F_b=F_b)
+    errs = state_errors(app, ref, sc.params.N)
+    assert all(e < 1e-4 for e in errs.values()), errs
+    # every liquid particle's F is cbrt(J) I again after the pass, but for the
+    # ones the reference's G2P guard freezes (original index > N^3)
+    upd = slice(0, sc.params.N ** 3 + 1)
+    cj = np.cbrt(app["Jp"][upd].astype(np.float64))
+    assert np.allclose(app["F"][upd], cj[:, None, None] * np.eye(3), rtol=3e-7, atol=0)
+
+
+def test_append_is_deterministic_and_empty_append_is_a_no_op(gpu_lib):
+    sc = scenes.snow_spheres(n=20_000, N=32)
+    st = sc.state()
+    outs = []
+    for _ in range(2):
+        e = engine(sc, slice(0, 10_000))
+        e.append()  # nothing staged since the upload
+        assert e.count() == 10_000
+        e.advance(7)
+        e.clear_particles()
+        e.add_particles(*(st[f][10_000:] for f in ("x", "material", "v", "C", "F", "Jp")))
+        e.append()
+        e.advance(9)
+        outs.append(e.download())
+        e.close()
+    for f in FIELDS:
+        assert np.array_equal(outs[0][f], outs[1][f]), f
