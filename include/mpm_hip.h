@@ -161,6 +161,29 @@ int mpm_download_aos104(mpm_handle h, void* particles, void* stream);
  * update the reference leaves m = 1 on nodes that had mass (mpm.cu:86-88). */
 int mpm_download_grid(mpm_handle h, float* grid, void* stream);
 
+/* ---- surface mesh of the particles (replaces MeshBuilder::computeMesh,
+ *      include/mesh_builder.h:181-211, with its distance field,
+ *      fillVoxelGrid_distance, mesh_builder.h:103-149; computed on the GPU
+ *      from the device particles, no host copy of them) ----
+ * mpm_mesh_build: distance field on grid_side^3 voxels (voxel_dx = N dx /
+ * grid_side; each voxel the distance to the nearest particle whose box of
+ * voxels, (particle_radius + 1) voxels around it, reaches it, else that
+ * cut-off), then the iso-surface at particle_radius voxels by marching
+ * tetrahedra (the reference runs libigl's marching cubes on the same field;
+ * its optional smoothing and decimation, mesh_builder.h:201-210, are not
+ * done). Returns the triangle count. Blocking.
+ * mpm_mesh_triangles: the triangles, 9 floats each (3 vertices, normal
+ * pointing out by the right-hand rule), and for each vertex the grid edge it
+ * lies on (lower node index << 32 | upper node index; NULL skips them).
+ * mpm_mesh_distance_field: the grid_side^3 distances of the last build
+ * (index (i*side+j)*side+k; grid_side must be that build's).
+ * mpm_mesh_write_obj: the mesh as OBJ text, one vertex per crossed grid edge. */
+int mpm_mesh_build(mpm_handle h, int32_t grid_side, int32_t particle_radius, int64_t* triangles,
+                   void* stream);
+int mpm_mesh_triangles(mpm_handle h, float* xyz, uint64_t* edges);
+int mpm_mesh_distance_field(mpm_handle h, int32_t grid_side, float* out);
+int mpm_mesh_write_obj(mpm_handle h, const char* path);
+
 /* ---- full-state checkpoint (no reference counterpart: the reference writes
  *      only x and v, particle_writer.h:32-37). Binary, little-endian. ---- */
 int mpm_save_state(mpm_handle h, const char* path);

@@ -70,6 +70,7 @@ struct CLIOptions {
   uint32_t mesh_particle_radius = 5;
   int32_t mesh_face_count = -1;
   int laplacian_smooth = 0;
+  bool no_mesh = false;  // (no counterpart: the reference always writes meshes with --save-dir)
   // not reference flags
   float gravity = -9.81f;
   float boundary = 0.05f;
@@ -93,6 +94,9 @@ struct CLIOptions {
       } else if (a == "help") {
         usage();
         std::exit(0);
+      } else if (a == "no-mesh") {
+        no_mesh = true;
+        continue;
       } else {
         if (i + 1 >= argc) die("option '" + a + "' needs a value");
         val = argv[++i];
@@ -128,10 +132,11 @@ struct CLIOptions {
         "  --save-dir arg              Where to save particles (default none)\n"
         "  --scene arg                 Scene file to load (default scenes/snowman.toml)\n"
         "  --particle-count arg        Particles per unit volume (default 500000)\n"
-        "  --mesh-grid arg             accepted, unused (no mesh export)\n"
-        "  --mesh-particle-radius arg  accepted, unused\n"
-        "  --mesh-face-count arg       accepted, unused\n"
-        "  --laplacian_smooth arg      accepted, unused\n"
+        "  --mesh-grid arg             Grid size for computing mesh (default 250)\n"
+        "  --mesh-particle-radius arg  Particle radius for the mesh, in grid points (default 5)\n"
+        "  --mesh-face-count arg       accepted, unused (no decimation)\n"
+        "  --laplacian_smooth arg      accepted, unused (no smoothing)\n"
+        "  --no-mesh                   with --save-dir, write particles only\n"
         "  --steps arg                 substeps to run (default 1000)\n"
         "  --gravity arg               y acceleration (default -9.81)\n"
         "  --boundary arg              wall fraction (default 0.05)\n"
@@ -320,6 +325,27 @@ class Simulation {
   std::vector<char> uploaded_;
   std::vector<size_t> on_device_;  // objects on the device, in the order of their particles there
   std::vector<Particle> all_;
+};
+
+// MeshBuilder (include/mesh_builder.h:19-35,181-211): the surface mesh of the
+// simulation's particles to an OBJ file, built on the GPU from the device
+// particles (mpm_mesh_build; no smoothing or decimation).
+class MeshBuilder {
+ public:
+  MeshBuilder(Simulation& sim, const CLIOptions& flags, uint32_t grid_size)
+      : sim_(sim), radius_(flags.mesh_particle_radius), side_(grid_size) {}
+  // the triangle count; throws on failure
+  int64_t computeMesh(const std::string& filename) {
+    int64_t n = 0;
+    check(mpm_mesh_build(sim_.handle(), (int32_t)side_, (int32_t)radius_, &n, nullptr), sim_.handle(),
+          "mpm_mesh_build");
+    check(mpm_mesh_write_obj(sim_.handle(), filename.c_str()), sim_.handle(), "mpm_mesh_write_obj");
+    return n;
+  }
+
+ private:
+  Simulation& sim_;
+  uint32_t radius_, side_;
 };
 
 }  // namespace mpm

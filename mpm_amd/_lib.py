@@ -98,6 +98,10 @@ SIGNATURES = {
     "mpm_clear_particles": (C.c_int, [_P]),
     "mpm_upload": (C.c_int, [_P, _P]),
     "mpm_append_particles": (C.c_int, [_P, _P]),
+    "mpm_mesh_build": (C.c_int, [_P, C.c_int32, C.c_int32, _P, _P]),
+    "mpm_mesh_triangles": (C.c_int, [_P, _P, _P]),
+    "mpm_mesh_distance_field": (C.c_int, [_P, C.c_int32, _P]),
+    "mpm_mesh_write_obj": (C.c_int, [_P, C.c_char_p]),
     "mpm_count": (C.c_int, [_P, _i64p]),
     "mpm_advance": (C.c_int, [_P, C.c_int32, _P]),
     "mpm_time": (C.c_int, [_P, C.POINTER(C.c_double)]),

@@ -39,7 +39,7 @@ COMMON = [
 ]
 
 SOURCES = ["capi.cpp", "engine.cpp", "batch.cpp", "substep.hip", "p2g_block.hip", "sort.hip",
-           "linalg.hip", "adjoint.hip", "scene.cpp"]
+           "linalg.hip", "adjoint.hip", "scene.cpp", "mesh.hip"]
 
 
 def _compile(src: Path) -> Path:

@@ -146,6 +146,27 @@ extern "C" int mpm_append_particles(mpm_handle h, void* stream) {
   return s->append((hipStream_t)stream);
 }
 
+extern "C" int mpm_mesh_build(mpm_handle h, int32_t grid_side, int32_t particle_radius, int64_t* triangles,
+                              void* stream) {
+  MPM_GET(h);
+  return s->mesh_build(grid_side, particle_radius, triangles, (hipStream_t)stream);
+}
+
+extern "C" int mpm_mesh_triangles(mpm_handle h, float* xyz, uint64_t* edges) {
+  MPM_GET(h);
+  return s->mesh_triangles(xyz, edges);
+}
+
+extern "C" int mpm_mesh_distance_field(mpm_handle h, int32_t grid_side, float* out) {
+  MPM_GET(h);
+  return s->mesh_distance(grid_side, out);
+}
+
+extern "C" int mpm_mesh_write_obj(mpm_handle h, const char* path) {
+  MPM_GET(h);
+  return s->mesh_write_obj(path);
+}
+
 extern "C" int mpm_count(mpm_handle h, int64_t* out) {
   MPM_GET(h);
   if (!out) return s->fail(MPM_ERR_INVALID, "mpm_count: null output");

@@ -89,9 +89,12 @@ Sim::~Sim() {
   if (d_hist_) (void)hipFree(d_hist_);
   if (d_mats_) (void)hipFree(d_mats_);
   for (uint32_t* p : {d_block_start_, d_block_items_, d_flag_, d_pos_, d_gather_, d_item_start_,
-                      d_item_block_, d_item_waves_, d_item_count_, d_dirty_, d_covered_, d_live_})
+                      d_item_block_, d_item_waves_, d_item_count_, d_dirty_, d_covered_, d_live_,
+                      d_sdf_, d_mcount_, d_moff_, d_mscratch_})
     if (p) (void)hipFree(p);
   if (d_tiles_) (void)hipFree(d_tiles_);
+  if (d_mxyz_) (void)hipFree(d_mxyz_);
+  if (d_mkeys_) (void)hipFree(d_mkeys_);
   for (auto& p : ev_) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
   for (auto ev : ev_pool_) (void)hipEventDestroy(ev);
   for (auto ev : open_) if (ev) (void)hipEventDestroy(ev);
@@ -520,6 +523,114 @@ int Sim::append(hipStream_t s) {
   primed_ = false;
   side_clean_ = false;      // the last pass's side-grid scatters are dropped with its tiles
   steps_since_sort_ = 1 << 30;
+  return MPM_OK;
+}
+
+// Surface mesh (mesh.hip) of the device particles, as the reference's
+// MeshBuilder::computeMesh does it on the host (mesh_builder.h:181-211):
+// voxel_dx = N dx / side (float N dx, then double), distance cut-off
+// (radius + 1) voxel_dx, iso-surface at radius * voxel_dx.
+int Sim::mesh_build(int side, int radius, int64_t* ntri, hipStream_t s) {
+  ++gen_;
+  if (!uploaded_) return fail(MPM_ERR_STATE, "mpm_mesh_build: nothing uploaded");
+  if (side < 2 || side > 1024) return fail(MPM_ERR_INVALID, "mpm_mesh_build: grid side must be 2..1024");
+  if (radius < 0 || radius > side) return fail(MPM_ERR_INVALID, "mpm_mesh_build: radius must be 0..side");
+  const double vdx = (double)((float)k_.N * k_.dx) / (double)side;
+  const float vdx_f = (float)vdx, cutoff = (float)((double)(radius + 1) * vdx);
+  const double rv = (double)radius * vdx;
+  const size_t nv = (size_t)side * side * side, nc = (size_t)(side - 1) * (side - 1) * (side - 1);
+  if (nv > sdf_cap_) {
+    if (d_sdf_) (void)hipFree(d_sdf_);
+    d_sdf_ = nullptr;
+    sdf_cap_ = 0;
+    HIPCHK(hipMalloc(&d_sdf_, sizeof(uint32_t) * nv), "hipMalloc(distance field)");
+    sdf_cap_ = nv;
+  }
+  if (nc > mcube_cap_) {
+    for (uint32_t** q : {&d_mcount_, &d_moff_, &d_mscratch_}) {
+      if (*q) (void)hipFree(*q);
+      *q = nullptr;
+    }
+    mcube_cap_ = 0;
+    HIPCHK(hipMalloc(&d_mcount_, sizeof(uint32_t) * nc), "hipMalloc(mesh counts)");
+    HIPCHK(hipMalloc(&d_moff_, sizeof(uint32_t) * (nc + 1)), "hipMalloc(mesh offsets)");
+    // scan scratch: one sum per tile of 2048 counts, and the total at the end
+    HIPCHK(hipMalloc(&d_mscratch_, sizeof(uint32_t) * (nc / 2048 + 2)), "hipMalloc(mesh scan)");
+    mcube_cap_ = nc;
+  }
+  HIPCHK(mpmk::mesh_distance_field(particles(), side, vdx_f, cutoff, d_sdf_, s), "mesh distance field");
+  HIPCHK(mpmk::mesh_count(d_sdf_, side, rv, d_mcount_, s), "mesh count");
+  uint32_t* total = d_moff_ + nc;
+  HIPCHK(mpmk::exclusive_scan(d_mcount_, d_moff_, (int64_t)nc, d_mscratch_, total, s), "mesh scan");
+  uint32_t n = 0;
+  HIPCHK(hipMemcpyAsync(&n, total, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "mesh size");
+  HIPCHK(hipStreamSynchronize(s), "mesh size");
+  if ((size_t)n > mtri_cap_) {
+    if (d_mxyz_) (void)hipFree(d_mxyz_);
+    if (d_mkeys_) (void)hipFree(d_mkeys_);
+    d_mxyz_ = nullptr;
+    d_mkeys_ = nullptr;
+    mtri_cap_ = 0;
+    HIPCHK(hipMalloc(&d_mxyz_, sizeof(float) * 9 * (size_t)n), "hipMalloc(mesh triangles)");
+    HIPCHK(hipMalloc(&d_mkeys_, sizeof(uint64_t) * 3 * (size_t)n), "hipMalloc(mesh edges)");
+    mtri_cap_ = n;
+  }
+  if (n > 0) HIPCHK(mpmk::mesh_emit(d_sdf_, side, rv, vdx, d_moff_, d_mxyz_, d_mkeys_, s), "mesh emit");
+  HIPCHK(hipStreamSynchronize(s), "mesh emit");
+  mesh_side_ = side;
+  mesh_ntri_ = n;
+  if (ntri) *ntri = n;
+  return MPM_OK;
+}
+
+int Sim::mesh_triangles(float* xyz, uint64_t* keys) {
+  if (mesh_ntri_ < 0) return fail(MPM_ERR_STATE, "mpm_mesh_triangles: no mesh built");
+  const size_t n = (size_t)mesh_ntri_;
+  if (n == 0) return MPM_OK;
+  if (xyz) HIPCHK(hipMemcpy(xyz, d_mxyz_, sizeof(float) * 9 * n, hipMemcpyDeviceToHost), "mesh triangles");
+  if (keys) HIPCHK(hipMemcpy(keys, d_mkeys_, sizeof(uint64_t) * 3 * n, hipMemcpyDeviceToHost), "mesh edges");
+  return MPM_OK;
+}
+
+int Sim::mesh_distance(int side, float* out) {
+  if (mesh_ntri_ < 0) return fail(MPM_ERR_STATE, "mpm_mesh_distance_field: no mesh built");
+  if (!out || side != mesh_side_)
+    return fail(MPM_ERR_INVALID, "mpm_mesh_distance_field: the last mesh has grid side %d", mesh_side_);
+  const size_t nv = (size_t)mesh_side_ * mesh_side_ * mesh_side_;
+  HIPCHK(hipMemcpy(out, d_sdf_, sizeof(float) * nv, hipMemcpyDeviceToHost), "mesh distance field");
+  return MPM_OK;
+}
+
+// OBJ of the last mesh: one vertex per grid edge the surface crosses (the
+// triangles that share an edge share its vertex bits), 1-based faces.
+int Sim::mesh_write_obj(const char* path) {
+  if (mesh_ntri_ < 0) return fail(MPM_ERR_STATE, "mpm_mesh_write_obj: no mesh built");
+  if (!path) return fail(MPM_ERR_INVALID, "mpm_mesh_write_obj: null path");
+  const size_t n = (size_t)mesh_ntri_;
+  std::vector<float> xyz(9 * n);
+  std::vector<uint64_t> keys(3 * n);
+  int rc = mesh_triangles(xyz.data(), keys.data());
+  if (rc) return rc;
+  std::vector<uint64_t> order(keys);
+  std::sort(order.begin(), order.end());
+  order.erase(std::unique(order.begin(), order.end()), order.end());
+  std::vector<int64_t> first(order.size(), -1);
+  std::vector<int64_t> face(3 * n);
+  for (size_t t = 0; t < 3 * n; ++t) {
+    const size_t v = (size_t)(std::lower_bound(order.begin(), order.end(), keys[t]) - order.begin());
+    if (first[v] < 0) first[v] = (int64_t)t;
+    face[t] = (int64_t)v + 1;
+  }
+  FILE* f = std::fopen(path, "w");
+  if (!f) return fail(MPM_ERR_IO, "mpm_mesh_write_obj: cannot open %s", path);
+  for (size_t v = 0; v < order.size(); ++v) {
+    const float* p = &xyz[3 * (size_t)first[v]];
+    std::fprintf(f, "v %.9g %.9g %.9g\n", p[0], p[1], p[2]);
+  }
+  for (size_t t = 0; t < n; ++t)
+    std::fprintf(f, "f %lld %lld %lld\n", (long long)face[3 * t], (long long)face[3 * t + 1],
+                 (long long)face[3 * t + 2]);
+  if (std::fclose(f) != 0) return fail(MPM_ERR_IO, "mpm_mesh_write_obj: cannot write %s", path);
   return MPM_OK;
 }
 

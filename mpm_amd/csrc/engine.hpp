@@ -56,6 +56,11 @@ class Sim {
   int clear_staging();
   int upload(hipStream_t s);
   int append(hipStream_t s);
+  // surface mesh of the device particles (mesh.hip)
+  int mesh_build(int side, int radius, int64_t* ntri, hipStream_t s);
+  int mesh_triangles(float* xyz, uint64_t* keys);
+  int mesh_distance(int side, float* out);
+  int mesh_write_obj(const char* path);
   int64_t count() const { return P_; }
   double time() const { return t_; }
   int advance(int32_t substeps, hipStream_t s);
@@ -161,6 +166,17 @@ class Sim {
   bool use_mlp() const;
   bool dense_ = false;
   size_t staged_on_device_ = 0;  // staged particles the last upload / append copied
+  // mesh buffers: distance field, per-cube counts and offsets, scan scratch,
+  // triangles; the last build's grid side and triangle count
+  uint32_t* d_sdf_ = nullptr;
+  uint32_t* d_mcount_ = nullptr;
+  uint32_t* d_moff_ = nullptr;
+  uint32_t* d_mscratch_ = nullptr;
+  float* d_mxyz_ = nullptr;
+  uint64_t* d_mkeys_ = nullptr;
+  size_t sdf_cap_ = 0, mcube_cap_ = 0, mtri_cap_ = 0;
+  int mesh_side_ = 0;
+  int64_t mesh_ntri_ = -1;
   int ensure_f(hipStream_t s);
   int64_t timed_subs_ = 0;   // substeps since timing_reset
 

@@ -196,6 +196,18 @@ hipError_t liquid_f(const ParticlesDev& P, hipStream_t s);
 hipError_t tile_reduce(const TileSums& in, const StepConsts& k, float4* grid, int update, int mlp,
                        hipStream_t s);
 
+// mesh.hip: surface mesh of the particles. The distance field (side^3
+// voxels of edge vdx, float bits) starts at `cutoff` and takes the distance
+// to every particle whose box of voxels reaches it; mesh_count writes the
+// triangle count of each of the (side-1)^3 cubes of S = rv - distance = 0,
+// mesh_emit writes them (9 floats each, and the 3 grid edges their
+// vertices lie on) at the scanned offsets.
+hipError_t mesh_distance_field(const ParticlesDev& P, int side, float vdx, float cutoff, uint32_t* sdf,
+                               hipStream_t s);
+hipError_t mesh_count(const uint32_t* sdf, int side, double rv, uint32_t* count, hipStream_t s);
+hipError_t mesh_emit(const uint32_t* sdf, int side, double rv, double vdx, const uint32_t* offset, float* xyz,
+                     uint64_t* keys, hipStream_t s);
+
 // adjoint.hip: the differentiable substep (24 state planes of n floats) and
 // its adjoint; grid is 4 N^3 floats of scratch, tape 8 N^3 floats.
 hipError_t diff_forward(const StepConsts& k, float mass, float volume, int64_t n, const float* st,

@@ -2,13 +2,18 @@
 // the C-ABI. Loads a scene TOML (materials, objects, meshes under
 // <scene dir>/meshes/), fills the objects with particles, and advances the
 // simulation, syncing every --sync-every substeps (main.cu:99-101) and, with
-// --save-dir, writing the active particles 240 times per simulated second
-// (main.cu:7,93,102-113) to <save-dir>/particles/particles_<frame>.ply.
+// --save-dir, writing 240 times per simulated second (main.cu:7,93,102-113)
+// the surface mesh to <save-dir>/meshes/mesh_<frame>.obj (MeshBuilder,
+// main.cu:78,104-106: --mesh-grid voxels, --mesh-particle-radius) and the
+// active particles to <save-dir>/particles/particles_<frame>.ply.
 //
 // Differences: the run stops after --steps substeps (the reference loops
 // until killed, rendering to a window); particles are written as binary PLY
 // (x, y, z, vx, vy, vz, id) because the reference's .bgeo writer needs the
-// Partio library (particle_writer.h:1-3, absent here); no OBJ surface meshes.
+// Partio library (particle_writer.h:1-3, absent here); the mesh is built on
+// the GPU by marching tetrahedra, without libigl's smoothing or decimation
+// (--laplacian_smooth, --mesh-face-count are accepted and ignored);
+// --no-mesh skips it.
 #include <sys/stat.h>
 
 #include <chrono>
@@ -79,10 +84,13 @@ int main(int argc, char* argv[]) {
     simulation.initCuda();
 
     const bool save = !flags.save_dir.empty();
+    const bool mesh = save && !flags.no_mesh;
     if (save) {
       make_dir(flags.save_dir);
       make_dir(flags.save_dir + "/particles");
+      if (mesh) make_dir(flags.save_dir + "/meshes");
     }
+    mpm::MeshBuilder mesher(simulation, flags, flags.mesh_grid);
     const uint32_t save_every = std::max<uint32_t>(1, (uint32_t)(1.0 / 240.0 / flags.dt));
     uint32_t frame_id = 0;
     const auto t0 = std::chrono::steady_clock::now();
@@ -96,6 +104,10 @@ int main(int argc, char* argv[]) {
       if (save && i % save_every == 0) {
         if (i % flags.sync_every != 0) simulation.syncDevice();
         char name[64];
+        if (mesh && simulation.active_particle_count > 0) {
+          std::snprintf(name, sizeof(name), "/meshes/mesh_%05u.obj", frame_id);
+          mesher.computeMesh(flags.save_dir + name);
+        }
         std::snprintf(name, sizeof(name), "/particles/particles_%05u.ply", frame_id++);
         if (!write_ply(flags.save_dir + name, simulation.getActiveParticleList())) {
           std::fprintf(stderr, "cannot write %s%s\n", flags.save_dir.c_str(), name);

@@ -161,6 +161,26 @@ class Engine:
         self._chk(lib().mpm_append_particles(self._h, _stream(stream)))
         self._n = self.count()
 
+    def mesh(self, grid_side: int = 250, radius: int = 5, stream=None) -> dict:
+        """Surface mesh of the device particles (mpm_mesh_build): 'triangles'
+        (n, 3, 3) float32 vertices, normals out by the right-hand rule, and
+        'edges' (n, 3) uint64, the grid edge each vertex lies on."""
+        n = C.c_int64()
+        self._chk(lib().mpm_mesh_build(self._h, grid_side, radius, C.byref(n), _stream(stream)))
+        tri = np.empty((n.value, 3, 3), np.float32)
+        edges = np.empty((n.value, 3), np.uint64)
+        self._chk(lib().mpm_mesh_triangles(self._h, _fp(tri), edges.ctypes.data))
+        return {"triangles": tri, "edges": edges, "grid_side": grid_side}
+
+    def mesh_distance_field(self, grid_side: int) -> np.ndarray:
+        """The distance field of the last mesh_build, (side, side, side)."""
+        d = np.empty((grid_side,) * 3, np.float32)
+        self._chk(lib().mpm_mesh_distance_field(self._h, grid_side, _fp(d)))
+        return d
+
+    def write_mesh_obj(self, path: str) -> None:
+        self._chk(lib().mpm_mesh_write_obj(self._h, str(path).encode()))
+
     def count(self) -> int:
         v = C.c_int64()
         self._chk(lib().mpm_count(self._h, C.byref(v)))

@@ -96,11 +96,18 @@ def test_lifetime_activation_matches_python_binding(scene, tmp_path, gpu_lib):
     out = tmp_path / "out"
     pc, dt, N, steps = 60000, 1e-3, 32, 30
     r = run("--scene", scene, "--particle-count", pc, "--dt", dt, "--N", N, "--steps", steps,
-            "--save-dir", out, "--sync-every", 20)
+            "--save-dir", out, "--sync-every", 20, "--mesh-grid", 64, "--mesh-particle-radius", 2)
     assert r.returncode == 0, r.stdout + r.stderr
     frames = sorted((out / "particles").glob("particles_*.ply"))
     save_every = int(1.0 / 240.0 / np.float32(dt))
     assert len(frames) == (steps + save_every - 1) // save_every
+    # a surface mesh per frame (MeshBuilder): OBJ with vertices and faces
+    meshes = sorted((out / "meshes").glob("mesh_*.obj"))
+    assert len(meshes) == len(frames)
+    text = meshes[-1].read_text().splitlines()
+    nv = sum(1 for l in text if l.startswith("v "))
+    faces = np.array([[int(v) for v in l.split()[1:]] for l in text if l.startswith("f ")])
+    assert nv > 100 and len(faces) > 100 and faces.min() == 1 and faces.max() == nv
     sc = load_scene(str(scene), pc, seed=1)
     a, b = sc.objects
     # frame k is written after substep i = k * save_every, i.e. after i + 1 substeps

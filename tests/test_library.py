@@ -23,7 +23,8 @@ def declared_functions():
 def test_header_declares_the_boundary():
     names = declared_functions()
     for must in ("mpm_create", "mpm_destroy", "mpm_set_materials", "mpm_add_particles_soa",
-                 "mpm_add_particles_aos104", "mpm_upload", "mpm_append_particles", "mpm_advance", "mpm_download_soa",
+                 "mpm_add_particles_aos104", "mpm_upload", "mpm_append_particles", "mpm_mesh_build",
+                 "mpm_mesh_triangles", "mpm_mesh_distance_field", "mpm_mesh_write_obj", "mpm_advance", "mpm_download_soa",
                  "mpm_download_aos104", "mpm_save_state", "mpm_load_state", "mpm_last_error"):
         assert must in names
 
