@@ -45,9 +45,13 @@ __global__ void __launch_bounds__(256)
   for (int i = beg[0]; i < end[0]; ++i) {
     const float dx = x[0] - (float)(uint32_t)i * vdx;
     const float dx2 = dx * dx;
+    // rows and planes whose nearest voxel is at the cut-off or beyond change
+    // nothing: d >= sqrt(dx^2 + dy^2) >= sqrt(dx^2) in float as well
+    if (sqrtf(dx2) >= cutoff) continue;
     for (int j = beg[1]; j < end[1]; ++j) {
       const float dy = x[1] - (float)(uint32_t)j * vdx;
       const float dxy = dx2 + dy * dy;
+      if (sqrtf(dxy) >= cutoff) continue;
       const size_t row = ((size_t)i * side + (size_t)j) * side;
       for (int k = beg[2]; k < end[2]; ++k) {
         const float dz = x[2] - (float)(uint32_t)k * vdx;

@@ -4,6 +4,8 @@ Subcommands (run as ``python -m mpm_amd.profiling <cmd>``):
 
   run        advance one of the benchmark scenes for --steps substeps after
              --warmup substeps (the program profiled under rocprofv3)
+  mesh       build the surface mesh of a benchmark scene --reps times after
+             --warmup substeps (mpm_mesh_build at --side voxels, --radius)
   copy       a plain device-to-device copy of --mib MiB, --reps times (the
              calibration kernel for the FETCH_SIZE / WRITE_SIZE counters)
   summarize  per-kernel table of a rocprofv3 output: a rocpd database
@@ -47,6 +49,28 @@ def cmd_run(a) -> int:
     eng.close()
     print(f"ran {a.workload}: {sc.n} particles, N={sc.params.N}, {a.warmup}+{a.steps} substeps",
           flush=True)
+    return 0
+
+
+def cmd_mesh(a) -> int:
+    import time
+    import torch
+    from . import Engine, scenes
+    sc = getattr(scenes, a.workload)()
+    st = sc.state()
+    eng = Engine(sc.params)
+    eng.set_materials(sc.materials)
+    eng.add_particles(st["x"], st["material"], st["v"], st["C"], st["F"], st["Jp"])
+    eng.upload()
+    eng.advance(a.warmup)
+    torch.cuda.synchronize()
+    for _ in range(a.reps):
+        t0 = time.perf_counter()
+        m = eng.mesh(a.side, a.radius)
+        t1 = time.perf_counter()
+        print(f"mesh {a.workload} side={a.side} radius={a.radius}: {len(m['triangles'])} triangles, "
+              f"{1e3 * (t1 - t0):.1f} ms (build + copy to host)", flush=True)
+    eng.close()
     return 0
 
 
@@ -123,6 +147,12 @@ def main(argv=None) -> int:
     r.add_argument("--steps", type=int, default=20)
     r.add_argument("--warmup", type=int, default=5)
     r.add_argument("--option", action="append", help="engine option name=value")
+    m = sub.add_parser("mesh")
+    m.add_argument("--workload", default="dam_break", choices=["dam_break", "snow_spheres", "elastic_sphere"])
+    m.add_argument("--warmup", type=int, default=200)
+    m.add_argument("--side", type=int, default=250)
+    m.add_argument("--radius", type=int, default=5)
+    m.add_argument("--reps", type=int, default=3)
     c = sub.add_parser("copy")
     c.add_argument("--mib", type=int, default=512)
     c.add_argument("--reps", type=int, default=3)
@@ -130,7 +160,7 @@ def main(argv=None) -> int:
     s.add_argument("paths", nargs="+")
     s.add_argument("--tail-of", default="", help="KERNEL:N -- only launches from the N-th last KERNEL on")
     a = ap.parse_args(argv)
-    return {"run": cmd_run, "copy": cmd_copy, "summarize": cmd_summarize}[a.cmd](a)
+    return {"run": cmd_run, "mesh": cmd_mesh, "copy": cmd_copy, "summarize": cmd_summarize}[a.cmd](a)
 
 
 if __name__ == "__main__":
