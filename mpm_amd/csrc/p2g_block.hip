@@ -729,27 +729,32 @@ __device__ __forceinline__ void reduce_block(const mpmk::ReduceDesc& d, const in
   float4* __restrict__ const grid = d.grid;
   const StepConsts k = d.k;
   const int nbx = d.nbx, update = d.update;
-  const int bz = (int)(nbk % nbx), by = (int)((nbk / nbx) % nbx), bx = (int)(nbk / ((int64_t)nbx * nbx));
+  // (32-bit: at most 256^3 node blocks)
+  const uint32_t nb32 = (uint32_t)nbk, nbxu = (uint32_t)nbx;
+  const int bz = (int)(nb32 % nbxu), by = (int)((nb32 / nbxu) % nbxu), bx = (int)(nb32 / (nbxu * nbxu));
   const int u0 = lane >> 4, u1 = (lane >> 2) & 3, u2 = lane & 3;
   const int N = k.N;
   const int x = bx * 4 + u0, y = by * 4 + u1, z = bz * 4 + u2;
   const bool in_grid = x < N && y < N && z < N;
   const size_t node = ((size_t)x * N + (size_t)y) * N + (size_t)z;
+  // the item ranges of the 27 home blocks around (lane j < 27: neighbour j),
+  // read in the same round trip as the block's flags
+  uint32_t r0 = 0, r1 = 0;
+  if (lane < 27) {
+    const int hx = bx + lane / 9 - 1, hy = by + (lane / 3) % 3 - 1, hz = bz + lane % 3 - 1;
+    if (hx >= 0 && hx < nbx && hy >= 0 && hy < nbx && hz >= 0 && hz < nbx) {
+      const uint32_t mb = morton3((uint32_t)hx, (uint32_t)hy, (uint32_t)hz);
+      const uint2 r = *reinterpret_cast<const uint2*>(block_items + 2 * (size_t)mb);
+      r0 = r.x;
+      r1 = r.y;
+    }
+  }
   const bool is_dirty = dirty[nbk] != 0u;
   const bool live = covered[nbk] || is_dirty;
   if (d.live && lane == 0) d.live[nbk] = live ? 1u : 0u;
   if (!live) {  // no tile and no side-grid scatter reaches it
     if (!d.skip_dead && in_grid) grid[node] = make_float4(0.f, 0.f, 0.f, 0.f);
     return;
-  }
-  uint32_t r0 = 0, r1 = 0;
-  if (lane < 27) {
-    const int hx = bx + lane / 9 - 1, hy = by + (lane / 3) % 3 - 1, hz = bz + lane % 3 - 1;
-    if (hx >= 0 && hx < nbx && hy >= 0 && hy < nbx && hz >= 0 && hz < nbx) {
-      const uint32_t mb = morton3((uint32_t)hx, (uint32_t)hy, (uint32_t)hz);
-      r0 = block_items[2 * (size_t)mb];
-      r1 = block_items[2 * (size_t)mb + 1];
-    }
   }
   float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
   if (__ballot(r0 < r1) != 0ull) {  // some home block around holds items
