@@ -99,3 +99,15 @@ def test_append_is_deterministic_and_empty_append_is_a_no_op(gpu_lib):
         e.close()
     for f in FIELDS:
         assert np.array_equal(outs[0][f], outs[1][f]), f
+
+
+def test_append_before_upload_is_an_error(gpu_lib):
+    from mpm_amd import MPMError
+    sc = scenes.snow_spheres(n=1_000, N=32)
+    st = sc.state()
+    e = Engine(sc.params)
+    e.set_materials(sc.materials)
+    e.add_particles(st["x"], st["material"])
+    with pytest.raises(MPMError, match="nothing uploaded"):
+        e.append()
+    e.close()

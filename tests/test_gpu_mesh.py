@@ -119,3 +119,20 @@ def test_scene_mesh_is_closed_and_obj_matches(gpu_lib, tmp_path):
     assert len(verts) == len(np.unique(keys)) and len(faces) == len(tri)
     assert faces.min() == 1 and faces.max() == len(verts)
     assert np.allclose(verts[faces - 1], tri, rtol=0, atol=1e-6)
+
+
+def test_mesh_errors_and_empty_set(gpu_lib):
+    from mpm_amd import MPMError
+    e = Engine(Params(dt=1e-4, N=32))
+    e.set_materials([make_material(MATERIAL_SNOW, 1e-6, 400.0)])
+    with pytest.raises(MPMError, match="nothing uploaded"):
+        e.mesh(32, 2)
+    e.upload()  # no particles: an empty mesh
+    m = e.mesh(32, 2)
+    assert len(m["triangles"]) == 0
+    with pytest.raises(MPMError, match="grid side"):
+        e.mesh(1, 2)
+    with pytest.raises(MPMError, match="grid side 32"):
+        e.mesh_distance_field(16)
+    assert np.all(e.mesh_distance_field(32) == voxel_geometry(32, 32, 2)[1])
+    e.close()
