@@ -10,7 +10,8 @@ Subcommands (run as ``python -m mpm_amd.profiling <cmd>``):
              calibration kernel for the FETCH_SIZE / WRITE_SIZE counters)
   summarize  per-kernel table of a rocprofv3 output: a rocpd database
              (kernel trace: calls, mean/min/max duration) or counter CSV files
-             (mean value per dispatch of every counter)
+             (mean value per dispatch of every counter, and the ratios of
+             the SQ counters: LDS bank conflicts, instructions per wavefront)
 
 The first two need a GPU; `summarize` runs anywhere.
 """
@@ -113,7 +114,22 @@ def summarize_db(path: Path, tail_of: str = "") -> str:
     return head + "\n".join(out)
 
 
-def summarize_csv(paths) -> str:
+# ratios printed under a kernel's counters when both counters were collected:
+# (name, numerator, denominator, meaning)
+_RATIOS = (
+    ("lds_bank_conflict_per_issue", "SQ_LDS_BANK_CONFLICT", "SQ_ACTIVE_INST_LDS",
+     "bank-conflict stall cycles per cycle spent issuing LDS instructions"),
+    ("lds_addr_conflict_per_issue", "SQ_LDS_ADDR_CONFLICT", "SQ_ACTIVE_INST_LDS",
+     "address-conflict stall cycles per LDS issue cycle"),
+    ("valu_insts_per_wave", "SQ_INSTS_VALU", "SQ_WAVES", "vector ALU instructions per wavefront"),
+    ("lds_insts_per_wave", "SQ_INSTS_LDS", "SQ_WAVES", "LDS instructions per wavefront"),
+    ("salu_insts_per_wave", "SQ_INSTS_SALU", "SQ_WAVES", "scalar ALU instructions per wavefront"),
+    ("lds_wait_frac", "SQ_WAIT_INST_LDS", "SQ_WAVE_CYCLES", "wave-cycles waiting on LDS / wave-cycles"),
+    ("any_wait_frac", "SQ_WAIT_ANY", "SQ_WAVE_CYCLES", "wave-cycles waiting on anything / wave-cycles"),
+)
+
+
+def summarize_csv(paths, kernels: str = "") -> str:
     per = defaultdict(lambda: defaultdict(list))
     for p in paths:
         with open(p) as f:
@@ -121,9 +137,15 @@ def summarize_csv(paths) -> str:
                 per[_clean(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
     out = []
     for n, d in sorted(per.items()):
+        if kernels and not re.search(kernels, n):
+            continue
         out.append(n)
+        mean = {c: sum(v) / len(v) for c, v in d.items()}
         for c, v in sorted(d.items()):
-            out.append(f"    {c:28s} {sum(v) / len(v):18.1f}   (mean of {len(v)} dispatches)")
+            out.append(f"    {c:28s} {mean[c]:18.1f}   (mean of {len(v)} dispatches)")
+        for name, num, den, what in _RATIOS:
+            if num in mean and den in mean and mean[den] > 0:
+                out.append(f"    = {name:26s} {mean[num] / mean[den]:18.4f}   ({what})")
     return "\n".join(out)
 
 
@@ -134,7 +156,7 @@ def cmd_summarize(a) -> int:
     for p in dbs:
         print(summarize_db(p, a.tail_of))
     if csvs:
-        print(summarize_csv(csvs))
+        print(summarize_csv(csvs, a.kernels))
     return 0
 
 
@@ -159,6 +181,7 @@ def main(argv=None) -> int:
     s = sub.add_parser("summarize")
     s.add_argument("paths", nargs="+")
     s.add_argument("--tail-of", default="", help="KERNEL:N -- only launches from the N-th last KERNEL on")
+    s.add_argument("--kernels", default="", help="regex: counter tables of the matching kernels only")
     a = ap.parse_args(argv)
     return {"run": cmd_run, "mesh": cmd_mesh, "copy": cmd_copy, "summarize": cmd_summarize}[a.cmd](a)
 
