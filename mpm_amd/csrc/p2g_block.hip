@@ -836,36 +836,74 @@ __global__ void __launch_bounds__(256) k_tile_reduce(mpmk::ReduceDesc d) {
   reduce_block<MLP>(d, nbk, lane);
 }
 
-// one workgroup per home block: tiles[first] += tiles[first + 1 .. end), in order
+// threads of a combine workgroup: two tile nodes each
+constexpr int kCombineThreads = mpmk::kTileNodes / 2;
+// home block b's tiles: tiles[first] += tiles[first + 1 .. end), in item order
 __device__ __forceinline__ void combine_block(const mpmk::ReduceDesc& d, const int64_t b) {
   constexpr int TN = mpmk::kTileNodes;
   const uint32_t r0 = d.block_items[2 * (size_t)b], r1 = d.block_items[2 * (size_t)b + 1];
   if (r1 <= r0 + 1) return;  // uniform for the workgroup
   float4* const t0 = const_cast<float4*>(d.tiles) + (size_t)r0 * TN;  // the tiles are the engine's scratch
-  for (int n = threadIdx.x; n < TN; n += 256) {
-    float4 acc = t0[n];
-    uint32_t it = r0 + 1;
-    for (; it + 4 <= r1; it += 4) {
-      float4 v[4];
+  // each thread sums its two nodes over the block's items in item order, 8
+  // items of both nodes in flight at a time (a crowded block holds dozens)
+  static_assert(TN == 2 * kCombineThreads, "two tile nodes per thread");
+  const int n0 = threadIdx.x, n1 = threadIdx.x + kCombineThreads;
+  float4 a0 = t0[n0], a1 = t0[n1];
+  const float4* __restrict__ const tiles = d.tiles;
+  uint32_t it = r0 + 1;
+  for (; it + 8 <= r1; it += 8) {
+    float4 v[8], u[8];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) v[u] = d.tiles[(size_t)(it + u) * TN + n];
+    for (int q = 0; q < 8; ++q) {
+      v[q] = tiles[(size_t)(it + q) * TN + n0];
+      u[q] = tiles[(size_t)(it + q) * TN + n1];
+    }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) { acc.x += v[u].x; acc.y += v[u].y; acc.z += v[u].z; acc.w += v[u].w; }
+    for (int q = 0; q < 8; ++q) {
+      a0.x += v[q].x; a0.y += v[q].y; a0.z += v[q].z; a0.w += v[q].w;
+      a1.x += u[q].x; a1.y += u[q].y; a1.z += u[q].z; a1.w += u[q].w;
     }
-    for (; it < r1; ++it) {
-      const float4 v = d.tiles[(size_t)it * TN + n];
-      acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
-    }
-    t0[n] = acc;
   }
+  for (; it < r1; ++it) {
+    const float4 v = tiles[(size_t)it * TN + n0], u = tiles[(size_t)it * TN + n1];
+    a0.x += v.x; a0.y += v.y; a0.z += v.z; a0.w += v.w;
+    a1.x += u.x; a1.y += u.y; a1.z += u.z; a1.w += u.w;
+  }
+  t0[n0] = a0;
+  t0[n1] = a1;
 }
 
-__global__ void __launch_bounds__(256)
+// kCombineBlocks home blocks per workgroup: lanes of wavefront 0 look up
+// their blocks together and the workgroup combines the crowded ones in turn
+// (most blocks hold at most one item: one workgroup per block spent its time
+// launching workgroups, many per workgroup serialised crowded neighbours)
+constexpr int kCombineBlocks = 4;
+__global__ void __launch_bounds__(kCombineThreads)
     k_tile_combine_multi(const mpmk::ReduceDesc* __restrict__ descs, const uint32_t* __restrict__ prefix,
                          int nscenes) {
-  const uint32_t g = blockIdx.x;
-  const int j = scene_of(prefix, nscenes, g);
-  combine_block(descs[j], (int64_t)(g - prefix[j]));
+  static_assert(kCombineBlocks <= 64, "one lane per block");
+  __shared__ int s_scene[kCombineBlocks];
+  __shared__ uint32_t s_block[kCombineBlocks];
+  __shared__ uint32_t s_mask;
+  if (threadIdx.x < 64) {
+    const uint32_t g = blockIdx.x * (uint32_t)kCombineBlocks + threadIdx.x;
+    bool need = false;
+    if (threadIdx.x < kCombineBlocks && g < prefix[nscenes]) {
+      const int j = scene_of(prefix, nscenes, g);
+      const uint32_t b = g - prefix[j];
+      const uint2 r = *reinterpret_cast<const uint2*>(descs[j].block_items + 2 * (size_t)b);
+      need = r.y > r.x + 1;
+      s_scene[threadIdx.x] = j;
+      s_block[threadIdx.x] = b;
+    }
+    const uint64_t m = __ballot(need);
+    if (threadIdx.x == 0) s_mask = (uint32_t)m;
+  }
+  __syncthreads();
+  for (uint32_t m = s_mask; m != 0u; m &= m - 1u) {
+    const int q = __builtin_ctz(m);
+    combine_block(descs[s_scene[q]], (int64_t)s_block[q]);
+  }
 }
 
 __global__ void __launch_bounds__(256)
@@ -937,8 +975,8 @@ hipError_t tile_combine_multi(const ReduceDesc* descs, const uint32_t* prefix, i
                               int64_t total_blocks, hipStream_t s) {
   if (nscenes <= 0 || total_blocks <= 0) return hipSuccess;
   if (!descs || !prefix || total_blocks > 0x7fffffff) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_tile_combine_multi, dim3((unsigned)total_blocks), dim3(256), 0, s, descs, prefix,
-                     nscenes);
+  hipLaunchKernelGGL(k_tile_combine_multi, dim3((unsigned)((total_blocks + kCombineBlocks - 1) / kCombineBlocks)),
+                     dim3(kCombineThreads), 0, s, descs, prefix, nscenes);
   return hipGetLastError();
 }
 
