@@ -38,7 +38,7 @@ COMMON = [
     "-I", str(PKG.parent / "include"),
 ]
 
-SOURCES = ["capi.cpp", "engine.cpp", "batch.cpp", "substep.hip", "p2g_block.hip", "sort.hip",
+SOURCES = ["capi.cpp", "material.cpp", "engine.cpp", "batch.cpp", "substep.hip", "p2g_block.hip", "sort.hip",
            "linalg.hip", "adjoint.hip", "scene.cpp", "mesh.hip"]
 
 

@@ -18,58 +18,16 @@
 
 #include "../../include/mpm_hip.h"
 #include "engine.hpp"
+#include "host_error.hpp"
 #include "kernels.hpp"
 
-namespace {
-
-thread_local std::string g_last_error;  // for calls without a usable handle
-
-void set_global_error(const char* fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof(buf), fmt, ap);
-  va_end(ap);
-  g_last_error = buf;
-}
-
-}  // namespace
-
+using mpm_host::set_global_error;
 using mpm_engine::Sim;
-
-// ---------------------------------------------------------------------------
-// material construction (MaterialModel.cuh:27-29,50-54,75-82; the TOML values
-// are doubles converted to the constructors' float parameters, main.cu:36-42)
-// ---------------------------------------------------------------------------
-extern "C" int mpm_material_init(mpm_material* m, int32_t kind, double volume, double density,
-                                 double E, double nu, double hardening, double clamp_lo,
-                                 double clamp_hi, double bulk_modulus) {
-  if (!m) {
-    set_global_error("mpm_material_init: null material");
-    return MPM_ERR_INVALID;
-  }
-  if (kind < MPM_MATERIAL_SNOW || kind > MPM_MATERIAL_WC_LIQUID) {
-    set_global_error("mpm_material_init: unknown material kind %d", kind);
-    return MPM_ERR_INVALID;
-  }
-  const float fv = (float)volume, fd = (float)density, fE = (float)E, fnu = (float)nu;
-  std::memset(m, 0, sizeof(*m));
-  m->kind = kind;
-  m->particle_volume = fv;
-  m->particle_mass = fd * fv;
-  m->mu0 = fE / (2.0f * (1.0f + fnu));
-  m->lambda0 = fE * fnu / ((1.0f + fnu) * (1.0f - 2.0f * fnu));
-  m->hardening = (float)hardening;
-  m->clamp_lo = (float)clamp_lo;
-  m->clamp_hi = (float)clamp_hi;
-  m->bulk_modulus = (float)bulk_modulus;
-  return MPM_OK;
-}
 
 extern "C" int mpm_abi_version(void) { return MPM_ABI_VERSION; }
 
 extern "C" const char* mpm_last_error(mpm_handle h) {
-  if (!h) return g_last_error.c_str();
+  if (!h) return mpm_host::global_error();
   return reinterpret_cast<Sim*>(h)->last_error().c_str();
 }
 
