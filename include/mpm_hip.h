@@ -214,6 +214,11 @@ int mpm_load_state(mpm_handle h, const char* path);
  *                at a time (more wavefronts resident). Same result either way.
  *   "timing"    k > 0: record HIP events around the kernel phases of one
  *                substep in every k (k = 1: every substep); 0 (default): off
+ *   "markers"   1: roctx ranges (rocprofiler-sdk) named "mpm_advance" around
+ *                each mpm_advance call and "sort", "g2p2g", "grid_update", ...
+ *                around the enqueue of each phase, for `rocprofv3
+ *                --marker-trace`; host-side (when the launches were issued,
+ *                not when the kernels ran). 0 (default): none
  * Results do not depend on the options beyond FP32 summation order. */
 int mpm_set_option(mpm_handle h, const char* name, int64_t value);
 int mpm_get_option(mpm_handle h, const char* name, int64_t* value);

@@ -17,6 +17,7 @@
 #include <numeric>
 
 #include "kernels.hpp"
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #define HIPCHK(expr, what)                 \
   do {                                     \
@@ -114,6 +115,7 @@ static hipError_t timing_event(hipEvent_t* ev) {
 }
 
 int Sim::phase_begin(int ph, hipStream_t s) {
+  if (opt_.markers) roctxRangePushA(phase_name(ph));
   if (!timing_now_) return MPM_OK;
   hipEvent_t ev;
   if (!ev_pool_.empty()) { ev = ev_pool_.back(); ev_pool_.pop_back(); }
@@ -124,6 +126,7 @@ int Sim::phase_begin(int ph, hipStream_t s) {
 }
 
 int Sim::phase_end(int ph, hipStream_t s) {
+  if (opt_.markers) roctxRangePop();
   if (!timing_now_ || !open_[ph]) return MPM_OK;
   hipEvent_t ev;
   if (!ev_pool_.empty()) { ev = ev_pool_.back(); ev_pool_.pop_back(); }
@@ -983,6 +986,13 @@ int Sim::advance(int32_t substeps, hipStream_t s) {
   if (rc0) return rc0;
   int rc = upload_materials();
   if (rc) return rc;
+  // a host-side range over the enqueue of the whole call (the phases inside
+  // it open their own, see phase_begin)
+  struct Range {
+    bool on;
+    explicit Range(bool o) : on(o) { if (on) roctxRangePushA("mpm_advance"); }
+    ~Range() { if (on) roctxRangePop(); }
+  } range(opt_.markers != 0);
   for (int32_t i = 0; i < substeps; ++i) {
     rc = substep(s);
     if (rc) return rc;
@@ -1307,6 +1317,7 @@ int Sim::set_option(const char* name, int64_t v) {
   if (n == "derive_f") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "derive_f must be 0 or 1"); opt_.derive_f = v; return MPM_OK; }
   if (n == "reduce_mlp") { if (v < -1 || v > 1) return fail(MPM_ERR_INVALID, "reduce_mlp must be -1, 0 or 1"); opt_.reduce_mlp = v; return MPM_OK; }
   if (n == "xcd_map") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "xcd_map must be 0 or 1"); opt_.xcd_map = v; return MPM_OK; }
+  if (n == "markers") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "markers must be 0 or 1"); opt_.markers = v; return MPM_OK; }
   if (n == "radix_bits") { if (v != 4 && v != 6 && v != 8) return fail(MPM_ERR_INVALID, "radix_bits must be 4, 6 or 8"); opt_.radix_bits = v; return MPM_OK; }
   if (n == "chunk") {
     if (v < 0 || v > mpmk::kMaxItemParticles || v % 64 != 0)
@@ -1342,6 +1353,7 @@ int Sim::get_option(const char* name, int64_t* v) {
   else if (n == "timing") *v = opt_.timing;
   else if (n == "derive_f") *v = opt_.derive_f;
   else if (n == "xcd_map") *v = opt_.xcd_map;
+  else if (n == "markers") *v = opt_.markers;
   else if (n == "reduce_mlp") *v = opt_.reduce_mlp;
   else return fail(MPM_ERR_INVALID, "mpm_get_option: unknown option '%s'", name);
   return MPM_OK;

@@ -31,6 +31,7 @@ struct Options {
   int64_t xcd_map = 1;      // "xcd_map": XCD-contiguous order of the fused kernel's items
   int64_t reduce_mlp = -1;  // "reduce_mlp": tile reduction batching further items of a block
                             // (1), one at a time (0), or chosen from the uploaded density (-1)
+  int64_t markers = 0;      // "markers": roctx ranges around each advance call and its phases
   int64_t timing = 0;       // "timing": k > 0 brackets the kernels of one substep in every k
                             // with HIP events (per-phase time); 0 off
 };

@@ -11,7 +11,8 @@ Subcommands (run as ``python -m mpm_amd.profiling <cmd>``):
   summarize  per-kernel table of a rocprofv3 output: a rocpd database
              (kernel trace: calls, mean/min/max duration) or counter CSV files
              (mean value per dispatch of every counter, and the ratios of
-             the SQ counters: LDS bank conflicts, instructions per wavefront)
+             the SQ counters: LDS bank conflicts, instructions per wavefront);
+             with --markers also the roctx ranges of a --marker-trace run
 
 The first two need a GPU; `summarize` runs anywhere.
 """
@@ -114,6 +115,25 @@ def summarize_db(path: Path, tail_of: str = "") -> str:
     return head + "\n".join(out)
 
 
+def summarize_markers(path: Path) -> str:
+    """roctx ranges of a --marker-trace run (engine option "markers"): count,
+    total and mean host-side duration per range name."""
+    import json
+    con = sqlite3.connect(str(path))
+    per = defaultdict(list)
+    for ext, start, end in con.execute("select extdata, start, end from regions"):
+        try:
+            msg = json.loads(ext).get("message") if ext else None
+        except ValueError:
+            msg = None
+        if msg:
+            per[msg].append((end - start) / 1e3)
+    out = [f"{'range':28s} {'calls':>6s} {'total_us':>11s} {'mean_us':>9s}   (host-side enqueue time)"]
+    for n, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+        out.append(f"{n[:28]:28s} {len(v):6d} {sum(v):11.1f} {sum(v) / len(v):9.2f}")
+    return "\n".join(out)
+
+
 # ratios printed under a kernel's counters when both counters were collected:
 # (name, numerator, denominator, meaning)
 _RATIOS = (
@@ -155,6 +175,9 @@ def cmd_summarize(a) -> int:
     csvs = [p for p in paths if p.suffix == ".csv"]
     for p in dbs:
         print(summarize_db(p, a.tail_of))
+        if a.markers:
+            print()
+            print(summarize_markers(p))
     if csvs:
         print(summarize_csv(csvs, a.kernels))
     return 0
@@ -182,6 +205,7 @@ def main(argv=None) -> int:
     s.add_argument("paths", nargs="+")
     s.add_argument("--tail-of", default="", help="KERNEL:N -- only launches from the N-th last KERNEL on")
     s.add_argument("--kernels", default="", help="regex: counter tables of the matching kernels only")
+    s.add_argument("--markers", action="store_true", help="also the roctx ranges of a --marker-trace run")
     a = ap.parse_args(argv)
     return {"run": cmd_run, "mesh": cmd_mesh, "copy": cmd_copy, "summarize": cmd_summarize}[a.cmd](a)
 

@@ -300,6 +300,21 @@ def test_reduce_strategies_are_bit_identical(gpu_lib):
             assert np.array_equal(outs[0][f], o[f]), f
 
 
+def test_markers_change_nothing(gpu_lib):
+    """roctx ranges (option "markers") are host-side only: the same bits with
+    and without them, timing events included."""
+    sc = scenes.snow_spheres(n=20_000, N=32)
+    outs = []
+    for markers in (1, 0):
+        eng = make_engine(sc, markers=markers, timing=1)
+        assert eng.get_option("markers") == markers
+        eng.advance(20)
+        outs.append(eng.download())
+        eng.close()
+    for f in ("x", "v", "C", "F", "Jp"):
+        assert np.array_equal(outs[0][f], outs[1][f]), f
+
+
 @pytest.mark.parametrize("opts", ["direct-unfused", "tiled-b4", "tiled-b4-chunk64", "tiled-unfused", "tiled-default"])
 def test_snow_10_substeps(gpu_lib, opts):
     sc = scenes.snow_spheres(n=40_000, N=32)
