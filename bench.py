@@ -242,7 +242,7 @@ def main() -> int:
         ms, n = eng.timing(ph)
         if n:
             phases[ph] = {"mean_us": 1e3 * ms / n, "launches": n}
-    eng_opts = {k: eng.get_option(k) for k in ("sort_every", "chunk")}
+    eng_opts = {k: eng.get_option(k) for k in ("sort_every", "chunk", "radix_bits")}
     out = eng.download()
     eng.close()
     finite = bool(torch.isfinite(torch.from_numpy(out["x"])).all())
@@ -273,7 +273,7 @@ def main() -> int:
                          "elastic_sphere": "elastic (snow model, clamps [0,1e30]) E=1e4 nu=0.3"}[args.workload],
             "substep_kernels": phases,
             "options": {"sort_every": eng_opts["sort_every"], "chunk": eng_opts["chunk"],
-                        "autotuned": tuned is not None},
+                        "radix_bits": eng_opts["radix_bits"], "autotuned": tuned is not None},
             "algorithmic_bytes_per_substep": scenes.algorithmic_bytes(P, N),
             "substep_fraction_of_hbm_roofline":
                 scenes.algorithmic_bytes(P, N) / sec_per_step / (HBM_PEAK_GBS * 1e9),

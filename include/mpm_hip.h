@@ -239,10 +239,12 @@ int mpm_timing_reset(mpm_handle h);
  * Times `trial_substeps` substeps (at least two sort intervals) of the
  * uploaded scene under each candidate sort interval (8, 12, 16, 24, 32, 48,
  * 64, 96 substeps) and work-item size (192 and 128 particles), each from the same
- * snapshot of the current state, after one interval of warm-up; keeps the
- * fastest as the "sort_every" and "chunk" options (tiled fused mode) and
- * puts the state and simulated time back as they were. The report (JSON,
- * truncated to report_len) lists the trials. Blocking. */
+ * snapshot of the current state, after one interval of warm-up; then, at the
+ * fastest of those, radix digits of 6 and 4 bits against the 8-bit ones (the
+ * sort bin width: a stable sort gives the same order with any). Keeps the
+ * fastest as the "sort_every", "chunk" and "radix_bits" options (tiled fused
+ * mode) and puts the state and simulated time back as they were. The report
+ * (JSON, truncated to report_len) lists the trials. Blocking. */
 int mpm_autotune(mpm_handle h, int32_t trial_substeps, char* report, int32_t report_len, void* stream);
 
 /* ---- kernel-level entry points used by the parity tests ---- */

@@ -688,8 +688,10 @@ int Sim::sort_particles(hipStream_t s) {
 int Sim::sort_desc(mpmk::SortDesc& d) {
   chunk_ = opt_.chunk > 0 ? (int)opt_.chunk : mpmk::kMaxItemParticles;
   const int64_t items = std::max<int64_t>(1, max_items());
-  if (P_ <= 0 || items > tiles_cap_ || opt_.radix_bits != 8)
-    return fail(MPM_ERR_STATE, "sort_desc: needs particles, allocated tiles and 8-bit radix digits");
+  // (the multi-scene sort always uses 8-bit digits: a stable sort gives the
+  // same order as the scene's own sort with any "radix_bits")
+  if (P_ <= 0 || items > tiles_cap_)
+    return fail(MPM_ERR_STATE, "sort_desc: needs particles and allocated tiles");
   d.P = particles();
   d.k = k_;
   d.keys[0] = d_keys_[0];
@@ -1243,32 +1245,46 @@ int Sim::autotune(int32_t trial_substeps, std::string* report, hipStream_t s) {
     t_ = t0;
     return r;
   };
-  struct Trial { int64_t sort_every, chunk; double us; };
+  struct Trial { int64_t sort_every, chunk, radix_bits; double us; };
   std::vector<Trial> trials;
+  auto run_trial = [&](int64_t se, int64_t ch, int64_t rb) -> bool {
+    if (e != hipSuccess) return false;
+    e = restore();
+    if (e != hipSuccess) return false;
+    opt_ = saved;
+    opt_.p2g_mode = 1;
+    opt_.fused = 1;
+    opt_.timing = 0;
+    opt_.sort_every = se;
+    opt_.chunk = ch;
+    opt_.radix_bits = rb;
+    const int32_t n = std::max<int32_t>(trial_substeps, (int32_t)(2 * se));
+    rc = advance((int32_t)se, s);  // one sort interval to the steady state
+    if (!rc) e = hipEventRecord(e0, s);
+    if (!rc && e == hipSuccess) rc = advance(n, s);
+    if (!rc && e == hipSuccess) e = hipEventRecord(e1, s);
+    if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (rc || e != hipSuccess) return false;
+    trials.push_back({se, ch, rb, 1e3 * (double)ms / n});
+    return true;
+  };
   const int64_t sorts[] = {8, 12, 16, 24, 32, 48, 64, 96};
   const int64_t chunks[] = {mpmk::kMaxItemParticles, 128};
-  for (int64_t ch : chunks) {
-    for (int64_t se : sorts) {
-      if (e != hipSuccess) break;
-      e = restore();
-      if (e != hipSuccess) break;
-      opt_ = saved;
-      opt_.p2g_mode = 1;
-      opt_.fused = 1;
-      opt_.timing = 0;
-      opt_.sort_every = se;
-      opt_.chunk = ch;
-      const int32_t n = std::max<int32_t>(trial_substeps, (int32_t)(2 * se));
-      rc = advance((int32_t)se, s);  // one sort interval to the steady state
-      if (!rc) e = hipEventRecord(e0, s);
-      if (!rc && e == hipSuccess) rc = advance(n, s);
-      if (!rc && e == hipSuccess) e = hipEventRecord(e1, s);
-      if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
-      float ms = 0.0f;
-      if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-      if (rc) break;
-      trials.push_back({se, ch, 1e3 * (double)ms / n});
-    }
+  bool ok = true;
+  for (int64_t ch : chunks)
+    for (int64_t se : sorts)
+      if (ok) ok = run_trial(se, ch, 8);
+  // then the radix digit width (sort bin width) at the best interval and
+  // item size: a stable sort gives the same order with any width
+  if (ok) {
+    size_t b = 0;
+    for (size_t i = 1; i < trials.size(); ++i)
+      if (trials[i].us < trials[b].us) b = i;
+    const Trial t = trials[b];
+    for (int64_t rb : {6, 4})
+      if (ok) ok = run_trial(t.sort_every, t.chunk, rb);
   }
   const hipError_t er = restore();
   if (e == hipSuccess) e = er;
@@ -1288,13 +1304,17 @@ int Sim::autotune(int32_t trial_substeps, std::string* report, hipStream_t s) {
   opt_.fused = 1;
   opt_.sort_every = trials[best].sort_every;
   opt_.chunk = trials[best].chunk == mpmk::kMaxItemParticles ? 0 : trials[best].chunk;
+  opt_.radix_bits = trials[best].radix_bits;
   if (report) {
-    char buf[160];
+    char buf[200];
     *report = "{\"sort_every\": " + std::to_string(opt_.sort_every) + ", \"chunk\": " +
-              std::to_string(trials[best].chunk) + ", \"trials\": [";
+              std::to_string(trials[best].chunk) + ", \"radix_bits\": " + std::to_string(opt_.radix_bits) +
+              ", \"trials\": [";
     for (size_t i = 0; i < trials.size(); ++i) {
-      std::snprintf(buf, sizeof(buf), "%s{\"sort_every\": %lld, \"chunk\": %lld, \"us_per_substep\": %.3f}",
-                    i ? ", " : "", (long long)trials[i].sort_every, (long long)trials[i].chunk, trials[i].us);
+      std::snprintf(buf, sizeof(buf),
+                    "%s{\"sort_every\": %lld, \"chunk\": %lld, \"radix_bits\": %lld, \"us_per_substep\": %.3f}",
+                    i ? ", " : "", (long long)trials[i].sort_every, (long long)trials[i].chunk,
+                    (long long)trials[i].radix_bits, trials[i].us);
       *report += buf;
     }
     *report += "]}";

@@ -208,8 +208,8 @@ class Engine:
 
     def autotune(self, trial_substeps: int = 64, cache: Optional[str] = None,
                  key: Optional[str] = None, stream=None) -> dict:
-        """Pick the sort interval and work-item size by timing them on this
-        scene (mpm_autotune; the state is restored). With `cache` (a JSON
+        """Pick the sort interval, work-item size and radix digit width by
+        timing them on this scene (mpm_autotune; the state is restored). With `cache` (a JSON
         file) and `key`, a stored choice is applied without timing, and a new
         one is stored. Returns the choice (and the trials when timed)."""
         import json
@@ -221,6 +221,7 @@ class Engine:
                 choice = stored[key]
                 self.set_option("sort_every", int(choice["sort_every"]))
                 self.set_option("chunk", int(choice["chunk"]))
+                self.set_option("radix_bits", int(choice.get("radix_bits", 8)))
                 return choice
         buf = C.create_string_buffer(8192)
         self._chk(lib().mpm_autotune(self._h, int(trial_substeps), buf, len(buf), _stream(stream)))
@@ -230,7 +231,8 @@ class Engine:
             if os.path.exists(cache):
                 with open(cache) as f:
                     stored = json.load(f)
-            stored[key] = {"sort_every": choice["sort_every"], "chunk": choice["chunk"]}
+            stored[key] = {"sort_every": choice["sort_every"], "chunk": choice["chunk"],
+                           "radix_bits": choice["radix_bits"]}
             with open(cache, "w") as f:
                 json.dump(stored, f, indent=1, sort_keys=True)
         return choice

@@ -108,3 +108,34 @@ def test_batched_workload_runs(gpu_lib):
     b.close()
     for e in engines:
         e.close()
+
+
+def test_batch_of_scenes_with_other_radix_digits(gpu_lib):
+    """A scene that sorts with 6- or 4-bit radix digits (an autotuner choice)
+    joins a batch, whose sort uses 8-bit digits: the stable sort gives the same
+    order, so the particles are the same bits as the scene's own run."""
+    scs = small_batch()[:3]
+    bits = (6, 4, 8)
+    k = 60
+
+    def with_bits(sc, b):
+        e = make(sc)
+        e.set_option("radix_bits", b)
+        return e
+
+    ref = []
+    for sc, b in zip(scs, bits):
+        e = with_bits(sc, b)
+        e.advance(k)
+        ref.append(e.download())
+        e.close()
+    engines = [with_bits(sc, b) for sc, b in zip(scs, bits)]
+    bt = Batch(engines)
+    bt.advance(k)
+    outs = [e.download() for e in engines]
+    bt.close()
+    for o, r in zip(outs, ref):
+        for f in ("x", "v", "C", "F", "Jp"):
+            assert np.array_equal(o[f], r[f]), f
+    for e in engines:
+        e.close()
