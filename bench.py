@@ -10,11 +10,17 @@ particle state resident in HBM. The default run times 2,000 substeps, the
 dam break's length. Prints one JSON line:
 
   value        particle-substeps/s (P / seconds per substep), whole job
-  roofline     the dominant kernel's algorithmic bytes per launch divided by
-               its mean launch time (HIP events recorded by the engine on the
-               launch stream during the timed region), against 8 TB/s HBM
+  roofline     the dominant kernel (largest share of a substep's time, the
+               sort amortised over its interval): its algorithmic bytes per
+               launch divided by its mean launch time (HIP events recorded by
+               the engine on the launch stream during the timed region, at
+               least 8 timed launches), against 8 TB/s HBM; `traffic` is the
+               rocprofv3-counted HBM bytes per launch of that kernel
   cpu_baseline the FP32 CPU oracle (a C++/OpenMP restatement of the
-               reference step) on a bounded sample of the same workload
+               reference step) on a bounded sample of the same workload, on
+               the OpenMP thread budget and on one thread, plus the 10,000-
+               particle elastic sphere; `accuracy` holds the engine's error
+               against the FP64 oracle over the workload's first 10 substeps
   batched      64 independent scenes advanced one after the other vs. as one
                batch (HIP graph replay): seconds and speedup
   adjoint      the differentiable elastic cube (32,768 particles, N=32, 200
@@ -37,6 +43,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 TIMING_EVERY = 16
+MIN_TIMED_LAUNCHES = 8
 
 
 def alg_bytes(phase: str, P: int, N: int) -> int:
@@ -71,19 +78,107 @@ def load_traffic(workload: str, kernel: str):
         return None
 
 
-def cpu_baseline(scene, steps: int, threads: int):
+def host_cpu() -> dict:
+    """The host CPU as the OS reports it: model name, logical CPUs, the CPUs
+    this process may run on, and the OpenMP thread budget (OMP_NUM_THREADS)."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return {"model": model, "logical_cpus": os.cpu_count(), "cpus_allowed": affinity,
+            "omp_num_threads": int(omp) if omp.isdigit() else None}
+
+
+def cpu_rate(scene, steps: int, threads: int, repeats: int) -> dict:
+    """Particle-substeps/s of the FP32 CPU oracle on `scene`: median of
+    `repeats` timed runs of `steps` substeps each, after one warm-up substep;
+    the state carries over from run to run."""
     import oracle
     st = scene.state()
     oracle.time_advance_f32(scene.params, scene.materials, st, 1, threads)  # warm-up substep
-    secs = oracle.time_advance_f32(scene.params, scene.materials, st, steps, threads)
+    secs = sorted(oracle.time_advance_f32(scene.params, scene.materials, st, steps, threads)
+                  for _ in range(repeats))
+    med = secs[len(secs) // 2]
+    return {"value": scene.n * steps / med, "threads": threads, "substeps": steps,
+            "repeats": repeats, "median_s": med, "workload": scene.name, "particles": scene.n,
+            "grid_N": scene.params.N}
+
+
+def fp64_check(scene, options: dict, steps: int, threads: int) -> dict:
+    """Error of the engine's first `steps` substeps of `scene` (run with the
+    bench's options) against the FP64 CPU oracle of the same step, with the
+    FP32 oracle's own error beside it. Per field: the norm-relative error
+    max|a - b| / max|b| and the element-wise one max |a - b| / max(|b|,
+    1e-3 max|b|) (C against the scale max(|C|, |v| N))."""
+    import numpy as np
+    import oracle
+    from mpm_amd import Engine
+
+    st = scene.state()
+    eng = Engine(scene.params)
+    eng.set_materials(scene.materials)
+    for k, v in options.items():
+        eng.set_option(k, v)
+    eng.add_particles(st["x"], st["material"], st["v"], st["C"], st["F"], st["Jp"])
+    eng.upload()
+    eng.advance(steps)
+    got = eng.download()
+    eng.close()
+    ref64 = oracle.advance(scene.params, scene.materials, st, steps, np.float64, threads=threads)
+    ref32 = oracle.advance(scene.params, scene.materials, st, steps, np.float32, threads=threads)
+    N = scene.params.N
+
+    def errs(a, b):
+        out = {}
+        vmax = float(np.abs(b["v"]).max())
+        for f in ("x", "v", "C", "F", "Jp"):
+            A = np.asarray(a[f], np.float64)
+            B = np.asarray(b[f], np.float64)
+            scale = max(float(np.abs(B).max()), vmax * N if f == "C" else 0.0)
+            diff = np.abs(A - B)
+            out[f] = {"norm_rel": float(diff.max() / scale) if scale > 0 else float(diff.max()),
+                      "elem_rel": float((diff / np.maximum(np.abs(B), 1e-3 * scale)).max())
+                      if scale > 0 else float(diff.max())}
+        return out
+
+    return {"substeps": steps, "workload": scene.name, "gpu_vs_fp64": errs(got, ref64),
+            "fp32_oracle_vs_fp64": errs(ref32, ref64)}
+
+
+def cpu_baseline_leg(scene, options: dict, cpu_steps: int):
+    """The reported CPU baseline (the FP32 C++/OpenMP oracle, `omp atomic`
+    scatter) on the host of the GPU: the bench workload on every thread of the
+    OpenMP budget, and on one thread; the 10,000-particle elastic sphere (N=16,
+    100 substeps) on both; and, as the check that the numbers time the same
+    step, the engine's first 10 substeps of the workload against the FP64
+    oracle."""
+    from mpm_amd import scenes
+
+    cpu = host_cpu()
+    threads = cpu["omp_num_threads"] or cpu["cpus_allowed"] or 1
+    main = cpu_rate(scene, max(1, cpu_steps // 5), threads, 5)
+    one = cpu_rate(scene, 1, 1, 3)
+    c1 = scenes.elastic_sphere()
+    c1_all = cpu_rate(c1, 100, threads, 3)
+    c1_one = cpu_rate(c1, 20, 1, 3)
+    sample = (f"{scene.name}: all {scene.n} particles, N={scene.params.N}, median of 5 runs of "
+              f"{main['substeps']} FP32 substeps after 1 warm-up, C++/OpenMP oracle (omp atomic "
+              f"scatter), {threads} threads ({main['median_s']:.2f} s per run); CPU "
+              f"{cpu['model']}, {cpu['cpus_allowed']} CPUs allowed of {cpu['logical_cpus']}")
     return {
-        "value": scene.n * steps / secs,
-        "unit": "particle-substeps/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": (f"{scene.name}: all {scene.n} particles, N={scene.params.N}, {steps} FP32 "
-                   f"substeps after 1 warm-up, C++/OpenMP oracle (omp atomic scatter), "
-                   f"{threads} threads, {secs:.2f} s"),
+        "value": main["value"], "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+        "sample": sample, "host": cpu,
+        "lines": [main, one, c1_all, c1_one],
+        "accuracy": fp64_check(scene, options, 10, threads),
     }
 
 
@@ -130,11 +225,17 @@ def batched_leg(warm: int, steps: int):
     for e in bat:
         e.close()
     P = sum(sc.n for sc in scs)
+    # algorithmic bytes of one substep of every scene (254 B per particle, 64 B
+    # per node of its own grid)
+    b_alg = sum(scenes.algorithmic_bytes(sc.n, sc.params.N) for sc in scs)
     return {
         "workload": f"{len(scs)} independent scenes, {P} particles, N in {{32,48,64}}",
         "substeps": steps, "warmup": warm,
         "sequential_s": t_seq, "batched_s": t_bat, "speedup": t_seq / t_bat,
         "batched_particle_substeps_per_s": P * steps / t_bat,
+        "algorithmic_bytes_per_substep": b_alg,
+        "batched_fraction_of_hbm_roofline": b_alg * steps / t_bat / (HBM_PEAK_GBS * 1e9),
+        "sequential_fraction_of_hbm_roofline": b_alg * steps / t_seq / (HBM_PEAK_GBS * 1e9),
         "graph_launches": launches,
         "multi_scene_cycles_recorded": fused,
     }
@@ -189,10 +290,11 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="dam_break", choices=["dam_break", "snow_spheres", "elastic_sphere"])
-    ap.add_argument("--cpu-steps", type=int, default=80)
+    ap.add_argument("--cpu-steps", type=int, default=20,
+                    help="substeps of the CPU baseline's main line (5 timed runs of a fifth each)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--batched-steps", type=int, default=128)
+    ap.add_argument("--batched-steps", type=int, default=500)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
@@ -226,10 +328,11 @@ def main() -> int:
     torch.cuda.synchronize()
 
     # timed region: events on the launch stream bracket the kernel phases of
-    # one substep in every k (k = the sort interval), rotating through the k
-    # positions of the interval (every substep timed would add ~8 us of event
-    # packets to each)
-    eng.set_option("timing", eng.get_option("sort_every") or TIMING_EVERY)
+    # one substep in every k, rotating through the k positions of a k-substep
+    # cycle; k is the sort interval, or less on a short run so that at least
+    # MIN_TIMED_LAUNCHES substeps are timed
+    sort_every = eng.get_option("sort_every") or TIMING_EVERY
+    eng.set_option("timing", max(1, min(sort_every, args.steps // MIN_TIMED_LAUNCHES)))
     eng.timing_reset()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -248,7 +351,10 @@ def main() -> int:
     finite = bool(torch.isfinite(torch.from_numpy(out["x"])).all())
 
     sec_per_step = (t1 - t0) / args.steps
-    dom = max(phases, key=lambda p: phases[p]["mean_us"] * phases[p]["launches"])
+    # the dominant kernel: the largest share of a substep's time, the sort
+    # counted once per sort interval, every other phase once per substep
+    per_substep = {ph: d["mean_us"] / (sort_every if ph == "sort" else 1) for ph, d in phases.items()}
+    dom = max(per_substep, key=per_substep.get)
     mean_s = phases[dom]["mean_us"] * 1e-6
     achieved = alg_bytes(dom, P, N) / mean_s / 1e9
     kernel_name = {"g2p2g": "k_g2p2g_tiled", "grid_update": "k_tile_reduce"}.get(dom, dom)
@@ -300,8 +406,7 @@ def main() -> int:
         result["adjoint"] = adjoint_leg()
         print(f"adjoint: {result['adjoint']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        result["cpu_baseline"] = cpu_baseline(scene, args.cpu_steps, threads)
+        result["cpu_baseline"] = cpu_baseline_leg(scene, eng_opts, args.cpu_steps)
     print(json.dumps(result), flush=True)
     return 0
 

@@ -80,11 +80,24 @@ struct CLIOptions {
   uint32_t sync_every = 20;
 
   CLIOptions() = default;
-  // --name value or --name=value; unknown flags and bad values print a
-  // message and exit(1), as cxxopts' failure does in options.h:48-51.
+  // --name value or --name=value; the grid size also in the reference's
+  // short form -N value (cxxopts 2.2 gives the one-letter option name "N",
+  // options.h:26, only a short flag), and -Nvalue. Unknown flags and bad
+  // values print a message and exit(1), as cxxopts' failure does in
+  // options.h:48-51.
   CLIOptions(int argc, char* argv[]) {
     for (int i = 1; i < argc; ++i) {
       std::string a = argv[i], val;
+      if (a.size() >= 2 && a[0] == '-' && a[1] == 'N') {  // short flag
+        if (a.size() > 2) {
+          val = a.substr(2);
+        } else {
+          if (i + 1 >= argc) die("option 'N' needs a value");
+          val = argv[++i];
+        }
+        N = (uint32_t)to_i("N", val, 1);
+        continue;
+      }
       if (a.rfind("--", 0) != 0) die("unexpected argument '" + a + "'");
       a = a.substr(2);
       const size_t eq = a.find('=');
@@ -128,7 +141,7 @@ struct CLIOptions {
     std::printf(
         "MPM simulation.\n"
         "  --dt arg                    Physics simulation time-step (default 1e-4)\n"
-        "  --N arg                     Grid dimensions (default 60)\n"
+        "  -N arg                      Grid dimensions (default 60; also --N)\n"
         "  --save-dir arg              Where to save particles (default none)\n"
         "  --scene arg                 Scene file to load (default scenes/snowman.toml)\n"
         "  --particle-count arg        Particles per unit volume (default 500000)\n"

@@ -51,6 +51,8 @@ void make_dir(const std::string& d) { ::mkdir(d.c_str(), 0755); }
 int main(int argc, char* argv[]) {
   mpm::CLIOptions flags(argc, argv);
   std::printf("Loading scene file: %s\n", flags.scene.c_str());
+  std::printf("grid: N=%u, dt=%g, %u particles per unit volume\n", flags.N, (double)flags.dt,
+              flags.particle_count);
   mpm_scene sc = nullptr;
   if (mpm_scene_load(flags.scene.c_str(), flags.particle_count, flags.seed, &sc) != MPM_OK) {
     std::fprintf(stderr, "%s\n", mpm_scene_last_error());

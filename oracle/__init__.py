@@ -97,10 +97,11 @@ def advance(params, materials, state, steps: int, dtype=np.float32, threads: int
     return (s, grid) if return_grid else s
 
 
-def advance_lame(params, material, state, mu, lam, steps: int, dtype=np.float32):
+def advance_lame(params, material, state, mu, lam, steps: int, dtype=np.float32, threads: int = 1):
     """`steps` substeps with every particle of `material` but its mu0, lambda0
     replaced per particle by mu[i], lam[i] (the differentiable step's
-    forward: fixed-corotated with compat FIXED). Returns the new state dict."""
+    forward: fixed-corotated with compat FIXED). Returns the new state dict.
+    threads > 1 scatters with OpenMP atomics (summation order varies)."""
     s = _state_arrays(state, dtype)
     n = s["x"].shape[0]
     ct = C.c_float if dtype == np.float32 else C.c_double
@@ -110,7 +111,7 @@ def advance_lame(params, material, state, mu, lam, steps: int, dtype=np.float32)
     cp = _conv_params(params)
     cm = _conv_mats([material])
     rc = fn(C.byref(cp), cm, C.c_int64(n), _ptr(s["x"], ct), _ptr(s["v"], ct), _ptr(s["C"], ct),
-            _ptr(s["F"], ct), _ptr(s["Jp"], ct), _ptr(lame, ct), int(steps))
+            _ptr(s["F"], ct), _ptr(s["Jp"], ct), _ptr(lame, ct), int(steps), int(threads))
     if rc != 0:
         raise RuntimeError("oracle advance_lame rejected its arguments")
     return s
@@ -192,3 +193,16 @@ def time_advance_f32(params, materials, state, steps: int, threads: int) -> floa
 
 def openmp_threads() -> int:
     return lib().oracle_openmp_max_threads()
+
+
+def check_threads() -> int:
+    """Threads for oracle runs inside tests: the OpenMP budget of the host
+    (OMP_NUM_THREADS, else the CPUs this process may use), at most 16."""
+    import os
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return min(16, int(env))
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except (AttributeError, OSError):
+        return max(1, min(16, os.cpu_count() or 1))

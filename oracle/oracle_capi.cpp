@@ -70,7 +70,7 @@ int advance(const mpm_params* prm, const mpm_material* mats, int nmat, int64_t n
 // lambda per particle): the forward of the differentiable step.
 template <class S>
 int advance_lame(const mpm_params* prm, const mpm_material* m, int64_t n, S* x, S* v, S* C, S* F,
-                 S* Jp, const S* lame, int steps) {
+                 S* Jp, const S* lame, int steps, int threads) {
   if (!prm || !m || (n > 0 && (!x || !v || !C || !F || !Jp || !lame)) || prm->N < 4 || prm->N > 1024)
     return 1;
   std::vector<uint8_t> mat((size_t)n, 0);
@@ -78,7 +78,7 @@ int advance_lame(const mpm_params* prm, const mpm_material* m, int64_t n, S* x, 
   load<S>(ps, n, x, v, C, F, Jp, mat.data());
   const StepConsts<S> k = make_consts<S>(*prm);
   std::vector<S> grid;
-  for (int s = 0; s < steps; ++s) step<S>(ps, m, 1, k, grid, 1, lame);
+  for (int s = 0; s < steps; ++s) step<S>(ps, m, 1, k, grid, threads, lame);
   store<S>(ps, x, v, C, F, Jp);
   return 0;
 }
@@ -201,12 +201,12 @@ int oracle_advance_f64(const mpm_params* p, const mpm_material* m, int nm, int64
 }
 
 int oracle_advance_lame_f32(const mpm_params* p, const mpm_material* m, int64_t n, float* x, float* v,
-                            float* C, float* F, float* Jp, const float* lame, int steps) {
-  return advance_lame<float>(p, m, n, x, v, C, F, Jp, lame, steps);
+                            float* C, float* F, float* Jp, const float* lame, int steps, int threads) {
+  return advance_lame<float>(p, m, n, x, v, C, F, Jp, lame, steps, threads);
 }
 int oracle_advance_lame_f64(const mpm_params* p, const mpm_material* m, int64_t n, double* x, double* v,
-                            double* C, double* F, double* Jp, const double* lame, int steps) {
-  return advance_lame<double>(p, m, n, x, v, C, F, Jp, lame, steps);
+                            double* C, double* F, double* Jp, const double* lame, int steps, int threads) {
+  return advance_lame<double>(p, m, n, x, v, C, F, Jp, lame, steps, threads);
 }
 
 // Wall-clock seconds of `steps` FP32 substeps (the CPU baseline leg).

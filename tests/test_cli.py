@@ -75,6 +75,92 @@ def test_help_and_bad_flags():
     assert r.returncode == 1 and "bad number" in r.stdout
 
 
+# The reference README's first example (README.md:14), scene
+# scenes/rubber_duck.toml: its material and objects, with stand-in meshes (the
+# reference's OBJ files are Git-LFS pointers): an icosphere for the duck.
+RUBBER_DUCK = """\
+[[material]]
+name = "rubber"
+density = 200
+E = 1.4e5
+Nu = 0.2
+hardening = 0
+plast_clamp_lower = 0.0
+plast_clamp_higher = 1.0e30
+
+[[object]]
+material = "rubber"
+mesh = "rubber_duck.obj"
+size = 0.4
+position = [0.3, 0.4, 0.3]
+velocity = [0.0, 0.0, 0.0]
+
+[[object]]
+material = "rubber"
+mesh = "cube.obj"
+size = 0.3
+position = [0.35, 0.5, 0.35]
+velocity = [0.0, 0.0, 0.0]
+lifetime_begin = 2.0
+"""
+
+
+@pytest.fixture()
+def rubber_duck(tmp_path):
+    (tmp_path / "meshes").mkdir()
+    V, F = icosphere(2)
+    write_obj(tmp_path / "meshes" / "rubber_duck.obj", V, F)
+    write_obj(tmp_path / "meshes" / "cube.obj", CUBE_V, CUBE_F)
+    p = tmp_path / "rubber_duck.toml"
+    p.write_text(RUBBER_DUCK)
+    return p
+
+
+def test_reference_short_flag_forms(rubber_duck):
+    """cxxopts 2.2 gives the one-letter option "N" only its short form: the
+    reference's documented runs pass `-N 16` (README.md:14-16). The program
+    takes it, -N16 and --N alike; a missing or bad value fails."""
+    for form in (["-N", "16"], ["-N16"], ["--N", "16"], ["--N=16"]):
+        r = run("--scene", rubber_duck, *form, "--particle-count", 10000, "--steps", 1, "--no-mesh")
+        assert "grid: N=16," in r.stdout, (form, r.stdout, r.stderr)
+    r = run("--scene", rubber_duck, "-N")
+    assert r.returncode == 1 and "needs a value" in r.stdout
+    r = run("--scene", rubber_duck, "-N", "abc")
+    assert r.returncode == 1 and "bad integer" in r.stdout
+
+
+@pytest.mark.gpu
+def test_reference_readme_command_line(rubber_duck, tmp_path, gpu_lib):
+    """README.md:14's run, `--scene scenes/rubber_duck.toml -N 16
+    --particle-count 10000`, for 12 substeps with frames saved: the program's
+    last frame equals the same scene advanced through the Python binding on a
+    16^3 grid (the second object starts at t = 2 s, outside the run)."""
+    from mpm_amd import Engine, Params
+    from mpm_amd.scene_file import load_scene
+
+    out = tmp_path / "out"
+    steps, dt = 12, 1e-4
+    r = run("--scene", rubber_duck, "-N", 16, "--particle-count", 10000, "--steps", steps,
+            "--save-dir", out, "--no-mesh")
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "grid: N=16," in r.stdout
+    frames = sorted((out / "particles").glob("particles_*.ply"))
+    assert len(frames) == 1  # 240 frames per second: substep 0 only
+    sc = load_scene(str(rubber_duck), 10000, seed=1)
+    a = sc.objects[0]
+    eng = Engine(Params(dt=dt, N=16, gravity=-9.81))
+    eng.set_materials(sc.materials)
+    sl = slice(a.first, a.first + a.count)
+    eng.add_particles(sc.x[sl], sc.material[sl], sc.v[sl])
+    eng.upload()
+    eng.advance(1)
+    ref = eng.download()
+    eng.close()
+    x0, v0, _ = read_ply(frames[0])
+    assert x0.shape[0] == a.count
+    assert np.array_equal(x0, ref["x"]) and np.array_equal(v0, ref["v"])
+
+
 def test_missing_scene_fails_cleanly(tmp_path):
     r = run("--scene", tmp_path / "none.toml", "--steps", "1")
     assert r.returncode == 1 and "cannot open" in r.stderr
@@ -95,7 +181,7 @@ def test_lifetime_activation_matches_python_binding(scene, tmp_path, gpu_lib):
 
     out = tmp_path / "out"
     pc, dt, N, steps = 60000, 1e-3, 32, 30
-    r = run("--scene", scene, "--particle-count", pc, "--dt", dt, "--N", N, "--steps", steps,
+    r = run("--scene", scene, "--particle-count", pc, "--dt", dt, "-N", N, "--steps", steps,
             "--save-dir", out, "--sync-every", 20, "--mesh-grid", 64, "--mesh-particle-radius", 2)
     assert r.returncode == 0, r.stdout + r.stderr
     frames = sorted((out / "particles").glob("particles_*.ply"))

@@ -139,3 +139,80 @@ def test_batch_of_scenes_with_other_radix_digits(gpu_lib):
             assert np.array_equal(o[f], r[f]), f
     for e in engines:
         e.close()
+
+
+def test_full_batch_equals_individual_runs(gpu_lib):
+    """The batched workload at full size (64 scenes, 3.35 M particles, N in
+    {32, 48, 64}) for 48 substeps -- 16 per scene, then one recorded cycle of
+    32 as multi-scene launches -- equals every scene's own run bit for bit."""
+    scs = scenes.batched_scenes()
+    assert len(scs) == 64
+    k = 48
+    engines = [make(sc) for sc in scs]
+    b = Batch(engines)
+    b.advance(k)
+    outs = [e.download() for e in engines]
+    fused = b.get_option("fused_captures")
+    b.close()
+    for e in engines:
+        e.close()
+    assert fused == 1
+    for j, sc in enumerate(scs):
+        e = make(sc)
+        e.advance(k)
+        r = e.download()
+        e.close()
+        for f in ("x", "v", "C", "F", "Jp"):
+            assert np.array_equal(outs[j][f], r[f]), (sc.name, f)
+
+
+def pick_oracle_scenes(scs, count=8):
+    """Scenes covering every material kind and every grid size, smallest
+    first within each (kind, N) pair, then the smallest others."""
+    by = {}
+    for j, sc in enumerate(scs):
+        kind = int(sc.name.split("_k")[1].split("_")[0])
+        by.setdefault((kind, sc.params.N), []).append(j)
+    chosen = [min(js, key=lambda j: scs[j].n) for js in by.values()]
+    rest = sorted((j for j in range(len(scs)) if j not in chosen), key=lambda j: scs[j].n)
+    return sorted(chosen + rest[:max(0, count - len(chosen))])
+
+
+def test_full_batch_against_oracle(gpu_lib):
+    """The multi-scene kernels (grid update, cell sort, fused G2P/P2G of every
+    scene in one launch each) against the oracle: the 64-scene workload with
+    a sort every 2 substeps runs 2 substeps per scene, then one recorded
+    cycle of 4 substeps (two multi-scene sorts); 8 scenes covering elastic,
+    snow and liquid at N = 32, 48 and 64 are checked against the FP32 and FP64
+    oracle after those 6 substeps (elastic: 1e-4; snow: 1e-3 and liquid: 1e-4,
+    or the FP32 oracle's own distance from FP64)."""
+    import oracle
+    from test_gpu_parity import compare_with_oracle
+
+    scs = scenes.batched_scenes()
+    k = 6
+    engines = []
+    for sc in scs:
+        e = make(sc)
+        e.set_option("sort_every", 2)
+        engines.append(e)
+    b = Batch(engines)
+    b.advance(k)
+    fused = b.get_option("fused_captures")
+    launches = b.get_option("graph_launches")
+    outs = [e.download() for e in engines]
+    b.close()
+    for e in engines:
+        e.close()
+    assert fused == 1 and launches == 1
+    picked = pick_oracle_scenes(scs)
+    kinds = {int(scs[j].name.split("_k")[1].split("_")[0]) for j in picked}
+    Ns = {scs[j].params.N for j in picked}
+    assert kinds == {0, 1, 2} and Ns == {32, 48, 64}, (kinds, Ns)
+    threads = oracle.check_threads()
+    for j in picked:
+        sc = scs[j]
+        kind = int(sc.name.split("_k")[1].split("_")[0])
+        tol = {0: 1e-4, 1: 1e-3, 2: 1e-4}[kind]
+        compare_with_oracle(sc, sc.state(), outs[j], k, tol, fp32_noise=kind != 0, threads=threads,
+                            label=f"{sc.name} ({sc.n} particles) in the batch")

@@ -13,6 +13,8 @@ through mpm_amd.diff).
     unit log E directions;
   * the checkpoint interval changes nothing beyond FP32 atomic-order noise.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -159,3 +161,74 @@ def test_checkpoint_interval_does_not_change_gradients(gpu_lib):
     for gv, gl in ((gv_b, gl_b), (gv_c, gl_c)):
         assert np.abs(gv - gv_a).max() <= 1e-4 * np.abs(gv_a).max()
         assert np.abs(gl - gl_a).max() <= 1e-3 * np.abs(gl_a).max()
+
+
+def test_full_cube_forward_and_gradients(gpu_lib):
+    """The differentiable cube as specified: 32,768 particles on a jittered
+    32^3 lattice in [0.35, 0.65]^3, N=32, per-particle log E = log 1e4 + 0.1
+    N(0,1), 200 substeps with a checkpoint every 20, loss |com(x_200) -
+    (0.6, 0.4, 0.5)|^2.
+      * forward: the final state against the FP32 oracle (1e-4 per field);
+      * dL/dv0: against FP64 central differences of the FP64 oracle (eps =
+        1e-3), every component within 1e-3 of the gradient's norm;
+      * dL/dlogE: the cube falls freely for the whole rollout, touching no
+        wall, and the step conserves momentum, so the centre of mass -- and
+        the loss -- does not depend on the stiffness: the gradient must
+        vanish. A unit log E perturbation along the gradient changes the
+        loss by |g|, which must stay below 1e-4 of the loss, and the FP64
+        central difference along it is as small."""
+    from mpm_amd.diff import lame_from_logE, pack_state, rollout, unpack_state
+    c = scenes.diff_cube()
+    n = c.x.shape[0]
+    assert n == 32768 and c.params.N == 32 and c.substeps == 200
+    threads = oracle.check_threads()
+    v0 = np.zeros(3)
+    dev = torch.device("cuda")
+    v0_t = torch.zeros(3, dtype=torch.float32, device=dev, requires_grad=True)
+    logE = torch.from_numpy(c.logE).to(dev).requires_grad_(True)
+    mu, lam = lame_from_logE(logE, c.nu)
+    st = rollout(make_stepper(c), pack_state(torch.from_numpy(c.x).to(dev), v0_t.expand(n, 3)),
+                 mu, lam, c.substeps, 20)
+    com = st[0:3].mean(1)
+    target = torch.tensor(c.target, dtype=torch.float32, device=dev)
+    loss = ((com - target) ** 2).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    gx, gv, gC, gF = (t.detach().cpu().numpy() for t in unpack_state(st))
+    got = {"x": gx, "v": gv, "C": gC, "F": gF}
+    g_v0 = v0_t.grad.cpu().numpy().astype(np.float64)
+    g_logE = logE.grad.cpu().numpy().astype(np.float64)
+
+    mu32 = mu.detach().cpu().numpy()
+    lam32 = lam.detach().cpu().numpy()
+    ref32 = oracle.advance_lame(c.params, oracle_material(c), oracle_state(c.x, v0), mu32, lam32,
+                                c.substeps, dtype=np.float32, threads=threads)
+    e32 = state_errors(got, ref32, c.params.N, keys=("x", "v", "F", "C"))
+    print("C5 forward vs FP32 oracle after 200 substeps:", e32)
+    for k, e in e32.items():
+        assert e < 1e-4, (k, e)
+
+    logE64 = c.logE.astype(np.float64)
+
+    def L(v, le):
+        m64, l64 = lame64(le, c.nu)
+        out = oracle.advance_lame(c.params, oracle_material(c), oracle_state(c.x, v), m64, l64,
+                                  c.substeps, dtype=np.float64, threads=threads)
+        return float(((out["x"].mean(0) - np.asarray(c.target)) ** 2).sum())
+
+    l64 = L(v0, logE64)
+    print("C5 loss gpu", float(loss.detach()), "fp64", l64)
+    assert abs(float(loss.detach()) - l64) < 1e-4 * l64
+    eps = 1e-3
+    fd_v0 = np.array([(L(v0 + eps * e, logE64) - L(v0 - eps * e, logE64)) / (2 * eps)
+                      for e in np.eye(3)])
+    print(f"C5 dL/dv0 gpu {g_v0} fd {fd_v0}")
+    # relative to the gradient's norm: the z component is ~0 (the target's z
+    # is the cube's centre), so a per-component ratio would measure noise
+    assert np.abs(g_v0 - fd_v0).max() <= 1e-3 * np.linalg.norm(fd_v0), (g_v0, fd_v0)
+    gnorm = float(np.linalg.norm(g_logE))
+    u = g_logE / gnorm if gnorm > 0 else np.ones_like(g_logE) / math.sqrt(n)
+    fd_u = (L(v0, logE64 + eps * u) - L(v0, logE64 - eps * u)) / (2 * eps)
+    print(f"C5 |dL/dlogE| gpu {gnorm:.3e}, FP64 central difference along it {fd_u:.3e}, loss {l64:.6e}")
+    assert gnorm < 1e-4 * l64
+    assert abs(fd_u) < 1e-4 * l64

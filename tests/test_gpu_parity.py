@@ -21,7 +21,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import state_errors
+from conftest import state_elem_errors, state_errors
 from mpm_amd import (COMPAT_FIXED, COMPAT_REFERENCE, MATERIAL_FIXED_COROTATED, MATERIAL_JELLY,
                      MATERIAL_SNOW, MATERIAL_WC_LIQUID, Engine, Params, device_svd, scenes)
 
@@ -143,8 +143,6 @@ def test_cell_sort_bit_exact(gpu_lib, radix_bits, which, block):
     if which == "snow_small":
         sc = scenes.snow_spheres(n=30_000, N=64)
     elif which == "dam_break_1M":
-        if radix_bits != 8:
-            pytest.skip("full size with 8-bit digits only")
         sc = scenes.dam_break()
     else:
         sc = scenes.small_scene(MATERIAL_SNOW, 5000, 21, N=16)
@@ -208,28 +206,39 @@ def test_p2g_matches_oracle(gpu_lib, name, compat, opts):
 
 
 # ---------------------------------------------------------------- full substeps
-def step_case(sc, steps, opts, tol, state=None, fp32_noise=False):
-    """Run `steps` substeps on the engine and on the oracle (FP32, FP64).
-    Strict mode: every field within `tol` of the FP32 oracle. fp32_noise mode
-    (liquid, snow): a field may instead pass if its error against FP64 is
-    within twice the FP32 oracle's own error against FP64 -- the engine is
-    then as accurate as the reference's FP32 arithmetic allows."""
-    st = sc.state() if state is None else state
-    out = run_engine(sc, steps, st, **OPTION_SETS[opts])
-    ref32 = oracle.advance(sc.params, sc.materials, st, steps, np.float32)
-    ref64 = oracle.advance(sc.params, sc.materials, st, steps, np.float64)
+def compare_with_oracle(sc, st, out, steps, tol, fp32_noise=False, threads=1, label=""):
+    """`out` (the engine's state after `steps` substeps from `st`) against
+    the FP32 and FP64 oracle. Strict mode: every field within `tol` of the
+    FP32 oracle (norm-relative). fp32_noise mode (liquid, snow): a field may
+    instead pass if its error against FP64 is within twice the FP32 oracle's
+    own error against FP64 -- the engine is then as accurate as the
+    reference's FP32 arithmetic allows. The element-wise relative errors are
+    printed beside the norm-relative ones."""
+    ref32 = oracle.advance(sc.params, sc.materials, st, steps, np.float32, threads=threads)
+    ref64 = oracle.advance(sc.params, sc.materials, st, steps, np.float64, threads=threads)
     N = sc.params.N
     e32 = state_errors(out, ref32, N)
     e64 = state_errors(out, ref64, N)
     o64 = state_errors(ref32, ref64, N)
-    print(f"{sc.name} k={steps} {opts}: vs FP32 oracle {fmt(e32)} | vs FP64 {fmt(e64)} | "
-          f"FP32 oracle vs FP64 {fmt(o64)}")
+    el32 = state_elem_errors(out, ref32, N)
+    el64 = state_elem_errors(out, ref64, N)
+    print(f"{label or sc.name} k={steps}: vs FP32 oracle {fmt(e32)} | vs FP64 {fmt(e64)} | "
+          f"FP32 oracle vs FP64 {fmt(o64)} | element-wise vs FP32 {fmt(el32)} vs FP64 {fmt(el64)}")
     for k in ("x", "v", "F", "C"):
         ok = e32[k] < tol
         if not ok and fp32_noise:
             ok = e64[k] <= 2.0 * o64[k] and e32[k] < 10 * tol
         assert ok, (k, e32[k], e64[k], o64[k])
     assert np.all(np.isfinite(out["x"]))
+    return e32
+
+
+def step_case(sc, steps, opts, tol, state=None, fp32_noise=False, threads=1):
+    """Run `steps` substeps on the engine (option set `opts`) and check them
+    against the oracle (compare_with_oracle)."""
+    st = sc.state() if state is None else state
+    out = run_engine(sc, steps, st, **OPTION_SETS[opts])
+    compare_with_oracle(sc, st, out, steps, tol, fp32_noise, threads, f"{sc.name} {opts}")
     return out
 
 
@@ -438,3 +447,15 @@ def test_dam_break_full_size_properties(gpu_lib):
     for k in ("x", "v", "F"):
         assert e[k] < 1e-4, (k, e[k])
     assert np.all(np.isfinite(a["x"])) and np.all(np.isfinite(a["F"]))
+
+
+@pytest.mark.parametrize("opts", ["tiled-default", "direct-fused"])
+def test_snow_spheres_full_size_10_substeps(gpu_lib, opts):
+    """The snow workload at its full size: two stacked balls, 500,000
+    particles, N=64 (compat REFERENCE: the original-index guard freezes the
+    237,855 particles past N^3 = 262,144 in G2P), 10 substeps against the FP32
+    and FP64 oracle, with the snow rule (1e-3, or the FP32 oracle's own
+    distance from FP64)."""
+    sc = scenes.snow_spheres()
+    assert sc.n == 500_000 and sc.params.N == 64
+    step_case(sc, 10, opts, 1e-3, fp32_noise=True, threads=oracle.check_threads())
