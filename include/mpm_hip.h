@@ -356,6 +356,18 @@ int mpm_diff_sizes(const mpm_params* params, int64_t* grid_floats, int64_t* tape
 int mpm_diff_substep(const mpm_params* params, float particle_mass, float particle_volume,
                      int64_t n, const float* state, float* state_out, const float* mu,
                      const float* lambda, float* grid, float* tape, void* stream);
+/* mpm_diff_substep without the `grid` workspace, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). No counterpart in the reference. The
+ * scatter grid is a static array of the library, one per device, sized for
+ * N <= 64 (4 MiB, held until the process ends); N > 64 returns
+ * MPM_ERR_CAPACITY. Nothing is allocated: the caller passes the inputs and the
+ * outputs (state_out, tape). Every call on a device shares that device's
+ * array, so calls that overlap on one device (concurrent streams, several
+ * threads) race; the array is cleared by a kernel at the start of each call. */
+int mpm_diff_substep_noalloc(const mpm_params* params, float particle_mass, float particle_volume,
+                             int64_t n, const float* state, float* state_out, const float* mu,
+                             const float* lambda, float* tape, void* stream);
 /* Given adj_out = dL/d(state_out) of the substep that read `state` (and wrote
  * `tape`), writes adj_in = dL/d(state) and ADDS dL/dmu, dL/dlambda into
  * adj_mu[n], adj_lambda[n]. adj_in must not alias adj_out. */

@@ -145,6 +145,8 @@ SIGNATURES = {
     # device pointers travel as void*
     "mpm_diff_substep": (C.c_int, [C.POINTER(mpm_params), C.c_float, C.c_float, C.c_int64,
                                    _P, _P, _P, _P, _P, _P, _P]),
+    "mpm_diff_substep_noalloc": (C.c_int, [C.POINTER(mpm_params), C.c_float, C.c_float, C.c_int64,
+                                           _P, _P, _P, _P, _P, _P]),
     "mpm_diff_substep_backward": (C.c_int, [C.POINTER(mpm_params), C.c_float, C.c_float,
                                             C.c_int64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
 }

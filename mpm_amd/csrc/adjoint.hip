@@ -24,6 +24,8 @@
 // four launches per substep.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "device_math.hpp"
 #include "engine_types.hpp"
 #include "kernels.hpp"
@@ -602,7 +604,52 @@ DiffK diff_consts(const StepConsts& k, float mass, float volume, int64_t n) {
 
 }  // namespace
 
+// ---------------------------------------------------------------- no-allocation forward
+// The scatter grid of the forward (4 floats per node) for callers that cannot
+// allocate: one array per device, sized for grids up to kNoAllocMaxN^3 nodes,
+// held until the process ends. Host code never uses the array's name (a
+// host-side stand-in, not a device address): the device address comes from
+// hipGetSymbolAddress, once per device with that device current.
+static __device__ __align__(256) float4 scratch[(size_t)mpmk::kNoAllocMaxN * mpmk::kNoAllocMaxN *
+                                                mpmk::kNoAllocMaxN];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float4* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of the current device's copy of `scratch`.
+hipError_t scratch_address(float4** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[dev] = static_cast<float4*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+}  // namespace
+
 namespace mpmk {
+
+hipError_t diff_forward_noalloc(const StepConsts& k, float mass, float volume, int64_t n,
+                                const float* st, float* out, const float* mu, const float* lam,
+                                float* tape, hipStream_t s) {
+  if (k.N < 1 || k.N > kNoAllocMaxN) return hipErrorInvalidValue;
+  float4* grid = nullptr;
+  const hipError_t e = scratch_address(&grid);
+  if (e != hipSuccess) return e;
+  // diff_forward clears the first 4 N^3 floats with a kernel before the
+  // scatter, whatever the last call left there
+  return diff_forward(k, mass, volume, n, st, out, mu, lam, grid, tape, s);
+}
 
 hipError_t diff_forward(const StepConsts& k, float mass, float volume, int64_t n, const float* st,
                         float* out, const float* mu, const float* lam, float4* grid, float* tape,

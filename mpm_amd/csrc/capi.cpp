@@ -393,6 +393,35 @@ extern "C" int mpm_diff_substep(const mpm_params* params, float particle_mass, f
   return MPM_OK;
 }
 
+extern "C" int mpm_diff_substep_noalloc(const mpm_params* params, float particle_mass,
+                                        float particle_volume, int64_t n, const float* state,
+                                        float* state_out, const float* mu, const float* lambda,
+                                        float* tape, void* stream) {
+  StepConsts k;
+  int rc = diff_check(__func__, params, particle_mass, particle_volume, n, &k);
+  if (rc) return rc;
+  if (params->N > (uint32_t)mpmk::kNoAllocMaxN) {
+    set_global_error("mpm_diff_substep_noalloc: N = %u exceeds the static scratch grid (N <= %d)",
+                     params->N, mpmk::kNoAllocMaxN);
+    return MPM_ERR_CAPACITY;
+  }
+  if (!tape || (n > 0 && (!state || !state_out || !mu || !lambda))) {
+    set_global_error("mpm_diff_substep_noalloc: null buffer");
+    return MPM_ERR_INVALID;
+  }
+  if (n > 0 && state == state_out) {
+    set_global_error("mpm_diff_substep_noalloc: state_out must not alias state");
+    return MPM_ERR_INVALID;
+  }
+  const hipError_t e = mpmk::diff_forward_noalloc(k, particle_mass, particle_volume, n, state,
+                                                  state_out, mu, lambda, tape, (hipStream_t)stream);
+  if (e != hipSuccess) {
+    set_global_error("mpm_diff_substep_noalloc: %s", hipGetErrorString(e));
+    return MPM_ERR_HIP;
+  }
+  return MPM_OK;
+}
+
 extern "C" int mpm_diff_substep_backward(const mpm_params* params, float particle_mass,
                                          float particle_volume, int64_t n, const float* state,
                                          const float* mu, const float* lambda, const float* tape,

@@ -213,6 +213,13 @@ hipError_t mesh_emit(const uint32_t* sdf, int side, double rv, double vdx, const
 hipError_t diff_forward(const StepConsts& k, float mass, float volume, int64_t n, const float* st,
                         float* out, const float* mu, const float* lam, float4* grid, float* tape,
                         hipStream_t s);
+// diff_forward with the scatter grid in a static device array of the
+// library (kNoAllocMaxN^3 nodes per device) instead of a caller's buffer: no
+// allocation. Calls that overlap on one device race on the array.
+constexpr int kNoAllocMaxN = 64;
+hipError_t diff_forward_noalloc(const StepConsts& k, float mass, float volume, int64_t n,
+                                const float* st, float* out, const float* mu, const float* lam,
+                                float* tape, hipStream_t s);
 hipError_t diff_backward(const StepConsts& k, float mass, float volume, int64_t n, const float* st,
                          const float* mu, const float* lam, const float* tape, const float* adj_out,
                          float* adj_in, float* adj_mu, float* adj_lam, float4* grid_adj,
