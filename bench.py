@@ -12,10 +12,13 @@ dam break's length. Prints one JSON line:
   value        particle-substeps/s (P / seconds per substep), whole job
   roofline     the dominant kernel (largest share of a substep's time, the
                sort amortised over its interval): its algorithmic bytes per
-               launch divided by its mean launch time (HIP events recorded by
-               the engine on the launch stream during the timed region, at
-               least 8 timed launches), against 8 TB/s HBM; `traffic` is the
-               rocprofv3-counted HBM bytes per launch of that kernel
+               launch divided by its mean launch time, against 8 TB/s HBM.
+               Launch times are HIP events the engine records on its launch
+               stream around one substep in every sort interval of the timed
+               region; when that gives fewer than 8 launches (a short run),
+               the substeps right after the region are timed one by one to
+               make up 8. `traffic` is the rocprofv3-counted HBM bytes per
+               launch of that kernel (profiles/traffic_<workload>.json)
   cpu_baseline the FP32 CPU oracle (a C++/OpenMP restatement of the
                reference step) on a bounded sample of the same workload, on
                the OpenMP thread budget and on one thread, plus the 10,000-
@@ -185,9 +188,10 @@ def cpu_baseline_leg(scene, options: dict, cpu_steps: int):
 def batched_leg(warm: int, steps: int):
     """The batched workload (64 independent scenes, mpm_amd.scenes.batched_scenes):
     seconds for `steps` substeps of every scene advanced one scene after the
-    other (each handle's own mpm_advance), against the same substeps through
-    one batch (HIP graph replay of multi-scene launches: one per phase for all
-    scenes). Both sets are warmed by `warm` substeps."""
+    other (each handle's own mpm_advance, with the settings its own tuner
+    picks), against the same substeps through one batch (HIP graph replay of
+    multi-scene launches: one per phase for all scenes) under the settings
+    the batch tuner picks. Both are warmed by `warm` substeps after tuning."""
     import torch
     from mpm_amd import Batch, Engine, scenes
 
@@ -202,6 +206,7 @@ def batched_leg(warm: int, steps: int):
     scs = scenes.batched_scenes()
     seq = [make(sc) for sc in scs]
     for e in seq:
+        e.autotune(trial_substeps=64)
         e.advance(warm)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -213,6 +218,7 @@ def batched_leg(warm: int, steps: int):
         e.close()
     bat = [make(sc) for sc in scs]
     b = Batch(bat)
+    tuned = b.autotune(trial_substeps=64)
     b.advance(warm)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -236,6 +242,8 @@ def batched_leg(warm: int, steps: int):
         "algorithmic_bytes_per_substep": b_alg,
         "batched_fraction_of_hbm_roofline": b_alg * steps / t_bat / (HBM_PEAK_GBS * 1e9),
         "sequential_fraction_of_hbm_roofline": b_alg * steps / t_seq / (HBM_PEAK_GBS * 1e9),
+        "batch_tuner_choice": tuned["choice"],
+        "batch_tuner_trials": tuned["trials"],
         "graph_launches": launches,
         "multi_scene_cycles_recorded": fused,
     }
@@ -274,6 +282,7 @@ def adjoint_leg(repeats: int = 3):
         if r > 0 and (best is None or (t2 - t1) / (t1 - t0) < best[1] / best[0]):
             best = (t1 - t0, t2 - t1)
     fwd, bwd = best
+    eng_us = engine_cube_forward_us(c)
     return {
         "workload": f"differentiable elastic cube, {n} particles, N={c.params.N}, "
                     f"{c.substeps} substeps, checkpoint every 20",
@@ -281,7 +290,34 @@ def adjoint_leg(repeats: int = 3):
         "forward_us_per_substep": fwd * 1e6 / c.substeps,
         "backward_us_per_substep": bwd * 1e6 / c.substeps,
         "bwd_fwd_ratio": bwd / fwd,
+        "engine_forward_us_per_substep": eng_us,
+        "bwd_over_engine_forward": bwd * 1e6 / c.substeps / eng_us,
     }
+
+
+def engine_cube_forward_us(c, steps: int = 400) -> float:
+    """Microseconds per substep of the same cube through the engine's plain
+    tiled step (mpm_advance; the fixed-corotated model with J = det F, compat
+    FIXED, one E = 1e4 for every particle, since the engine's materials are
+    per kind, not per particle), after 48 warm-up substeps."""
+    import numpy as np
+    import torch
+    from mpm_amd import MATERIAL_FIXED_COROTATED, Engine, material_as_numpy
+
+    mat = material_as_numpy(MATERIAL_FIXED_COROTATED, c.particle_volume, 1000.0, E=1e4, nu=c.nu)
+    mat.particle_mass = c.particle_mass
+    e = Engine(c.params)
+    e.set_materials([mat])
+    e.add_particles(c.x, np.zeros(c.x.shape[0], np.uint8))
+    e.upload()
+    e.advance(48)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.advance(steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    e.close()
+    return (t1 - t0) * 1e6 / steps
 
 
 def main() -> int:
@@ -328,23 +364,32 @@ def main() -> int:
     torch.cuda.synchronize()
 
     # timed region: events on the launch stream bracket the kernel phases of
-    # one substep in every k, rotating through the k positions of a k-substep
-    # cycle; k is the sort interval, or less on a short run so that at least
-    # MIN_TIMED_LAUNCHES substeps are timed
+    # one substep in every k (k = the sort interval), rotating through the k
+    # positions of a k-substep cycle; timing more substeps would slow the
+    # wall clock (each event pair waits for its kernel to drain)
     sort_every = eng.get_option("sort_every") or TIMING_EVERY
-    eng.set_option("timing", max(1, min(sort_every, args.steps // MIN_TIMED_LAUNCHES)))
+    eng.set_option("timing", sort_every)
     eng.timing_reset()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     eng.advance(args.steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    in_region = {ph: eng.timing(ph)[1] for ph in ("g2p2g", "grid_update")}
+    # a short run times few launches: the substeps right after the region,
+    # every one timed, bring each per-substep phase to MIN_TIMED_LAUNCHES
+    top_up = max(0, MIN_TIMED_LAUNCHES - min(in_region.values()))
+    if top_up:
+        eng.set_option("timing", 1)
+        eng.advance(top_up)
+        torch.cuda.synchronize()
     eng.set_option("timing", 0)
     phases = {}
     for ph in ("g2p2g", "grid_update", "sort", "p2g", "g2p", "clear"):
         ms, n = eng.timing(ph)
         if n:
             phases[ph] = {"mean_us": 1e3 * ms / n, "launches": n}
+    kernel_timing = {"timed_in_region": in_region, "timed_after_region": top_up}
     eng_opts = {k: eng.get_option(k) for k in ("sort_every", "chunk", "radix_bits")}
     out = eng.download()
     eng.close()
@@ -378,6 +423,7 @@ def main() -> int:
                          "snow_spheres": "snow E=1.4e5 nu=0.2 rho=400 xi=10",
                          "elastic_sphere": "elastic (snow model, clamps [0,1e30]) E=1e4 nu=0.3"}[args.workload],
             "substep_kernels": phases,
+            "kernel_timing": kernel_timing,
             "options": {"sort_every": eng_opts["sort_every"], "chunk": eng_opts["chunk"],
                         "radix_bits": eng_opts["radix_bits"], "autotuned": tuned is not None},
             "algorithmic_bytes_per_substep": scenes.algorithmic_bytes(P, N),

@@ -62,6 +62,10 @@ extern "C" {
 #define MPM_BOUNDARY_FRACTION 0 /* walls where node_index/N < b or > 1-b (reference, b=0.05) */
 #define MPM_BOUNDARY_CELLS 1    /* walls on the first/last b node layers of each axis */
 
+/* interpolation kernels (option "interpolation") */
+#define MPM_INTERP_QUADRATIC 0 /* QuadraticInterpolationKernel (reference) */
+#define MPM_INTERP_CUBIC 1     /* cubic B-spline, generic D^-1 */
+
 /* compatibility with result-changing details of the reference step */
 #define MPM_COMPAT_REFERENCE 0 /* reproduce them (the default for parity) */
 #define MPM_COMPAT_FIXED 1     /* every particle updated, P2G clipped, J = det F */
@@ -141,6 +145,16 @@ int mpm_upload(mpm_handle h, void* stream);
  * reference's host round trip when an object's lifetime begins
  * (Simulation::particlesToHost + reallocateParticles, mpm.cu:288-314). */
 int mpm_append_particles(mpm_handle h, void* stream);
+/* Remove the device particles whose original index (the order downloads
+ * list them in) lies in [first, first + count): they leave the simulation,
+ * the others stay on the device in their order, and the indices above the
+ * range move down by count (downloads list the remaining particles as before,
+ * without the removed ones). With `removed` non-NULL it first receives the
+ * removed particles' last state in the reference's 104-byte layout, count
+ * entries in index order. Blocking. Replaces the reference's host round trip
+ * when an object's lifetime ends (SimObject::isActive, mpm.cuh:38-40;
+ * Simulation::particlesToHost + reallocateParticles, mpm.cu:288-314). */
+int mpm_remove_particles(mpm_handle h, int64_t first, int64_t count, void* removed, void* stream);
 /* Number of particles on the device (Simulation::getFullParticleCount,
  * mpm.cu:253-259, for the uploaded set). */
 int mpm_count(mpm_handle h, int64_t* out);
@@ -194,6 +208,14 @@ int mpm_load_state(mpm_handle h, const char* path);
  *   "p2g_mode"   1 (default): cell-sorted particles, LDS-tiled fused kernel,
  *                tile stores reduced per node block; 0: global FP32 atomics
  *                per node in particle order (the reference's scheme)
+ *   "interpolation" the InterpolationKernel of the MLS-APIC TransferScheme
+ *                (include/InterpolationKernel.cuh:12-74, bound at
+ *                include/mpm.cuh:24-27): MPM_INTERP_QUADRATIC (default, the
+ *                reference's, constant D^-1 = 4/dx^2 I) or MPM_INTERP_CUBIC (a
+ *                cubic B-spline over 4 nodes per axis whose D^-1 is the generic
+ *                per-particle inverse, InterpolationKernel.cuh:21-50; no
+ *                reference counterpart). The tiles of p2g_mode 1 fit the
+ *                quadratic stencil: the cubic kernel runs the direct kernels
  *   "fused"      1 (default): G2P of substep n fused with P2G of substep n+1;
  *                0: the reference's order P2G, grid update, G2P
  *   "sort_every" re-sort the particles by cell every k substeps (default 16)
@@ -298,6 +320,17 @@ int mpm_batch_advance(mpm_batch b, int32_t substeps, void* stream);
  * "fused_captures". */
 int mpm_batch_set_option(mpm_batch b, const char* name, int64_t value);
 int mpm_batch_get_option(mpm_batch b, const char* name, int64_t* value);
+/* Batch tuner: times candidate settings through the batch itself (its graph
+ * capture and multi-scene launches included), each from the same snapshot of
+ * every scene, and keeps the fastest: every scene's own mpm_autotune choice
+ * (sort interval and work-item size; the batch sorts with 8-bit digits), or
+ * one sort interval (16..64) and work-item size (192, 128) for all scenes.
+ * Scenes with different sort intervals share a cycle of their least common
+ * multiple. The scenes' states and times are put back. The report (JSON,
+ * truncated to report_len) lists the trials in microseconds per batch
+ * substep. Blocking. Every scene must be on the tiled path. */
+int mpm_batch_autotune(mpm_batch b, int32_t trial_substeps, char* report, int32_t report_len,
+                       void* stream);
 
 /* ---- scene input (src/main.cu:31-66, Simulation::loadMesh / addParticles
  *      src/mpm.cu:331-394) ----
@@ -376,6 +409,30 @@ int mpm_diff_substep_backward(const mpm_params* params, float particle_mass,
                               const float* mu, const float* lambda, const float* tape,
                               const float* adj_out, float* adj_in, float* adj_mu,
                               float* adj_lambda, float* grid, void* stream);
+
+/* A run of `steps` >= 1 differentiable substeps in one call: the kernels of
+ * mpm_diff_substep, two launches per substep, all enqueued here. From
+ * state0, the state after substep j+1 goes to states + j*state_stride
+ * (state_stride >= 24 n floats; state0 must not lie in that output) and its
+ * grid tape to tapes + j*tape_stride (tape_stride >= 8 N^3 floats, or 0:
+ * every substep writes the same tape, when no backward follows). grid: 4 N^3
+ * floats of scratch. No reference counterpart. */
+int mpm_diff_forward_steps(const mpm_params* params, float particle_mass, float particle_volume,
+                           int64_t n, int32_t steps, const float* state0, float* states,
+                           int64_t state_stride, const float* mu, const float* lambda, float* grid,
+                           float* tapes, int64_t tape_stride, void* stream);
+/* The adjoint of a run of `steps` substeps: states + j*state_stride is the
+ * state before substep j (j = 0..steps-1) and tapes + j*tape_stride its tape,
+ * as mpm_diff_forward_steps wrote them; adj_out is dL/d(state after the last
+ * substep). Writes adj_in = dL/d(state before substep 0) and ADDS dL/dmu,
+ * dL/dlambda into adj_mu[n], adj_lambda[n]. Scratch: grid 8 N^3 floats, work
+ * 24 n floats (adj_in may be work; neither may alias adj_out). Two launches
+ * per substep. No reference counterpart. */
+int mpm_diff_backward_steps(const mpm_params* params, float particle_mass, float particle_volume,
+                            int64_t n, int32_t steps, const float* states, int64_t state_stride,
+                            const float* mu, const float* lambda, const float* tapes,
+                            int64_t tape_stride, const float* adj_out, float* adj_in, float* adj_mu,
+                            float* adj_lambda, float* grid, float* work, void* stream);
 
 #ifdef __cplusplus
 }

@@ -248,16 +248,19 @@ class Simulation {
     upload_active();
   }
 
-  // Simulation::advance (mpm.cu:323-329): one substep, then t += dt
+  // Simulation::advance (mpm.cu:323-329): one substep, then t += dt. Objects
+  // whose lifetime ended leave the device (their last state copied back into
+  // the object), objects whose lifetime began join it behind the others;
+  // the rest of the particles stay on the device (the reference copies every
+  // particle to the host and back, particlesToHost + reallocateParticles,
+  // mpm.cu:288-314).
   void advance() {
     const std::vector<char> now = active_mask(t);
     if (now != uploaded_) {
-      bool ended = false;
-      for (size_t k = 0; k < objects.size(); ++k) ended = ended || (uploaded_[k] && !now[k]);
-      if (ended || active_particle_count == 0) {
-        syncDevice();
+      if (active_particle_count == 0) {
         upload_active();
       } else {
+        remove_ended(now);
         append_started(now);
       }
     }
@@ -314,6 +317,26 @@ class Simulation {
     }
     active_particle_count = n;
     if (n > 0) check(mpm_upload(h_, nullptr), h_, "mpm_upload");
+  }
+  // objects that stopped being active: their particles leave the device, their
+  // last state back into the object
+  void remove_ended(const std::vector<char>& now) {
+    int64_t first = 0;
+    std::vector<size_t> kept;
+    for (size_t k : on_device_) {
+      const int64_t n = (int64_t)objects[k].particles.size();
+      if (now[k]) {
+        kept.push_back(k);
+        first += n;
+        continue;
+      }
+      if (n > 0)
+        check(mpm_remove_particles(h_, first, n, objects[k].particles.data(), nullptr), h_,
+              "mpm_remove_particles");
+      uploaded_[k] = 0;
+      active_particle_count -= n;
+    }
+    on_device_ = kept;
   }
   // objects that became active, in object order, behind the device's particles
   void append_started(const std::vector<char>& now) {

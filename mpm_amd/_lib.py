@@ -39,6 +39,9 @@ BOUNDARY_CELLS = 1
 COMPAT_REFERENCE = 0
 COMPAT_FIXED = 1
 
+INTERP_QUADRATIC = 0
+INTERP_CUBIC = 1
+
 
 class mpm_params(C.Structure):
     _fields_ = [
@@ -103,6 +106,7 @@ SIGNATURES = {
     "mpm_mesh_distance_field": (C.c_int, [_P, C.c_int32, _P]),
     "mpm_mesh_write_obj": (C.c_int, [_P, C.c_char_p]),
     "mpm_count": (C.c_int, [_P, _i64p]),
+    "mpm_remove_particles": (C.c_int, [_P, C.c_int64, C.c_int64, _P, _P]),
     "mpm_advance": (C.c_int, [_P, C.c_int32, _P]),
     "mpm_time": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "mpm_download_soa": (C.c_int, [_P, _f, _f, _f, _f, _f, _u8, _P]),
@@ -128,6 +132,7 @@ SIGNATURES = {
     "mpm_batch_advance": (C.c_int, [_P, C.c_int32, _P]),
     "mpm_batch_set_option": (C.c_int, [_P, C.c_char_p, C.c_int64]),
     "mpm_batch_get_option": (C.c_int, [_P, C.c_char_p, _i64p]),
+    "mpm_batch_autotune": (C.c_int, [_P, C.c_int32, C.c_char_p, C.c_int32, _P]),
     "mpm_scene_load": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
     "mpm_scene_destroy": (C.c_int, [_P]),
     "mpm_scene_last_error": (C.c_char_p, []),
@@ -147,6 +152,11 @@ SIGNATURES = {
                                    _P, _P, _P, _P, _P, _P, _P]),
     "mpm_diff_substep_noalloc": (C.c_int, [C.POINTER(mpm_params), C.c_float, C.c_float, C.c_int64,
                                            _P, _P, _P, _P, _P, _P]),
+    "mpm_diff_forward_steps": (C.c_int, [C.POINTER(mpm_params), C.c_float, C.c_float, C.c_int64,
+                                         C.c_int32, _P, _P, C.c_int64, _P, _P, _P, _P, C.c_int64, _P]),
+    "mpm_diff_backward_steps": (C.c_int, [C.POINTER(mpm_params), C.c_float, C.c_float, C.c_int64,
+                                          C.c_int32, _P, C.c_int64, _P, _P, _P, C.c_int64, _P, _P, _P, _P,
+                                          _P, _P, _P]),
     "mpm_diff_substep_backward": (C.c_int, [C.POINTER(mpm_params), C.c_float, C.c_float,
                                             C.c_int64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
 }

@@ -6,22 +6,37 @@
 // 56-61) with J = det F, per-particle Lame parameters mu_p, lambda_p, no
 // plasticity, and the stencil clipped at the grid faces (N - base). State
 // planes (24 x n floats, plane-major): x(3) v(3) C(9) F(9), 3x3 row-major.
-// The forward records a grid tape per node (velocity and mass after the
-// update, velocity before the walls, wall flags; 8 floats) that the backward
-// replays:
-//   k_adj_g2p   G2P backward: adjoints of x', v', C', F' -> partial x, F
-//               adjoints of the particle, and the grid velocity adjoint
-//               (scattered with FP32 atomics);
-//   k_adj_grid  grid update backward: momentum and mass adjoints per node;
-//   k_adj_p2g   P2G and stress backward: completes the particle adjoints
-//               (x, v, C, F) and accumulates the mu, lambda adjoints.
+// The forward records a grid tape per node that the backward replays: the
+// velocity and mass after the update (N^3 float4, read by G2P), then the
+// velocity before the walls and the wall flags (N^3 float4).
+//
+// Kernels, one particle per lane, one wavefront per workgroup:
+//   k_diff_step<G2P, P2G>   G2P of substep n (gather of the tape velocities,
+//                           F and x update) fused with the P2G of substep n+1
+//                           (stress through the polar rotation of F, scatter)
+//   k_diff_grid             grid update into the tape; clears the grid
+//   k_adj_step<P2G', G2P'>  the adjoint of the P2G of substep s (gather of the
+//                           momentum and mass adjoints, stress adjoint through
+//                           the SVD, mu and lambda adjoints) fused with the
+//                           adjoint of the G2P of substep s-1 (scatter of the
+//                           grid velocity adjoint)
+//   k_adj_grid              grid update backward: momentum and mass adjoints
+// so a substep is two launches forward and two backward, and a run of
+// substeps is enqueued from one host call (diff_forward_steps /
+// diff_backward_steps).
+//
 // The stress adjoint differentiates the polar rotation R of F = U S V^T:
 // with Rbar the adjoint of R and M = V^T skew(R^T Rbar) V,
 // Fbar += 2 U N V^T, N_ij = M_ij / (s_i + s_j).
 //
-// One thread per particle. The two scatters stage their sums in an LDS tile
-// per workgroup (the box tile below); the forward and backward each take
-// four launches per substep.
+// Scatters (P2G: 27 nodes x 4 values; the grid adjoint of G2P: 27 x 3): each
+// lane writes its particle's values into a row of an LDS table; the lanes
+// then take (run, value) pairs, where a run is a stretch of lanes whose
+// particles share a base cell (particles sorted by cell make long runs:
+// ~37 per cell for a dense lattice), sum the run's column in lane order and
+// add the sum to the grid with one FP32 atomic. A dense scene issues ~3 runs
+// x 108 atomics per wavefront instead of 64 x 108; unsorted particles only
+// give shorter runs.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -35,6 +50,12 @@ using namespace mpmdev;
 namespace {
 
 enum : int { DX = 0, DV = 3, DC = 6, DF = 15 };
+
+constexpr int kWave = 64;        // lanes per workgroup (one wavefront)
+constexpr int kP2GVals = 108;    // 27 nodes x (momentum, mass)
+constexpr int kP2GStride = 109;  // odd LDS row stride: a column spreads over the banks
+constexpr int kAdjVals = 81;     // 27 nodes x grid velocity adjoint
+constexpr int kAdjStride = 83;
 
 struct DiffK {
   StepConsts k;
@@ -95,6 +116,20 @@ __device__ __forceinline__ void node_offset(const Stencil& s, int a, int b, int 
   dp[2] = (float)(s.base[2] + c) * k.dx - x[2];
 }
 
+// the base cell of a particle inside the domain (base in [-3, N) per axis)
+// as one non-negative integer
+__device__ __forceinline__ int cell_key(const Stencil& s, int N) {
+  const int M = N + 3;
+  return ((s.base[0] + 3) * M + (s.base[1] + 3)) * M + (s.base[2] + 3);
+}
+
+__device__ __forceinline__ void load_xF(const float* st, int64_t n, int64_t p, float x[3], float F[9]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) x[a] = st[(DX + a) * n + p];
+#pragma unroll
+  for (int a = 0; a < 9; ++a) F[a] = st[(DF + a) * n + p];
+}
+
 __device__ __forceinline__ void load_state(const float* st, int64_t n, int64_t p, float x[3],
                                            float v[3], float C[9], float F[9]) {
 #pragma unroll
@@ -144,153 +179,58 @@ __device__ __forceinline__ float stress_scale(const DiffK& d) {
   return ((-d.k.dinv) * d.k.dt) * d.volume;
 }
 
-// ---------------------------------------------------------------- box tile
-// The scatters (P2G, and the grid adjoint of G2P) stage a workgroup's
-// contributions in LDS over the bounding box of its particles' stencils,
-// then add each touched node to HBM once. Particles that neighbour in memory
-// and in space (lattice or sorted order) share most nodes, so this turns
-// ~1000 contended global atomics per node into a few. When the box exceeds
-// the LDS tile (scattered particle order), the workgroup adds to HBM directly.
-constexpr int kBoxFloats = 8192;  // 32 KiB
-
-struct Box {
-  int lo[3], ext[3];
-  bool lds;  // the tile holds the box
+// ---------------------------------------------------------------- run scatter
+// Runs of lanes with equal keys (key < 0: the lane adds nothing) and their
+// base cells, as the head lanes wrote them.
+struct Runs {
+  int start[kWave + 1];
+  int key[kWave];
+  int base[3 * kWave];
 };
 
-// Every thread of the workgroup calls this (it synchronises).
-__device__ __forceinline__ Box block_box(const Stencil& sn, bool active, int N, int nf, int* sh) {
-  if (threadIdx.x < 6) sh[threadIdx.x] = threadIdx.x < 3 ? 0x7fffffff : -0x7fffffff;
-  __syncthreads();
-  if (active) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      atomicMin(&sh[a], max(sn.base[a], 0));
-      atomicMax(&sh[3 + a], min(sn.base[a] + 2, N - 1));
-    }
+// Adds the LDS table rows (NV values per lane, row stride STRIDE; VPN values
+// per node, stencil node o = 9a + 3b + c at columns VPN o .. VPN o + VPN - 1)
+// to `grid` (4 floats per node): one sum per (run, column), in lane order,
+// and one FP32 atomic per non-zero sum of a node inside the grid. Every lane
+// of the (one-wavefront) workgroup calls it.
+template <int NV, int STRIDE, int VPN>
+__device__ __forceinline__ void run_scatter(const float* table, Runs& runs, int key, const int base[3],
+                                            int N, float* __restrict__ grid) {
+  const int lane = threadIdx.x;
+  const int prev = __shfl_up(key, 1);
+  const bool head = lane == 0 || key != prev;
+  const unsigned long long heads = __ballot(head);
+  const int nruns = __popcll(heads);
+  if (head) {
+    const int r = __popcll(heads & ((1ull << lane) - 1ull));
+    runs.start[r] = lane;
+    runs.key[r] = key;
+    runs.base[3 * r] = base[0];
+    runs.base[3 * r + 1] = base[1];
+    runs.base[3 * r + 2] = base[2];
   }
-  __syncthreads();
-  Box b;
-  long vol = 1;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    b.lo[a] = sh[a];
-    b.ext[a] = sh[3 + a] - sh[a] + 1;
-    vol *= b.ext[a] > 0 ? b.ext[a] : 0;
+  if (lane == 0) runs.start[nruns] = kWave;
+  __syncthreads();  // the table rows and the run list are visible to every lane
+  const int total = nruns * NV;
+  for (int t = lane; t < total; t += kWave) {
+    const int r = t / NV;
+    const int q = t - r * NV;
+    if (runs.key[r] < 0) continue;
+    const int p1 = runs.start[r + 1];
+    float sum = 0.0f;
+    for (int p = runs.start[r]; p < p1; ++p) sum += table[p * STRIDE + q];
+    if (sum == 0.0f) continue;
+    const int o = q / VPN;
+    const int comp = q - o * VPN;
+    const int gi = runs.base[3 * r] + o / 9, gj = runs.base[3 * r + 1] + (o / 3) % 3,
+              gk = runs.base[3 * r + 2] + o % 3;
+    if ((unsigned)gi >= (unsigned)N || (unsigned)gj >= (unsigned)N || (unsigned)gk >= (unsigned)N) continue;
+    unsafeAtomicAdd(grid + 4 * (((size_t)gi * N + (size_t)gj) * N + (size_t)gk) + comp, sum);
   }
-  b.lds = vol > 0 && vol * nf <= kBoxFloats;
-  return b;
-}
-
-__device__ __forceinline__ int box_slot(const Box& b, const Stencil& sn, int a, int c1, int c2) {
-  return ((sn.base[0] + a - b.lo[0]) * b.ext[1] + (sn.base[1] + c1 - b.lo[1])) * b.ext[2] +
-         (sn.base[2] + c2 - b.lo[2]);
-}
-
-__device__ __forceinline__ void box_clear(const Box& b, int nf, float* tile) {
-  const int n = b.ext[0] * b.ext[1] * b.ext[2] * nf;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) tile[i] = 0.0f;
-}
-
-// adds the tile's nonzero entries to the grid (4 floats per node)
-__device__ __forceinline__ void box_flush(const Box& b, int nf, const float* tile, float* grid, int N) {
-  const int n = b.ext[0] * b.ext[1] * b.ext[2];
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const int z = i % b.ext[2], y = (i / b.ext[2]) % b.ext[1], x = i / (b.ext[2] * b.ext[1]);
-    float* g = grid + 4 * (((size_t)(b.lo[0] + x) * N + (size_t)(b.lo[1] + y)) * N + (size_t)(b.lo[2] + z));
-    for (int r = 0; r < nf; ++r) {
-      const float v = tile[nf * i + r];
-      if (v != 0.0f) unsafeAtomicAdd(g + r, v);
-    }
-  }
-}
-
-// ---------------------------------------------------------------- forward
-__global__ void __launch_bounds__(256)
-    k_diff_p2g(const float* __restrict__ st, const float* __restrict__ mu,
-               const float* __restrict__ lam, DiffK d, float* __restrict__ grid) {
-  __shared__ float tile[kBoxFloats];
-  __shared__ int sh[6];
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const StepConsts& k = d.k;
-  float x[3] = {0.f, 0.f, 0.f}, v[3], C[9], F[9];
-  Stencil sn;
-  bool active = p < d.n;
-  if (active) {
-    load_state(st, d.n, p, x, v, C, F);
-    sn = make_stencil(x, k);
-    active = !sn.outside;
-  }
-  const Box box = block_box(sn, active, k.N, 4, sh);
-  if (box.lds) {
-    box_clear(box, 4, tile);
-    __syncthreads();
-  }
-  if (active) {
-    float U[9], s[3], V[9], R[9], tau[9];
-    fc_stress(F, mu[p], lam[p], U, s, V, R, tau);
-    const float sc = stress_scale(d);
-    float A[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) A[i] = sc * tau[i] + d.mass * C[i];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          if (!node_ok(sn, a, b, c, k.N)) continue;
-          const float wt = weight(sn, a, b, c);
-          float dp[3];
-          node_offset(sn, a, b, c, x, k, dp);
-          float q[4];
-#pragma unroll
-          for (int r = 0; r < 3; ++r)
-            q[r] = wt * (d.mass * v[r] + (A[3 * r] * dp[0] + A[3 * r + 1] * dp[1] + A[3 * r + 2] * dp[2]));
-          q[3] = wt * d.mass;
-          float* g = box.lds ? tile + 4 * box_slot(box, sn, a, b, c)
-                             : grid + 4 * node_index(sn, a, b, c, k.N);
-#pragma unroll
-          for (int r = 0; r < 4; ++r) unsafeAtomicAdd(g + r, q[r]);
-        }
-  }
-  if (box.lds) {
-    __syncthreads();
-    box_flush(box, 4, tile, grid, k.N);
-  }
-}
-
-// grid update with tape: tape[8 node + 0..3] = (velocity, mass) after the
-// update, [4..6] = velocity before the walls, [7] = wall flags (1 sticky,
-// 2 floor)
-__global__ void __launch_bounds__(256)
-    k_diff_grid(const float4* __restrict__ grid, float* __restrict__ tape, StepConsts k) {
-  const int N = k.N;
-  const size_t nodes = (size_t)N * N * N;
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nodes) return;
-  const float4 c = grid[t];
-  float out[8] = {0.f, 0.f, 0.f, c.w, 0.f, 0.f, 0.f, 0.f};
-  if (c.w > 0.0f) {
-    const int zi = (int)(t % N), yi = (int)((t / N) % N), xi = (int)(t / ((size_t)N * N));
-    out[4] = c.x / c.w;
-    out[5] = c.y / c.w + k.dt * k.gravity;
-    out[6] = c.z / c.w;
-    const float4 u = grid_node_update(c, xi, yi, zi, k);
-    bool sticky, floor_;
-    wall_flags(xi, yi, zi, k, sticky, floor_);
-    out[0] = u.x;
-    out[1] = u.y;
-    out[2] = u.z;
-    out[7] = (float)((sticky ? 1 : 0) | (floor_ ? 2 : 0));
-  }
-  float4* o = reinterpret_cast<float4*>(tape + 8 * t);
-  o[0] = make_float4(out[0], out[1], out[2], out[3]);
-  o[1] = make_float4(out[4], out[5], out[6], out[7]);
 }
 
 // v' = sum_i w_i v_i and C' = Dinv sum_i w_i v_i (x_i - x_p)^T over the stencil
-__device__ __forceinline__ void g2p_affine(const Stencil& sn, const float x[3], const float* tape,
+__device__ __forceinline__ void g2p_affine(const Stencil& sn, const float x[3], const float4* vel,
                                            const StepConsts& k, float vn[3], float Cn[9]) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) vn[a] = 0.f;
@@ -304,7 +244,7 @@ __device__ __forceinline__ void g2p_affine(const Stencil& sn, const float x[3], 
       for (int c = 0; c < 3; ++c) {
         if (!node_ok(sn, a, b, c, k.N)) continue;
         const float wt = weight(sn, a, b, c);
-        const float4 gv = *reinterpret_cast<const float4*>(tape + 8 * node_index(sn, a, b, c, k.N));
+        const float4 gv = vel[node_index(sn, a, b, c, k.N)];
         float dp[3];
         node_offset(sn, a, b, c, x, k, dp);
         const float g[3] = {gv.x, gv.y, gv.z};
@@ -317,86 +257,158 @@ __device__ __forceinline__ void g2p_affine(const Stencil& sn, const float x[3], 
       }
 }
 
-__global__ void __launch_bounds__(256)
-    k_diff_g2p(const float* __restrict__ st, float* __restrict__ out, const float* __restrict__ tape,
-               DiffK d) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= d.n) return;
+// ---------------------------------------------------------------- forward
+// G2P of substep n from `st` with the tape velocities `vel` into `out`
+// (particles outside the grid are copied), fused with the P2G of substep n+1
+// of the new state into `grid`. Without G2P: the P2G of `st`. Without P2G:
+// the G2P only.
+template <bool G2P, bool P2G>
+__global__ void __launch_bounds__(kWave)
+    k_diff_step(const float* __restrict__ st, float* __restrict__ out, const float4* __restrict__ vel,
+                const float* __restrict__ mu, const float* __restrict__ lam, DiffK d,
+                float* __restrict__ grid) {
+  __shared__ float table[P2G ? kWave * kP2GStride : 1];
+  __shared__ Runs runs;
+  const int lane = threadIdx.x;
+  const int64_t p = (int64_t)blockIdx.x * kWave + lane;
   const StepConsts& k = d.k;
-  float x[3], v[3], C[9], F[9];
-  load_state(st, d.n, p, x, v, C, F);
-  const Stencil sn = make_stencil(x, k);
-  if (!sn.outside) {
-    float vn[3], Cn[9];
-    g2p_affine(sn, x, tape, k, vn, Cn);
-    float A[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) A[i] = ((i % 4 == 0) ? 1.0f : 0.0f) + k.dt * Cn[i];
-    mul_ab(A, F, F);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      v[a] = vn[a];
-      x[a] = x[a] + k.dt * vn[a];
-    }
-#pragma unroll
-    for (int i = 0; i < 9; ++i) C[i] = Cn[i];
+  const bool valid = p < d.n;
+  float x[3] = {0.f, 0.f, 0.f}, v[3] = {0.f, 0.f, 0.f}, C[9], F[9];
+  Stencil sn;
+  sn.base[0] = sn.base[1] = sn.base[2] = 0;
+  bool inside = false;
+  if (valid) {
+    load_state(st, d.n, p, x, v, C, F);
+    sn = make_stencil(x, k);
+    inside = !sn.outside;
   }
-  store_state(out, d.n, p, x, v, C, F);
+  if (G2P) {
+    if (inside) {
+      float vn[3], Cn[9];
+      g2p_affine(sn, x, vel, k, vn, Cn);
+      float A[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) A[i] = ((i % 4 == 0) ? 1.0f : 0.0f) + k.dt * Cn[i];
+      mul_ab(A, F, F);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        v[a] = vn[a];
+        x[a] = x[a] + k.dt * vn[a];
+      }
+#pragma unroll
+      for (int i = 0; i < 9; ++i) C[i] = Cn[i];
+    }
+    if (valid) store_state(out, d.n, p, x, v, C, F);
+    if (P2G && valid) {
+      sn = make_stencil(x, k);
+      inside = !sn.outside;
+    }
+  }
+  if (P2G) {
+    int key = -1;
+    if (inside) {
+      float U[9], s[3], V[9], R[9], tau[9];
+      fc_stress(F, mu[p], lam[p], U, s, V, R, tau);
+      const float sc = stress_scale(d);
+      float A[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) A[i] = sc * tau[i] + d.mass * C[i];
+      float* row = table + lane * kP2GStride;
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const int o = 4 * (9 * a + 3 * b + c);
+            const float wt = weight(sn, a, b, c);
+            float dp[3];
+            node_offset(sn, a, b, c, x, k, dp);
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+              row[o + r] = wt * (d.mass * v[r] + (A[3 * r] * dp[0] + A[3 * r + 1] * dp[1] + A[3 * r + 2] * dp[2]));
+            row[o + 3] = wt * d.mass;
+          }
+      key = cell_key(sn, k.N);
+    }
+    run_scatter<kP2GVals, kP2GStride, 4>(table, runs, key, sn.base, k.N, grid);
+  }
+}
+
+// grid update into the tape: tv = (velocity, mass) after the update, tw =
+// (velocity before the walls, wall flags: 1 sticky, 2 floor); clears the
+// grid for the next scatter
+__global__ void __launch_bounds__(256)
+    k_diff_grid(float4* __restrict__ grid, float4* __restrict__ tv, float4* __restrict__ tw, StepConsts k) {
+  const int N = k.N;
+  const size_t nodes = (size_t)N * N * N;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nodes) return;
+  const float4 c = grid[t];
+  grid[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 o0 = make_float4(0.f, 0.f, 0.f, c.w), o1 = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (c.w > 0.0f) {
+    const int zi = (int)(t % N), yi = (int)((t / N) % N), xi = (int)(t / ((size_t)N * N));
+    o1.x = c.x / c.w;
+    o1.y = c.y / c.w + k.dt * k.gravity;
+    o1.z = c.z / c.w;
+    const float4 u = grid_node_update(c, xi, yi, zi, k);
+    bool sticky, floor_;
+    wall_flags(xi, yi, zi, k, sticky, floor_);
+    o0 = make_float4(u.x, u.y, u.z, c.w);  // the mass, not the update's w (1)
+    o1.w = (float)((sticky ? 1 : 0) | (floor_ ? 2 : 0));
+  }
+  tv[t] = o0;
+  tw[t] = o1;
 }
 
 // ---------------------------------------------------------------- backward
-// Adjoint planes use the state layout. adj_in receives the partial x and F
-// adjoints here (k_adj_p2g completes them); grid_adj (4 floats per node)
-// receives the adjoint of the grid velocity after the update.
-__global__ void __launch_bounds__(256)
-    k_adj_g2p(const float* __restrict__ st, const float* __restrict__ tape,
-              const float* __restrict__ adj_out, float* __restrict__ adj_in,
-              float* __restrict__ grid_adj, DiffK d) {
-  __shared__ float tile[kBoxFloats];
-  __shared__ int sh[6];
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// The adjoint of the P2G of substep s (P2GA: the momentum and mass adjoints
+// `gm` of that substep, gathered) fused with the adjoint of the G2P of
+// substep s-1 (G2PA: the grid velocity adjoint scattered into `gv_out`).
+//   adj_src   P2GA: the partial adjoint of X_s the previous G2PA left (x and F
+//             parts; the whole adjoint for particles outside the grid);
+//             otherwise the full adjoint of X_s.
+//   adj_dst   G2PA: the partial adjoint of X_{s-1} (may be adj_src);
+//             otherwise the full adjoint of X_s.
+// X_s = st_s (state before substep s), X_{s-1} = st_prev with tape velocities
+// vel_prev. The mu and lambda adjoints are added into adj_mu, adj_lam.
+template <bool P2GA, bool G2PA>
+__global__ void __launch_bounds__(kWave)
+    k_adj_step(const float* __restrict__ st_s, const float* __restrict__ mu, const float* __restrict__ lam,
+               const float4* __restrict__ gm, const float* adj_src, const float* __restrict__ st_prev,
+               const float4* __restrict__ vel_prev, float* adj_dst, float* __restrict__ adj_mu,
+               float* __restrict__ adj_lam, float* __restrict__ gv_out, DiffK d) {
+  __shared__ float table[G2PA ? kWave * kAdjStride : 1];
+  __shared__ Runs runs;
+  const int lane = threadIdx.x;
+  const int64_t p = (int64_t)blockIdx.x * kWave + lane;
   const StepConsts& k = d.k;
   const int64_t n = d.n;
-  float x[3] = {0.f, 0.f, 0.f}, v[3], C[9], F[9];
-  Stencil sn;
-  bool active = p < n;
-  if (active) {
-    load_state(st, n, p, x, v, C, F);
-    sn = make_stencil(x, k);
-    active = !sn.outside;
-  }
-  const Box box = block_box(sn, active, k.N, 3, sh);
-  if (box.lds) {
-    box_clear(box, 3, tile);
-    __syncthreads();
-  }
-  if (p < n) {
-    float xb2[3], vb2[3], Cb2[9], Fb2[9];
-    load_state(adj_out, n, p, xb2, vb2, Cb2, Fb2);
-    if (!active) {
-      store_state(adj_in, n, p, xb2, vb2, Cb2, Fb2);  // the identity map
-    } else {
-      float vn[3], Cn[9];
-      g2p_affine(sn, x, tape, k, vn, Cn);
-      // x' = x + dt v', F' = (I + dt C') F: adjoints of v' and C' through them
-      float vbt[3], Cbt[9], FbFt[9];
+  const bool valid = p < n;
+  // adjoint of X_s
+  float xb[3] = {0.f, 0.f, 0.f}, vb[3] = {0.f, 0.f, 0.f}, Cb[9], Fb[9];
 #pragma unroll
-      for (int a = 0; a < 3; ++a) vbt[a] = vb2[a] + k.dt * xb2[a];
-      mul_abt(Fb2, F, FbFt);
+  for (int i = 0; i < 9; ++i) Cb[i] = Fb[i] = 0.f;
+  if (valid) load_state(adj_src, n, p, xb, vb, Cb, Fb);
+
+  if (P2GA && valid) {
+    float x[3], v[3], C[9], F[9];
+    load_state(st_s, n, p, x, v, C, F);
+    const Stencil sn = make_stencil(x, k);
+    if (!sn.outside) {  // outside: the partial already holds the identity map
+      const float mup = mu[p], lamp = lam[p];
+      float U[9], s[3], V[9], R[9], tau[9];
+      fc_stress(F, mup, lamp, U, s, V, R, tau);
+      const float sc = stress_scale(d);
+      float A[9];
 #pragma unroll
-      for (int i = 0; i < 9; ++i) Cbt[i] = Cb2[i] + k.dt * FbFt[i];
-      float xb[3] = {xb2[0], xb2[1], xb2[2]}, Fb[9];
+      for (int i = 0; i < 9; ++i) A[i] = sc * tau[i] + d.mass * C[i];
+      float Ab[9];
 #pragma unroll
-      for (int r = 0; r < 3; ++r)
+      for (int i = 0; i < 3; ++i) vb[i] = 0.f;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          // ((I + dt C')^T Fb')_rc
-          float acc = 0.f;
-#pragma unroll
-          for (int q = 0; q < 3; ++q)
-            acc += (((q == r) ? 1.0f : 0.0f) + k.dt * Cn[3 * q + r]) * Fb2[3 * q + c];
-          Fb[3 * r + c] = acc;
-        }
+      for (int i = 0; i < 9; ++i) Ab[i] = 0.f;
 #pragma unroll
       for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -408,49 +420,182 @@ __global__ void __launch_bounds__(256)
             float gw[3], dp[3];
             weight_grad(sn, a, b, c, gw);
             node_offset(sn, a, b, c, x, k, dp);
-            const size_t ni = node_index(sn, a, b, c, k.N);
-            const float4 gv4 = *reinterpret_cast<const float4*>(tape + 8 * ni);
+            const float4 g = gm[node_index(sn, a, b, c, k.N)];
+            const float pb[3] = {g.x, g.y, g.z};
+            float val = g.w * d.mass;  // pb . (m v + A dp) + mb m
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+              vb[r] += d.mass * wt * pb[r];
+              val += pb[r] * (d.mass * v[r] + (A[3 * r] * dp[0] + A[3 * r + 1] * dp[1] + A[3 * r + 2] * dp[2]));
+#pragma unroll
+              for (int q = 0; q < 3; ++q) Ab[3 * r + q] += wt * pb[r] * dp[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+              const float atp = A[q] * pb[0] + A[3 + q] * pb[1] + A[6 + q] * pb[2];  // (A^T pb)_q
+              xb[q] += gw[q] * val - wt * atp;
+            }
+          }
+      // stress adjoint tb = sc Ab; mu and lambda adjoints
+      float tb[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) tb[i] = sc * Ab[i];
+      const float J = det3(F);
+      float FmR[9], FmRFt[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) FmR[i] = F[i] - R[i];
+      mul_abt(FmR, F, FmRFt);
+      float dmu = 0.f;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) dmu += FmRFt[i] * tb[i];
+      const float trt = tb[0] + tb[4] + tb[8];
+      adj_mu[p] += 2.0f * dmu;
+      adj_lam[p] += ((J - 1.0f) * J) * trt;
+      // with R held: d/dF of 2 mu (F - R) F^T contracted with tb = 2 mu (tb F + tb^T (F - R))
+      float tbF[9], tbtFmR[9];
+      mul_ab(tb, F, tbF);
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          tbtFmR[3 * r + c] = tb[r] * FmR[c] + tb[3 + r] * FmR[3 + c] + tb[6 + r] * FmR[6 + c];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) Fb[i] += 2.0f * mup * (tbF[i] + tbtFmR[i]);
+      // through R: Rbar = -2 mu tb F, M = R^T Rbar
+      float M[9];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          M[3 * r + c] = -2.0f * mup * (R[r] * tbF[c] + R[3 + r] * tbF[3 + c] + R[6 + r] * tbF[6 + c]);
+      float Sk[9], T1[9], Mt[9];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Sk[3 * r + c] = 0.5f * (M[3 * r + c] - M[3 * c + r]);
+      mul_ab(Sk, V, T1);  // skew(M) V
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Mt[3 * r + c] = V[r] * T1[c] + V[3 + r] * T1[3 + c] + V[6 + r] * T1[6 + c];
+      float Nm[9];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float den = s[r] + s[c];
+          Nm[3 * r + c] = (r == c || den == 0.0f) ? 0.0f : Mt[3 * r + c] / den;
+        }
+      float UN[9], UNVt[9];
+      mul_ab(U, Nm, UN);
+      mul_abt(UN, V, UNVt);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) Fb[i] += 2.0f * UNVt[i];
+      // volumetric: lambda (2J - 1) tr(tb) dJ/dF, dJ/dF = cof(F)
+      const float cv = lamp * (2.0f * J - 1.0f) * trt;
+      const float cof[9] = {F[4] * F[8] - F[5] * F[7], F[5] * F[6] - F[3] * F[8], F[3] * F[7] - F[4] * F[6],
+                            F[2] * F[7] - F[1] * F[8], F[0] * F[8] - F[2] * F[6], F[1] * F[6] - F[0] * F[7],
+                            F[1] * F[5] - F[2] * F[4], F[2] * F[3] - F[0] * F[5], F[0] * F[4] - F[1] * F[3]};
+#pragma unroll
+      for (int i = 0; i < 9; ++i) Fb[i] += cv * cof[i];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) Cb[i] = d.mass * Ab[i];
+    }
+  }
+  // (xb, vb, Cb, Fb) is the adjoint of X_s
+  if (!G2PA) {
+    if (valid) store_state(adj_dst, n, p, xb, vb, Cb, Fb);
+    return;
+  }
+  // G2P of substep s-1: X_s = G2P(X_{s-1}); the adjoint of X_s flows back
+  // to x and F of X_{s-1} and to the grid velocities of substep s-1
+  int key = -1;
+  Stencil sn;
+  sn.base[0] = sn.base[1] = sn.base[2] = 0;
+  if (valid) {
+    float x[3], F[9];
+    load_xF(st_prev, n, p, x, F);
+    sn = make_stencil(x, k);
+    if (sn.outside) {
+      store_state(adj_dst, n, p, xb, vb, Cb, Fb);  // the identity map
+    } else {
+      float vn[3], Cn[9];
+      g2p_affine(sn, x, vel_prev, k, vn, Cn);
+      // x' = x + dt v', F' = (I + dt C') F: adjoints of v' and C' through them
+      float vbt[3], Cbt[9], FbFt[9];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) vbt[a] = vb[a] + k.dt * xb[a];
+      mul_abt(Fb, F, FbFt);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) Cbt[i] = Cb[i] + k.dt * FbFt[i];
+      float xo[3] = {xb[0], xb[1], xb[2]}, Fo[9];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          // ((I + dt C')^T Fb')_rc
+          float acc = 0.f;
+#pragma unroll
+          for (int q = 0; q < 3; ++q) acc += (((q == r) ? 1.0f : 0.0f) + k.dt * Cn[3 * q + r]) * Fb[3 * q + c];
+          Fo[3 * r + c] = acc;
+        }
+      float* row = table + lane * kAdjStride;
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const int o = 3 * (9 * a + 3 * b + c);
+            if (!node_ok(sn, a, b, c, k.N)) {
+              row[o] = row[o + 1] = row[o + 2] = 0.0f;
+              continue;
+            }
+            const float wt = weight(sn, a, b, c);
+            float gw[3], dp[3];
+            weight_grad(sn, a, b, c, gw);
+            node_offset(sn, a, b, c, x, k, dp);
+            const float4 gv4 = vel_prev[node_index(sn, a, b, c, k.N)];
             const float gv[3] = {gv4.x, gv4.y, gv4.z};
             float Cdp[3];
 #pragma unroll
-            for (int r = 0; r < 3; ++r)
-              Cdp[r] = Cbt[3 * r] * dp[0] + Cbt[3 * r + 1] * dp[1] + Cbt[3 * r + 2] * dp[2];
+            for (int r = 0; r < 3; ++r) Cdp[r] = Cbt[3 * r] * dp[0] + Cbt[3 * r + 1] * dp[1] + Cbt[3 * r + 2] * dp[2];
             const float s1 = gv[0] * vbt[0] + gv[1] * vbt[1] + gv[2] * vbt[2];
             const float s2 = gv[0] * Cdp[0] + gv[1] * Cdp[1] + gv[2] * Cdp[2];
             const float coef = s1 + k.dinv * s2;
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
               const float ctg = Cbt[q] * gv[0] + Cbt[3 + q] * gv[1] + Cbt[6 + q] * gv[2];  // (C'^T g)_q
-              xb[q] += gw[q] * coef - (k.dinv * wt) * ctg;
+              xo[q] += gw[q] * coef - (k.dinv * wt) * ctg;
             }
-            float* ga = box.lds ? tile + 3 * box_slot(box, sn, a, b, c) : grid_adj + 4 * ni;
 #pragma unroll
-            for (int r = 0; r < 3; ++r) unsafeAtomicAdd(ga + r, wt * (vbt[r] + k.dinv * Cdp[r]));
+            for (int r = 0; r < 3; ++r) row[o + r] = wt * (vbt[r] + k.dinv * Cdp[r]);
           }
 #pragma unroll
-      for (int a = 0; a < 3; ++a) adj_in[(DX + a) * n + p] = xb[a];
+      for (int a = 0; a < 3; ++a) adj_dst[(DX + a) * n + p] = xo[a];
 #pragma unroll
-      for (int i = 0; i < 9; ++i) adj_in[(DF + i) * n + p] = Fb[i];
+      for (int i = 0; i < 9; ++i) adj_dst[(DF + i) * n + p] = Fo[i];
+      key = cell_key(sn, k.N);
     }
   }
-  if (box.lds) {
-    __syncthreads();
-    box_flush(box, 3, tile, grid_adj, k.N);
-  }
+  run_scatter<kAdjVals, kAdjStride, 3>(table, runs, key, sn.base, k.N, gv_out);
 }
 
-// adjoint of the updated velocity -> adjoints of the P2G momentum and mass
+// adjoint of the updated velocity (gadj.xyz) -> adjoints of the P2G momentum
+// and mass (gadj, in place); clears `clear` (the next scatter's target)
 __global__ void __launch_bounds__(256)
-    k_adj_grid(float4* __restrict__ grid_adj, const float* __restrict__ tape, StepConsts k) {
+    k_adj_grid(float4* __restrict__ gadj, float4* __restrict__ clear, const float4* __restrict__ tv,
+               const float4* __restrict__ tw, StepConsts k) {
   const size_t nodes = (size_t)k.N * k.N * k.N;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nodes) return;
-  const float4 t0 = *reinterpret_cast<const float4*>(tape + 8 * t);
-  const float4 t1 = *reinterpret_cast<const float4*>(tape + 8 * t + 4);
+  if (clear) clear[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 t0 = tv[t];
+  const float4 t1 = tw[t];
   const float m = t0.w;
   float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
   if (m > 0.0f) {
-    const float4 g = grid_adj[t];
+    const float4 g = gadj[t];
     float b0 = g.x, b1 = g.y, b2 = g.z;
     const int flags = (int)t1.w;
     if (flags & 1) { b0 = 0.f; b1 = 0.f; b2 = 0.f; }
@@ -462,130 +607,7 @@ __global__ void __launch_bounds__(256)
     out.z = b2 / m;
     out.w = -(b0 * t1.x + b1 * py + b2 * t1.z) / m;
   }
-  grid_adj[t] = out;
-}
-
-__global__ void __launch_bounds__(256)
-    k_adj_p2g(const float* __restrict__ st, const float* __restrict__ mu,
-              const float* __restrict__ lam, const float4* __restrict__ grid_adj,
-              float* __restrict__ adj_in, float* __restrict__ adj_mu, float* __restrict__ adj_lam,
-              DiffK d) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= d.n) return;
-  const StepConsts& k = d.k;
-  const int64_t n = d.n;
-  float x[3], v[3], C[9], F[9];
-  load_state(st, n, p, x, v, C, F);
-  const Stencil sn = make_stencil(x, k);
-  if (sn.outside) return;  // k_adj_g2p wrote the identity adjoints
-  const float mup = mu[p], lamp = lam[p];
-  float U[9], s[3], V[9], R[9], tau[9];
-  fc_stress(F, mup, lamp, U, s, V, R, tau);
-  const float sc = stress_scale(d);
-  float A[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) A[i] = sc * tau[i] + d.mass * C[i];
-  float xb[3], Fb[9];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) xb[a] = adj_in[(DX + a) * n + p];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) Fb[i] = adj_in[(DF + i) * n + p];
-  float vb[3] = {0.f, 0.f, 0.f}, Ab[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) Ab[i] = 0.f;
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        if (!node_ok(sn, a, b, c, k.N)) continue;
-        const float wt = weight(sn, a, b, c);
-        float gw[3], dp[3];
-        weight_grad(sn, a, b, c, gw);
-        node_offset(sn, a, b, c, x, k, dp);
-        const float4 g = grid_adj[node_index(sn, a, b, c, k.N)];
-        const float pb[3] = {g.x, g.y, g.z};
-        float val = g.w * d.mass;  // pb . (m v + A dp) + mb m
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-          vb[r] += d.mass * wt * pb[r];
-          val += pb[r] * (d.mass * v[r] + (A[3 * r] * dp[0] + A[3 * r + 1] * dp[1] + A[3 * r + 2] * dp[2]));
-#pragma unroll
-          for (int q = 0; q < 3; ++q) Ab[3 * r + q] += wt * pb[r] * dp[q];
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          const float atp = A[q] * pb[0] + A[3 + q] * pb[1] + A[6 + q] * pb[2];  // (A^T pb)_q
-          xb[q] += gw[q] * val - wt * atp;
-        }
-      }
-  // stress adjoint tb = sc Ab; mu and lambda adjoints
-  float tb[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) tb[i] = sc * Ab[i];
-  const float J = det3(F);
-  float FmR[9], FmRFt[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) FmR[i] = F[i] - R[i];
-  mul_abt(FmR, F, FmRFt);
-  float dmu = 0.f;
-#pragma unroll
-  for (int i = 0; i < 9; ++i) dmu += FmRFt[i] * tb[i];
-  const float trt = tb[0] + tb[4] + tb[8];
-  adj_mu[p] += 2.0f * dmu;
-  adj_lam[p] += ((J - 1.0f) * J) * trt;
-  // with R held: d/dF of 2 mu (F - R) F^T contracted with tb = 2 mu (tb F + tb^T (F - R))
-  float tbF[9], tbtFmR[9];
-  mul_ab(tb, F, tbF);
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      tbtFmR[3 * r + c] = tb[r] * FmR[c] + tb[3 + r] * FmR[3 + c] + tb[6 + r] * FmR[6 + c];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) Fb[i] += 2.0f * mup * (tbF[i] + tbtFmR[i]);
-  // through R: Rbar = -2 mu tb F, M = R^T Rbar
-  float M[9];
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      M[3 * r + c] = -2.0f * mup * (R[r] * tbF[c] + R[3 + r] * tbF[3 + c] + R[6 + r] * tbF[6 + c]);
-  float Sk[9], T1[9], Mt[9];
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) Sk[3 * r + c] = 0.5f * (M[3 * r + c] - M[3 * c + r]);
-  mul_ab(Sk, V, T1);  // skew(M) V
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) Mt[3 * r + c] = V[r] * T1[c] + V[3 + r] * T1[3 + c] + V[6 + r] * T1[6 + c];
-  float Nm[9];
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const float den = s[r] + s[c];
-      Nm[3 * r + c] = (r == c || den == 0.0f) ? 0.0f : Mt[3 * r + c] / den;
-    }
-  float UN[9], UNVt[9];
-  mul_ab(U, Nm, UN);
-  mul_abt(UN, V, UNVt);
-#pragma unroll
-  for (int i = 0; i < 9; ++i) Fb[i] += 2.0f * UNVt[i];
-  // volumetric: lambda (2J - 1) tr(tb) dJ/dF, dJ/dF = cof(F)
-  const float cv = lamp * (2.0f * J - 1.0f) * trt;
-  const float cof[9] = {F[4] * F[8] - F[5] * F[7], F[5] * F[6] - F[3] * F[8], F[3] * F[7] - F[4] * F[6],
-                        F[2] * F[7] - F[1] * F[8], F[0] * F[8] - F[2] * F[6], F[1] * F[6] - F[0] * F[7],
-                        F[1] * F[5] - F[2] * F[4], F[2] * F[3] - F[0] * F[5], F[0] * F[4] - F[1] * F[3]};
-#pragma unroll
-  for (int i = 0; i < 9; ++i) Fb[i] += cv * cof[i];
-  float Cb[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) Cb[i] = d.mass * Ab[i];
-  store_state(adj_in, n, p, xb, vb, Cb, Fb);
+  gadj[t] = out;
 }
 
 __global__ void __launch_bounds__(256) k_zero4(float4* __restrict__ g, size_t n) {
@@ -601,6 +623,18 @@ DiffK diff_consts(const StepConsts& k, float mass, float volume, int64_t n) {
   d.n = n;
   return d;
 }
+
+size_t nodes_of(const StepConsts& k) { return (size_t)k.N * k.N * k.N; }
+unsigned node_blocks(const StepConsts& k) { return (unsigned)((nodes_of(k) + 255) / 256); }
+unsigned particle_blocks(int64_t n) { return (unsigned)((n + kWave - 1) / kWave); }
+
+// the two halves of a tape slot: velocities (read by G2P), then the rest
+const float4* tape_v(const float* tape) { return reinterpret_cast<const float4*>(tape); }
+const float4* tape_w(const float* tape, const StepConsts& k) {
+  return reinterpret_cast<const float4*>(tape) + nodes_of(k);
+}
+float4* tape_v(float* tape) { return reinterpret_cast<float4*>(tape); }
+float4* tape_w(float* tape, const StepConsts& k) { return reinterpret_cast<float4*>(tape) + nodes_of(k); }
 
 }  // namespace
 
@@ -651,34 +685,84 @@ hipError_t diff_forward_noalloc(const StepConsts& k, float mass, float volume, i
   return diff_forward(k, mass, volume, n, st, out, mu, lam, grid, tape, s);
 }
 
+
+hipError_t diff_forward_steps(const StepConsts& k, float mass, float volume, int64_t n, int32_t steps,
+                              const float* st0, float* states, int64_t state_stride, const float* mu,
+                              const float* lam, float4* grid, float* tapes, int64_t tape_stride,
+                              hipStream_t s) {
+  const DiffK d = diff_consts(k, mass, volume, n);
+  const unsigned gp = particle_blocks(n), gn = node_blocks(k);
+  hipLaunchKernelGGL(k_zero4, dim3(gn), dim3(256), 0, s, grid, nodes_of(k));
+  if (n > 0)
+    hipLaunchKernelGGL((k_diff_step<false, true>), dim3(gp), dim3(kWave), 0, s, st0, (float*)nullptr,
+                       (const float4*)nullptr, mu, lam, d, (float*)grid);
+  for (int32_t j = 0; j < steps; ++j) {
+    float* tape = tapes + (size_t)j * (size_t)tape_stride;
+    hipLaunchKernelGGL(k_diff_grid, dim3(gn), dim3(256), 0, s, grid, tape_v(tape), tape_w(tape, k), k);
+    if (n == 0) continue;
+    const float* in = j == 0 ? st0 : states + (size_t)(j - 1) * (size_t)state_stride;
+    float* out = states + (size_t)j * (size_t)state_stride;
+    if (j + 1 < steps)
+      hipLaunchKernelGGL((k_diff_step<true, true>), dim3(gp), dim3(kWave), 0, s, in, out,
+                         (const float4*)tape_v(tape), mu, lam, d, (float*)grid);
+    else
+      hipLaunchKernelGGL((k_diff_step<true, false>), dim3(gp), dim3(kWave), 0, s, in, out,
+                         (const float4*)tape_v(tape), mu, lam, d, (float*)nullptr);
+  }
+  return hipGetLastError();
+}
+
 hipError_t diff_forward(const StepConsts& k, float mass, float volume, int64_t n, const float* st,
                         float* out, const float* mu, const float* lam, float4* grid, float* tape,
                         hipStream_t s) {
-  const size_t nodes = (size_t)k.N * k.N * k.N;
+  return diff_forward_steps(k, mass, volume, n, 1, st, out, 0, mu, lam, grid, tape, 0, s);
+}
+
+hipError_t diff_backward_steps(const StepConsts& k, float mass, float volume, int64_t n, int32_t steps,
+                               const float* states, int64_t state_stride, const float* mu,
+                               const float* lam, const float* tapes, int64_t tape_stride,
+                               const float* adj_out, float* adj_in, float* adj_mu, float* adj_lam,
+                               float4* grid2, float* work, hipStream_t s) {
   const DiffK d = diff_consts(k, mass, volume, n);
-  const unsigned gp = (unsigned)((n + 255) / 256), gn = (unsigned)((nodes + 255) / 256);
-  hipLaunchKernelGGL(k_zero4, dim3(gn), dim3(256), 0, s, grid, nodes);
-  if (n > 0) hipLaunchKernelGGL(k_diff_p2g, dim3(gp), dim3(256), 0, s, st, mu, lam, d, (float*)grid);
-  hipLaunchKernelGGL(k_diff_grid, dim3(gn), dim3(256), 0, s, (const float4*)grid, tape, k);
-  if (n > 0) hipLaunchKernelGGL(k_diff_g2p, dim3(gp), dim3(256), 0, s, st, out, tape, d);
+  const unsigned gp = particle_blocks(n), gn = node_blocks(k);
+  const size_t nodes = nodes_of(k);
+  float4* ga = grid2;
+  float4* gb = steps > 1 ? grid2 + nodes : nullptr;  // a single substep needs one grid
+  auto state = [&](int32_t j) { return states + (size_t)j * (size_t)state_stride; };
+  auto tape = [&](int32_t j) { return tapes + (size_t)j * (size_t)tape_stride; };
+  hipLaunchKernelGGL(k_zero4, dim3(gn), dim3(256), 0, s, ga, nodes);
+  // the G2P adjoint of the last substep, from the caller's adjoint
+  if (n > 0)
+    hipLaunchKernelGGL((k_adj_step<false, true>), dim3(gp), dim3(kWave), 0, s, (const float*)nullptr,
+                       (const float*)nullptr, (const float*)nullptr, (const float4*)nullptr, adj_out,
+                       state(steps - 1), tape_v(tape(steps - 1)), work, (float*)nullptr, (float*)nullptr,
+                       (float*)ga, d);
+  for (int32_t j = steps - 1; j >= 0; --j) {
+    hipLaunchKernelGGL(k_adj_grid, dim3(gn), dim3(256), 0, s, ga, j > 0 ? gb : (float4*)nullptr,
+                       tape_v(tape(j)), tape_w(tape(j), k), k);
+    if (n == 0) continue;
+    if (j > 0) {
+      hipLaunchKernelGGL((k_adj_step<true, true>), dim3(gp), dim3(kWave), 0, s, state(j), mu, lam,
+                         (const float4*)ga, (const float*)work, state(j - 1), tape_v(tape(j - 1)), work, adj_mu,
+                         adj_lam, (float*)gb, d);
+      float4* t = ga;
+      ga = gb;
+      gb = t;
+    } else {
+      hipLaunchKernelGGL((k_adj_step<true, false>), dim3(gp), dim3(kWave), 0, s, state(0), mu, lam,
+                         (const float4*)ga, (const float*)work, (const float*)nullptr, (const float4*)nullptr,
+                         adj_in, adj_mu, adj_lam, (float*)nullptr, d);
+    }
+  }
   return hipGetLastError();
 }
 
 hipError_t diff_backward(const StepConsts& k, float mass, float volume, int64_t n, const float* st,
                          const float* mu, const float* lam, const float* tape, const float* adj_out,
-                         float* adj_in, float* adj_mu, float* adj_lam, float4* grid_adj,
-                         hipStream_t s) {
-  const size_t nodes = (size_t)k.N * k.N * k.N;
-  const DiffK d = diff_consts(k, mass, volume, n);
-  const unsigned gp = (unsigned)((n + 255) / 256), gn = (unsigned)((nodes + 255) / 256);
-  hipLaunchKernelGGL(k_zero4, dim3(gn), dim3(256), 0, s, grid_adj, nodes);
-  if (n > 0)
-    hipLaunchKernelGGL(k_adj_g2p, dim3(gp), dim3(256), 0, s, st, tape, adj_out, adj_in, (float*)grid_adj, d);
-  hipLaunchKernelGGL(k_adj_grid, dim3(gn), dim3(256), 0, s, grid_adj, tape, k);
-  if (n > 0)
-    hipLaunchKernelGGL(k_adj_p2g, dim3(gp), dim3(256), 0, s, st, mu, lam, (const float4*)grid_adj, adj_in,
-                       adj_mu, adj_lam, d);
-  return hipGetLastError();
+                         float* adj_in, float* adj_mu, float* adj_lam, float4* grid_adj, hipStream_t s) {
+  // one substep: one grid, and adj_in holds the partial adjoint in between
+  return diff_backward_steps(k, mass, volume, n, 1, st, 0, mu, lam, tape, 0, adj_out, adj_in, adj_mu,
+                             adj_lam, grid_adj, adj_in, s);
 }
 
 }  // namespace mpmk

@@ -6,9 +6,11 @@
 // captured on one of a few streams forked from the batch's origin stream, so
 // independent scenes run concurrently on the GPU, and the graph is replayed
 // with one launch per capture cycle (two sort intervals of the scenes), with
-// no host work per kernel. Substeps before the first cycle boundary, a
-// remainder shorter than a cycle, or scenes that cannot be captured (not in
-// the default fused tiled mode, timing on) run eagerly on the same streams.
+// no host work per kernel. Substeps before the first cycle boundary and a
+// remainder shorter than a cycle run eagerly: as the same multi-scene
+// launches when every scene is in the fused steady state (below), else as
+// each scene's own kernel chain on the same streams (scenes that cannot be
+// captured: not in the default fused tiled mode, timing on).
 // A scene whose settings, materials, particles or current buffer changed since
 // the capture is captured again.
 //
@@ -22,6 +24,9 @@
 // arguments the scene's own launch would take, so the results are the same
 // bits). The tables are written into pinned host memory while the cycle is
 // recorded and copied to the device by one copy at the head of the graph.
+// An eager fused substep writes its tables into one of a few pinned slots and
+// copies each table on the batch's stream ahead of the launches that read it;
+// a slot is written again once the copies queued from it have run.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -59,6 +64,17 @@ struct mpm_batch_s {
   uint8_t* h_tables = nullptr;
   uint8_t* d_tables = nullptr;
   size_t tables_cap = 0;
+  // argument tables of eager fused substeps: pinned slots used in turn, each
+  // with the event recorded after the substep that last copied from it, and
+  // one device copy (the copies and launches are ordered on the origin stream)
+  static constexpr int kEagerSlots = 4;
+  uint8_t* h_eager[kEagerSlots] = {};
+  hipEvent_t eager_ev[kEagerSlots] = {};
+  bool eager_pending[kEagerSlots] = {};
+  uint8_t* d_eager = nullptr;
+  size_t eager_cap = 0;
+  int eager_next = 0;
+  int64_t eager_fused = 0;  // substeps run eagerly as multi-scene launches (for tests and reports)
 
   int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -83,6 +99,12 @@ struct mpm_batch_s {
     drop_graph();
     if (h_tables) (void)hipHostFree(h_tables);
     if (d_tables) (void)hipFree(d_tables);
+    if (origin) (void)hipStreamSynchronize(origin);  // the eager copies have read their slots
+    for (int i = 0; i < kEagerSlots; ++i) {
+      if (h_eager[i]) (void)hipHostFree(h_eager[i]);
+      if (eager_ev[i]) (void)hipEventDestroy(eager_ev[i]);
+    }
+    if (d_eager) (void)hipFree(d_eager);
     for (auto st : streams) (void)hipStreamDestroy(st);
     for (auto ev : joins) (void)hipEventDestroy(ev);
     if (origin) (void)hipStreamDestroy(origin);
@@ -138,23 +160,38 @@ int run_scenes(mpm_batch_s* b, int32_t n) {
 }
 
 // ---- fused cycles ----------------------------------------------------------
-// Tables are laid out in the pinned buffer as they are produced; identical
-// tables (most substeps between two sorts) share one slot.
+// Tables are laid out in a pinned buffer as they are produced; identical
+// tables (most substeps between two sorts) share one slot. With `copy` set
+// (eager substeps) each new table is copied to its device slot on that stream
+// at once; otherwise one copy of the whole buffer heads the recorded cycle.
 struct TableWriter {
   mpm_batch_s* b;
+  uint8_t* host;
+  uint8_t* dev;
+  size_t cap;
+  hipStream_t copy = nullptr;
   size_t used = 0;
+  hipError_t err = hipSuccess;
   std::vector<std::pair<size_t, size_t>> slots;  // (offset, bytes)
-  // returns the device address of a table holding `bytes` at `src`
+  // returns the device address of a table holding `bytes` at `src`, or
+  // nullptr (see failed())
   const void* put(const void* src, size_t bytes) {
     for (const auto& sl : slots)
-      if (sl.second == bytes && std::memcmp(b->h_tables + sl.first, src, bytes) == 0)
-        return b->d_tables + sl.first;
+      if (sl.second == bytes && std::memcmp(host + sl.first, src, bytes) == 0) return dev + sl.first;
     const size_t off = (used + 255) & ~(size_t)255;
-    if (off + bytes > b->tables_cap) return nullptr;
-    std::memcpy(b->h_tables + off, src, bytes);
+    if (off + bytes > cap) return nullptr;
+    std::memcpy(host + off, src, bytes);
+    if (copy) {
+      err = hipMemcpyAsync(dev + off, host + off, bytes, hipMemcpyHostToDevice, copy);
+      if (err != hipSuccess) return nullptr;
+    }
     slots.push_back({off, bytes});
     used = off + bytes;
-    return b->d_tables + off;
+    return dev + off;
+  }
+  int failed() const {
+    return err != hipSuccess ? b->hipfail(err, "batch: argument table copy")
+                             : b->fail(MPM_ERR_CAPACITY, "batch: argument tables overflow");
   }
 };
 
@@ -169,13 +206,37 @@ bool fusable(const mpm_batch_s* b) {
   return true;
 }
 
+// table bytes of one substep at most: one reduce table, two pass tables and
+// a sort table, each launch with its prefix (the sort: about 20 launches)
+size_t tables_per_substep(size_t ns) {
+  return ns * (sizeof(mpmk::ReduceDesc) + 2 * sizeof(mpmk::ItemDesc) + sizeof(mpmk::SortDesc)) +
+         24 * (4 * (ns + 1) + 256) + 4 * 256;
+}
+
+int ensure_eager_tables(mpm_batch_s* b) {
+  const size_t want = tables_per_substep(b->scenes.size()) + 4096;
+  if (want <= b->eager_cap) return MPM_OK;
+  // the copies queued from the old slots must have run before they go
+  BCHK(hipStreamSynchronize(b->origin), "hipStreamSynchronize");
+  for (int i = 0; i < mpm_batch_s::kEagerSlots; ++i) {
+    if (b->h_eager[i]) (void)hipHostFree(b->h_eager[i]);
+    b->h_eager[i] = nullptr;
+    b->eager_pending[i] = false;
+  }
+  if (b->d_eager) (void)hipFree(b->d_eager);
+  b->d_eager = nullptr;
+  b->eager_cap = 0;
+  for (int i = 0; i < mpm_batch_s::kEagerSlots; ++i) {
+    BCHK(hipHostMalloc((void**)&b->h_eager[i], want, 0), "hipHostMalloc(batch tables)");
+    if (!b->eager_ev[i]) BCHK(hipEventCreateWithFlags(&b->eager_ev[i], hipEventDisableTiming), "hipEventCreate");
+  }
+  BCHK(hipMalloc((void**)&b->d_eager, want), "hipMalloc(batch tables)");
+  b->eager_cap = want;
+  return MPM_OK;
+}
+
 int ensure_tables(mpm_batch_s* b, int32_t n) {
-  const size_t ns = b->scenes.size();
-  // per substep at most: one reduce table, two pass tables and a sort table,
-  // each launch with its prefix (the sort: about 20 launches)
-  const size_t per = ns * (sizeof(mpmk::ReduceDesc) + 2 * sizeof(mpmk::ItemDesc) + sizeof(mpmk::SortDesc)) +
-                     24 * (4 * (ns + 1) + 256) + 4 * 256;
-  const size_t want = per * (size_t)n + 4096;
+  const size_t want = tables_per_substep(b->scenes.size()) * (size_t)n + 4096;
   if (want <= b->tables_cap) return MPM_OK;
   if (b->h_tables) (void)hipHostFree(b->h_tables);
   if (b->d_tables) (void)hipFree(b->d_tables);
@@ -188,21 +249,30 @@ int ensure_tables(mpm_batch_s* b, int32_t n) {
   return MPM_OK;
 }
 
-// Record `n` substeps of every scene as multi-scene launches on the origin
-// stream (the stream is being captured).
-int run_scenes_fused(mpm_batch_s* b, int32_t n) {
+// `n` substeps of every scene as multi-scene launches on the origin stream:
+// recorded (the stream is being captured) or, with `eager`, run at once.
+int run_scenes_fused(mpm_batch_s* b, int32_t n, bool eager) {
   const size_t nsc = b->scenes.size();
   const size_t ns = std::min(b->streams.size(), nsc);
-  TableWriter tw{b};
-  // the tables reach the device before any kernel of the cycle runs
-  BCHK(hipMemcpyAsync(b->d_tables, b->h_tables, b->tables_cap, hipMemcpyHostToDevice, b->origin),
-       "hipMemcpyAsync(batch tables)");
+  TableWriter tw{b, b->h_tables, b->d_tables, b->tables_cap};
+  // recorded: the tables reach the device before any kernel of the cycle runs
+  if (!eager)
+    BCHK(hipMemcpyAsync(b->d_tables, b->h_tables, b->tables_cap, hipMemcpyHostToDevice, b->origin),
+         "hipMemcpyAsync(batch tables)");
   std::vector<mpmk::ReduceDesc> rd(nsc);
   std::vector<uint32_t> rp(nsc + 1), cp(nsc + 1);
   std::vector<mpmk::ItemDesc> gd[2];
   std::vector<uint32_t> gp[2];
   std::vector<mpmk::ItemDesc> done(nsc);
   for (int32_t step = 0; step < n; ++step) {
+    int slot = -1;
+    if (eager) {  // the next pinned slot, once the copies queued from it ran
+      slot = b->eager_next;
+      b->eager_next = (slot + 1) % mpm_batch_s::kEagerSlots;
+      if (b->eager_pending[slot]) BCHK(hipEventSynchronize(b->eager_ev[slot]), "hipEventSynchronize");
+      b->eager_pending[slot] = false;
+      tw = TableWriter{b, b->h_eager[slot], b->d_eager, b->eager_cap, b->origin};
+    }
     // 1. the tile reductions with the grid update, after the tiles of home
     //    blocks with several items were summed into one (small dense scenes
     //    hold dozens of items per block)
@@ -217,7 +287,7 @@ int run_scenes_fused(mpm_batch_s* b, int32_t n) {
     const void* rdt = tw.put(rd.data(), sizeof(mpmk::ReduceDesc) * nsc);
     const void* rpt = tw.put(rp.data(), sizeof(uint32_t) * (nsc + 1));
     const void* cpt = tw.put(cp.data(), sizeof(uint32_t) * (nsc + 1));
-    if (!rdt || !rpt || !cpt) return b->fail(MPM_ERR_CAPACITY, "batch: argument tables overflow");
+    if (!rdt || !rpt || !cpt) return tw.failed();
     BCHK(mpmk::tile_combine_multi((const mpmk::ReduceDesc*)rdt, (const uint32_t*)cpt, (int)nsc, cp[nsc],
                                   b->origin),
          "multi-scene tile combine");
@@ -241,7 +311,7 @@ int run_scenes_fused(mpm_batch_s* b, int32_t n) {
         due.push_back(s);
       }
       const void* sdt = tw.put(sd.data(), sizeof(mpmk::SortDesc) * sd.size());
-      if (!sdt) return b->fail(MPM_ERR_CAPACITY, "batch: argument tables overflow");
+      if (!sdt) return tw.failed();
       int maxp = 0;
       for (const auto& d : sd) maxp = std::max(maxp, d.npasses);
       std::vector<std::pair<int, int>> seq = {{mpmk::SORT_FILL, 0}, {mpmk::SORT_KEYS, 0}};
@@ -255,7 +325,7 @@ int run_scenes_fused(mpm_batch_s* b, int32_t n) {
           pre[j + 1] = pre[j] + (uint32_t)mpmk::sort_blocks(sd[j], pp.first, pp.second);
         if (pre.back() == 0) continue;
         const void* pt = tw.put(pre.data(), sizeof(uint32_t) * pre.size());
-        if (!pt) return b->fail(MPM_ERR_CAPACITY, "batch: argument tables overflow");
+        if (!pt) return tw.failed();
         BCHK(mpmk::sort_multi(pp.first, pp.second, (const mpmk::SortDesc*)sdt, (const uint32_t*)pt,
                               (int)sd.size(), pre.back(), b->origin),
              "multi-scene sort");
@@ -294,7 +364,7 @@ int run_scenes_fused(mpm_batch_s* b, int32_t n) {
       if (gd[g].empty()) continue;
       const void* dt = tw.put(gd[g].data(), sizeof(mpmk::ItemDesc) * gd[g].size());
       const void* pt = tw.put(gp[g].data(), sizeof(uint32_t) * gp[g].size());
-      if (!dt || !pt) return b->fail(MPM_ERR_CAPACITY, "batch: argument tables overflow");
+      if (!dt || !pt) return tw.failed();
       BCHK(mpmk::g2p2g_multi((const mpmk::ItemDesc*)dt, (const uint32_t*)pt, (int)gd[g].size(),
                              gp[g].back(), g, b->origin),
            "multi-scene G2P/P2G");
@@ -303,18 +373,43 @@ int run_scenes_fused(mpm_batch_s* b, int32_t n) {
       b->scenes[j]->fused_pass_done(done[j]);
       b->scenes[j]->substep_done();
     }
+    if (eager) {
+      BCHK(hipEventRecord(b->eager_ev[slot], b->origin), "hipEventRecord");
+      b->eager_pending[slot] = true;
+      ++b->eager_fused;
+    }
   }
   return MPM_OK;
 }
 
+// Longest cycle recorded into one graph (substeps).
+constexpr int64_t kMaxCycle = 1024;
+
+int64_t gcd64(int64_t a, int64_t b) {
+  while (b) {
+    const int64_t t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+// Every scene is at the start of a capture cycle (its next substep sorts).
+// Scenes may sort at different intervals: the batch's cycle is the least
+// common multiple of their cycles (two sort intervals each), after which every
+// scene is back in the same state with the same particle buffer current.
 bool all_at_cycle_start(const mpm_batch_s* b, int* cycle) {
   *cycle = 0;
+  int64_t L = 1;
   for (Sim* s : b->scenes) {
     if (!s->at_cycle_start()) return false;
-    if (*cycle == 0) *cycle = s->cycle_substeps();
-    if (s->cycle_substeps() != *cycle) return false;
+    const int64_t c = s->cycle_substeps();
+    if (c <= 0) return false;
+    L = L / gcd64(L, c) * c;
+    if (L > kMaxCycle) return false;
   }
-  return *cycle > 0;
+  *cycle = (int)L;
+  return L > 0;
 }
 
 bool signature_matches(const mpm_batch_s* b) {
@@ -339,7 +434,7 @@ int capture_and_launch(mpm_batch_s* b, int32_t cycle) {
   if (fused)
     for (Sim* s : b->scenes) locks.emplace_back(s->mutex());
   BCHK(hipStreamBeginCapture(b->origin, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
-  const int rc = fused ? run_scenes_fused(b, cycle) : run_scenes(b, cycle);
+  const int rc = fused ? run_scenes_fused(b, cycle, false) : run_scenes(b, cycle);
   locks.clear();
   if (fused && !rc) ++b->fused_captures;
   hipGraph_t g = nullptr;
@@ -431,13 +526,23 @@ extern "C" int mpm_batch_get_option(mpm_batch b, const char* name, int64_t* v) {
   else if (n == "fuse") *v = b->fuse;
   else if (n == "multi_sort") *v = b->multi_sort;
   else if (n == "fused_captures") *v = b->fused_captures;
+  else if (n == "fused_eager_substeps") *v = b->eager_fused;
   else return b->fail(MPM_ERR_INVALID, "mpm_batch_get_option: unknown option '%s'", name);
   return MPM_OK;
+}
+
+namespace {
+int advance_locked(mpm_batch_s* b, int32_t substeps, void* stream);
 }
 
 extern "C" int mpm_batch_advance(mpm_batch b, int32_t substeps, void* stream) {
   if (!b) return MPM_ERR_INVALID;
   std::lock_guard<std::mutex> lock(b->mu);
+  return advance_locked(b, substeps, stream);
+}
+
+namespace {
+int advance_locked(mpm_batch_s* b, int32_t substeps, void* stream) {
   if (substeps < 0) return b->fail(MPM_ERR_INVALID, "mpm_batch_advance: negative substep count");
   if (b->scenes.empty() || substeps == 0) return MPM_OK;
   int rc = ensure_streams(b);
@@ -462,6 +567,14 @@ extern "C" int mpm_batch_advance(mpm_batch b, int32_t substeps, void* stream) {
         if (rc) return rc;
       }
       left -= cycle;
+    } else if (fusable(b)) {
+      rc = ensure_eager_tables(b);
+      if (rc) return rc;
+      std::vector<std::unique_lock<std::mutex>> locks;
+      for (Sim* s : b->scenes) locks.emplace_back(s->mutex());
+      rc = run_scenes_fused(b, 1, true);
+      if (rc) return rc;
+      left -= 1;
     } else {
       rc = run_scenes(b, 1);
       if (rc) return rc;
@@ -470,5 +583,139 @@ extern "C" int mpm_batch_advance(mpm_batch b, int32_t substeps, void* stream) {
   }
   BCHK(hipEventRecord(b->done, b->origin), "hipEventRecord");
   BCHK(hipStreamWaitEvent(user, b->done, 0), "hipStreamWaitEvent");
+  return MPM_OK;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Batch tuner: candidate settings for the whole batch, each timed from the
+// same snapshot of every scene through the batch itself (graph capture and
+// multi-scene launches included), the fastest kept.
+extern "C" int mpm_batch_autotune(mpm_batch b, int32_t trial_substeps, char* report, int32_t report_len,
+                                  void* stream) {
+  if (!b) return MPM_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(b->mu);
+  if (b->scenes.empty()) return b->fail(MPM_ERR_STATE, "mpm_batch_autotune: the batch is empty");
+  if (trial_substeps < 1 || trial_substeps > 100000)
+    return b->fail(MPM_ERR_INVALID, "mpm_batch_autotune: trial_substeps must be 1..100000");
+  hipStream_t user = (hipStream_t)stream;
+  const size_t ns = b->scenes.size();
+  for (size_t j = 0; j < ns; ++j)
+    if (b->scenes[j]->count() <= 0 || !b->scenes[j]->tiled())
+      return b->fail(MPM_ERR_STATE,
+                     "mpm_batch_autotune: scene %zu is empty or not on the tiled path (quadratic kernel, "
+                     "p2g_mode 1)", j);
+  // each scene's own choice (its tuner restores its state), then snapshots
+  std::vector<mpm_engine::Options> own(ns), saved(ns);
+  for (size_t j = 0; j < ns; ++j) {
+    Sim* s = b->scenes[j];
+    std::lock_guard<std::mutex> l2(s->mutex());
+    saved[j] = s->options();
+    const int rc = s->autotune(trial_substeps, nullptr, user);
+    if (rc) return b->fail(rc, "scene %zu: %s", j, s->last_error().c_str());
+    own[j] = s->options();
+    own[j].radix_bits = 8;  // the batch sorts every scene with 8-bit digits (the same order)
+    const int rs = s->snapshot_take(user);
+    if (rs) return b->fail(rs, "scene %zu: %s", j, s->last_error().c_str());
+  }
+  struct Cand { int64_t sort_every, chunk; bool own; double us; };
+  std::vector<Cand> cands = {{0, 0, true, 0.0}};
+  for (int64_t ch : {(int64_t)mpmk::kMaxItemParticles, (int64_t)128})
+    for (int64_t se : {16, 24, 32, 48, 64}) cands.push_back({se, ch, false, 0.0});
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  BCHK(hipEventCreate(&e0), "hipEventCreate");
+  BCHK(hipEventCreate(&e1), "hipEventCreate");
+  auto apply = [&](const Cand& c) {
+    for (size_t j = 0; j < ns; ++j) {
+      Sim* s = b->scenes[j];
+      std::lock_guard<std::mutex> l2(s->mutex());
+      mpm_engine::Options o = c.own ? own[j] : saved[j];
+      if (!c.own) {
+        o.sort_every = c.sort_every;
+        o.chunk = c.chunk == mpmk::kMaxItemParticles ? 0 : c.chunk;
+        o.radix_bits = 8;
+      }
+      o.p2g_mode = 1;
+      o.fused = 1;
+      o.timing = 0;
+      s->set_options(o);
+    }
+  };
+  auto restore_all = [&]() -> int {
+    for (size_t j = 0; j < ns; ++j) {
+      Sim* s = b->scenes[j];
+      std::lock_guard<std::mutex> l2(s->mutex());
+      const hipError_t e = s->snapshot_restore(user);
+      if (e != hipSuccess) return b->hipfail(e, "mpm_batch_autotune: restore");
+    }
+    b->drop_graph();
+    return MPM_OK;
+  };
+  int rc = MPM_OK;
+  for (Cand& c : cands) {
+    rc = restore_all();
+    if (rc) break;
+    apply(c);
+    // the cycle: least common multiple of the scenes' two sort intervals
+    int64_t L = 1;
+    for (Sim* s : b->scenes) {
+      const int64_t cs = s->cycle_substeps();
+      L = L / gcd64(L, cs) * cs;
+    }
+    if (L > kMaxCycle) {
+      c.us = 1e30;  // never recorded as one graph: not a candidate
+      continue;
+    }
+    // to the first common cycle start, and through one recorded cycle
+    rc = advance_locked(b, (int32_t)(L / 2 + L), user);
+    if (rc) break;
+    const int32_t n = (int32_t)(L * ((std::max<int64_t>(trial_substeps, 2 * L) + L - 1) / L));
+    BCHK(hipEventRecord(e0, user), "hipEventRecord");
+    rc = advance_locked(b, n, user);
+    if (rc) break;
+    BCHK(hipEventRecord(e1, user), "hipEventRecord");
+    BCHK(hipEventSynchronize(e1), "hipEventSynchronize");
+    float ms = 0.0f;
+    BCHK(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    c.us = 1e3 * (double)ms / n;
+  }
+  const int rr = restore_all();
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (Sim* s : b->scenes) s->snapshot_drop();
+  if (rc || rr) {
+    for (size_t j = 0; j < ns; ++j) b->scenes[j]->set_options(saved[j]);
+    return rc ? rc : rr;
+  }
+  size_t best = 0;
+  for (size_t i = 1; i < cands.size(); ++i)
+    if (cands[i].us < cands[best].us) best = i;
+  apply(cands[best]);
+  if (report && report_len > 0) {
+    std::string r = "{\"choice\": ";
+    char buf[160];
+    auto desc = [](const Cand& c) {
+      if (c.own) return std::string("\"own\"");
+      char d[96];
+      std::snprintf(d, sizeof(d), "{\"sort_every\": %lld, \"chunk\": %lld}", (long long)c.sort_every,
+                    (long long)c.chunk);
+      return std::string(d);
+    };
+    r += desc(cands[best]) + ", \"trials\": [";
+    for (size_t i = 0; i < cands.size(); ++i) {
+      r += i ? ", {\"setting\": " : "{\"setting\": ";
+      r += desc(cands[i]);
+      std::snprintf(buf, sizeof(buf), ", \"us_per_substep\": %.3f}", cands[i].us < 1e29 ? cands[i].us : -1.0);
+      r += buf;
+    }
+    r += "], \"own\": [";
+    for (size_t j = 0; j < ns; ++j) {
+      std::snprintf(buf, sizeof(buf), "%s[%lld, %lld]", j ? ", " : "", (long long)own[j].sort_every,
+                    (long long)(own[j].chunk ? own[j].chunk : mpmk::kMaxItemParticles));
+      r += buf;
+    }
+    r += "]}";
+    std::snprintf(report, (size_t)report_len, "%s", r.c_str());
+  }
   return MPM_OK;
 }

@@ -125,6 +125,11 @@ extern "C" int mpm_mesh_write_obj(mpm_handle h, const char* path) {
   return s->mesh_write_obj(path);
 }
 
+extern "C" int mpm_remove_particles(mpm_handle h, int64_t first, int64_t count, void* removed, void* stream) {
+  MPM_GET(h);
+  return s->remove(first, count, removed, (hipStream_t)stream);
+}
+
 extern "C" int mpm_count(mpm_handle h, int64_t* out) {
   MPM_GET(h);
   if (!out) return s->fail(MPM_ERR_INVALID, "mpm_count: null output");
@@ -417,6 +422,71 @@ extern "C" int mpm_diff_substep_noalloc(const mpm_params* params, float particle
                                                   state_out, mu, lambda, tape, (hipStream_t)stream);
   if (e != hipSuccess) {
     set_global_error("mpm_diff_substep_noalloc: %s", hipGetErrorString(e));
+    return MPM_ERR_HIP;
+  }
+  return MPM_OK;
+}
+
+extern "C" int mpm_diff_forward_steps(const mpm_params* params, float particle_mass,
+                                      float particle_volume, int64_t n, int32_t steps, const float* state0,
+                                      float* states, int64_t state_stride, const float* mu,
+                                      const float* lambda, float* grid, float* tapes, int64_t tape_stride,
+                                      void* stream) {
+  StepConsts k;
+  int rc = diff_check(__func__, params, particle_mass, particle_volume, n, &k);
+  if (rc) return rc;
+  const int64_t nodes = (int64_t)params->N * params->N * params->N;
+  if (steps < 1 || state_stride < 24 * n || (tape_stride != 0 && tape_stride < 8 * nodes)) {
+    set_global_error("mpm_diff_forward_steps: need steps >= 1, state_stride >= 24 n and tape_stride 0 or >= 8 N^3");
+    return MPM_ERR_INVALID;
+  }
+  if (!grid || !tapes || (n > 0 && (!state0 || !states || !mu || !lambda))) {
+    set_global_error("mpm_diff_forward_steps: null buffer");
+    return MPM_ERR_INVALID;
+  }
+  if (n > 0 && state0 >= states && state0 < states + (size_t)steps * (size_t)state_stride) {
+    set_global_error("mpm_diff_forward_steps: state0 must not lie in the output states");
+    return MPM_ERR_INVALID;
+  }
+  const hipError_t e = mpmk::diff_forward_steps(k, particle_mass, particle_volume, n, steps, state0, states,
+                                                state_stride, mu, lambda, (float4*)grid, tapes, tape_stride,
+                                                (hipStream_t)stream);
+  if (e != hipSuccess) {
+    set_global_error("mpm_diff_forward_steps: %s", hipGetErrorString(e));
+    return MPM_ERR_HIP;
+  }
+  return MPM_OK;
+}
+
+extern "C" int mpm_diff_backward_steps(const mpm_params* params, float particle_mass,
+                                       float particle_volume, int64_t n, int32_t steps, const float* states,
+                                       int64_t state_stride, const float* mu, const float* lambda,
+                                       const float* tapes, int64_t tape_stride, const float* adj_out,
+                                       float* adj_in, float* adj_mu, float* adj_lambda, float* grid,
+                                       float* work, void* stream) {
+  StepConsts k;
+  int rc = diff_check(__func__, params, particle_mass, particle_volume, n, &k);
+  if (rc) return rc;
+  const int64_t nodes = (int64_t)params->N * params->N * params->N;
+  if (steps < 1 || state_stride < 24 * n || tape_stride < 8 * nodes) {
+    set_global_error("mpm_diff_backward_steps: need steps >= 1, state_stride >= 24 n and tape_stride >= 8 N^3");
+    return MPM_ERR_INVALID;
+  }
+  if (!grid || !tapes ||
+      (n > 0 && (!states || !mu || !lambda || !adj_out || !adj_in || !adj_mu || !adj_lambda || !work))) {
+    set_global_error("mpm_diff_backward_steps: null buffer");
+    return MPM_ERR_INVALID;
+  }
+  if (n > 0 && (adj_in == adj_out || work == adj_out)) {
+    set_global_error("mpm_diff_backward_steps: adj_in and work must not alias adj_out");
+    return MPM_ERR_INVALID;
+  }
+  const hipError_t e = mpmk::diff_backward_steps(k, particle_mass, particle_volume, n, steps, states,
+                                                 state_stride, mu, lambda, tapes, tape_stride, adj_out,
+                                                 adj_in, adj_mu, adj_lambda, (float4*)grid, work,
+                                                 (hipStream_t)stream);
+  if (e != hipSuccess) {
+    set_global_error("mpm_diff_backward_steps: %s", hipGetErrorString(e));
     return MPM_ERR_HIP;
   }
   return MPM_OK;

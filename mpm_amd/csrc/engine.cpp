@@ -529,6 +529,89 @@ int Sim::append(hipStream_t s) {
   return MPM_OK;
 }
 
+// Removes the particles with original index in [first, first + count) on the
+// device: the others are packed into the other particle copy in their order,
+// with the indices above the range lowered by count, so downloads list them
+// as before without the removed ones. `removed` (when non-null) receives the
+// removed particles' state in the reference's 104-byte layout, in index
+// order. The items are rebuilt at the next substep (a sort, then the P2G of
+// the remaining set), as after an append.
+int Sim::remove(int64_t first, int64_t count, void* removed, hipStream_t s) {
+  ++gen_;
+  if (!uploaded_) return fail(MPM_ERR_STATE, "mpm_remove_particles: nothing uploaded");
+  if (first < 0 || count < 0 || first + count > P_)
+    return fail(MPM_ERR_INVALID, "mpm_remove_particles: range [%lld, %lld) outside the %lld particles",
+                (long long)first, (long long)(first + count), (long long)P_);
+  if (count == 0) return MPM_OK;
+  if (removed) {  // the F planes, when the liquid's pass left them implicit
+    const int rc = ensure_f(s);
+    if (rc) return rc;
+  }
+  float* gone = nullptr;
+  uint8_t* gone_mat = nullptr;
+  if (removed) {
+    HIPCHK(hipMalloc(&gone, sizeof(float) * NUM_PLANES * (size_t)count), "hipMalloc(removed particles)");
+    hipError_t e = hipMalloc(&gone_mat, (size_t)count);
+    if (e != hipSuccess) {
+      (void)hipFree(gone);
+      return hip_fail(e, "hipMalloc(removed particles)");
+    }
+  }
+  ParticlesDev src = particles();
+  ParticlesDev dst = src;
+  dst.f = d_f_[cur_ ^ 1];
+  dst.mat = d_mat_[cur_ ^ 1];
+  dst.id = d_id_[cur_ ^ 1];
+  hipError_t e = mpmk::remove_range(src, dst, (uint32_t)first, (uint32_t)(first + count), d_flag_, d_pos_,
+                                    d_hist_, d_item_count_, gone, gone_mat, s);
+  std::vector<float> planes;
+  std::vector<uint8_t> mats;
+  if (e == hipSuccess && removed) {
+    planes.resize((size_t)NUM_PLANES * (size_t)count);
+    mats.resize((size_t)count);
+    e = hipMemcpyAsync(planes.data(), gone, sizeof(float) * planes.size(), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(mats.data(), gone_mat, (size_t)count, hipMemcpyDeviceToHost, s);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (gone) (void)hipFree(gone);
+  if (gone_mat) (void)hipFree(gone_mat);
+  HIPCHK(e, "mpm_remove_particles");
+  if (removed) {
+    // planes -> the reference's particle bytes (types.h:24-35, TransferScheme.h:46-54)
+    uint8_t* out = static_cast<uint8_t*>(removed);
+    const size_t m = (size_t)count;
+    for (size_t j = 0; j < m; ++j) {
+      uint8_t* rec = out + 104 * j;
+      std::memset(rec, 0, 104);
+      rec[0] = mats[j];
+      float x[3], v[3], F[9], C[9];
+      for (int a = 0; a < 3; ++a) {
+        x[a] = planes[(PL_X + a) * m + j];
+        v[a] = planes[(PL_V + a) * m + j];
+      }
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {  // row-major planes -> column-major bytes
+          F[3 * c + r] = planes[(PL_F + 3 * r + c) * m + j];
+          C[3 * c + r] = planes[(PL_C + 3 * r + c) * m + j];
+        }
+      const float jp = planes[PL_JP * m + j];
+      std::memcpy(rec + 4, x, 12);
+      std::memcpy(rec + 16, v, 12);
+      std::memcpy(rec + 28, F, 36);
+      std::memcpy(rec + 64, C, 36);
+      std::memcpy(rec + 100, &jp, 4);
+    }
+  }
+  cur_ ^= 1;
+  P_ -= count;
+  items_valid_ = false;
+  pending_gather_ = false;  // the next sort starts from the packed order
+  primed_ = false;
+  side_clean_ = false;      // the last pass's side-grid scatters are dropped with its tiles
+  steps_since_sort_ = 1 << 30;
+  return MPM_OK;
+}
+
 // Surface mesh (mesh.hip) of the device particles, as the reference's
 // MeshBuilder::computeMesh does it on the host (mesh_builder.h:181-211):
 // voxel_dx = N dx / side (float N dx, then double), distance cut-off
@@ -755,7 +838,7 @@ bool Sim::use_mlp() const { return opt_.reduce_mlp < 0 ? dense_ : opt_.reduce_ml
 
 bool Sim::derives_f() const {
   // only the fused tiled path: the unfused one's G2P kernel reads the F planes
-  if (!opt_.derive_f || !f_canonical_ || opt_.p2g_mode != 1 || !opt_.fused || mats_.empty()) return false;
+  if (!opt_.derive_f || !f_canonical_ || !tiled() || !opt_.fused || mats_.empty()) return false;
   for (const auto& m : mats_)
     if (m.kind != MPM_MATERIAL_WC_LIQUID) return false;
   return true;
@@ -908,12 +991,12 @@ int Sim::substep(hipStream_t s) {
   timing_now_ = k > 0 && timed_subs_ % k == (timed_subs_ / k) % k;
   if (k > 0) ++timed_subs_;
   // every path but the fused tiled one reads the F planes
-  if (!(opt_.p2g_mode == 1 && opt_.fused)) {
+  if (!(tiled() && opt_.fused)) {
     int rc = ensure_f(s);
     if (rc) return rc;
   }
   const ParticlesDev P0 = particles();
-  if (opt_.p2g_mode == 1) {
+  if (tiled()) {
     if (!opt_.fused) {
       // reference order: P2G, grid update, G2P
       int rc = tiled_p2g(s);
@@ -953,9 +1036,9 @@ int Sim::substep(hipStream_t s) {
   if (!opt_.fused) {
     // reference order: clear, P2G, grid update, G2P
     PHASE(PH_CLEAR, HIPCHK(mpmk::fill_zero(gA, grid_nodes_, s), "clear grid"));
-    PHASE(PH_P2G, HIPCHK(mpmk::p2g_direct(P0, d_mats_, k_, gA, s), "p2g"));
+    PHASE(PH_P2G, HIPCHK(mpmk::p2g_direct(P0, d_mats_, k_, gA, s, (int)opt_.interp), "p2g"));
     PHASE(PH_GRID_UPDATE, HIPCHK(mpmk::grid_update(gA, nullptr, k_, s), "grid update"));
-    PHASE(PH_G2P, HIPCHK(mpmk::g2p(P0, d_mats_, k_, gA, s), "g2p"));
+    PHASE(PH_G2P, HIPCHK(mpmk::g2p(P0, d_mats_, k_, gA, s, (int)opt_.interp), "g2p"));
     vel_grid_ = gA;
     gcur_ ^= 1;
     primed_ = false;
@@ -963,12 +1046,12 @@ int Sim::substep(hipStream_t s) {
   }
   if (!primed_) {
     PHASE(PH_CLEAR, HIPCHK(mpmk::fill_zero(gA, grid_nodes_, s), "clear grid"));
-    PHASE(PH_P2G, HIPCHK(mpmk::p2g_direct(P0, d_mats_, k_, gA, s), "p2g"));
+    PHASE(PH_P2G, HIPCHK(mpmk::p2g_direct(P0, d_mats_, k_, gA, s, (int)opt_.interp), "p2g"));
     primed_ = true;
   }
   // gA: P2G of this substep -> velocities; gB cleared in the same pass
   PHASE(PH_GRID_UPDATE, HIPCHK(mpmk::grid_update(gA, gB, k_, s), "grid update"));
-  PHASE(PH_G2P2G, HIPCHK(mpmk::g2p2g_direct(P0, d_mats_, k_, gA, gB, s), "g2p2g"));
+  PHASE(PH_G2P2G, HIPCHK(mpmk::g2p2g_direct(P0, d_mats_, k_, gA, gB, s, (int)opt_.interp), "g2p2g"));
   vel_grid_ = gA;
   gcur_ ^= 1;  // gB (primed P2G of the next substep) becomes current
   return MPM_OK;
@@ -1093,7 +1176,7 @@ int Sim::debug_p2g(float* grid, hipStream_t s) {
   rc = ensure_f(s);
   if (rc) return rc;
   float4* g = nullptr;
-  if (opt_.p2g_mode == 1) {
+  if (tiled()) {
     // fresh sort and P2G into the tiles; the raw sums land in d_grid_[0]
     items_valid_ = false;
     rc = tiled_p2g(s);
@@ -1104,11 +1187,11 @@ int Sim::debug_p2g(float* grid, hipStream_t s) {
     // the non-current grid, so a primed direct-path grid survives
     g = d_grid_[gcur_ ^ 1];
     HIPCHK(mpmk::fill_zero(g, grid_nodes_, s), "clear grid");
-    HIPCHK(mpmk::p2g_direct(particles(), d_mats_, k_, g, s), "p2g");
+    HIPCHK(mpmk::p2g_direct(particles(), d_mats_, k_, g, s, (int)opt_.interp), "p2g");
   }
   HIPCHK(hipMemcpyAsync(grid, g, grid_nodes_ * sizeof(float4), hipMemcpyDeviceToHost, s), "download grid");
   HIPCHK(hipStreamSynchronize(s), "p2g sync");
-  if (opt_.p2g_mode == 1) primed_ = false;  // the reduction consumed the side grid
+  if (tiled()) primed_ = false;  // the reduction consumed the side grid
   vel_grid_ = nullptr;
   return MPM_OK;
 }
@@ -1201,9 +1284,61 @@ int Sim::load_state(const char* path) {
   return MPM_OK;
 }
 
+// A copy of the current particle state (planes, materials, ids), the
+// simulated time and the F bookkeeping, to try settings from and come back to.
+int Sim::snapshot_take(hipStream_t s) {
+  snapshot_drop();
+  const size_t plane_bytes = sizeof(float) * NUM_PLANES * stride_;
+  HIPCHK(hipMalloc(&snap_.f, plane_bytes), "hipMalloc(snapshot)");
+  HIPCHK(hipMalloc(&snap_.m, (size_t)std::max<int64_t>(P_, 1)), "hipMalloc(snapshot)");
+  HIPCHK(hipMalloc(&snap_.id, sizeof(uint32_t) * (size_t)std::max<int64_t>(P_, 1)), "hipMalloc(snapshot)");
+  HIPCHK(hipMemcpyAsync(snap_.f, d_f_[cur_], plane_bytes, hipMemcpyDeviceToDevice, s), "snapshot copy");
+  HIPCHK(hipMemcpyAsync(snap_.m, d_mat_[cur_], (size_t)P_, hipMemcpyDeviceToDevice, s), "snapshot copy");
+  HIPCHK(hipMemcpyAsync(snap_.id, d_id_[cur_], sizeof(uint32_t) * (size_t)P_, hipMemcpyDeviceToDevice, s),
+         "snapshot copy");
+  snap_.t = t_;
+  snap_.canonical = f_canonical_;
+  snap_.opt = opt_;
+  return MPM_OK;
+}
+
+// Puts the snapshot's particles back (into copy 0) and restarts the step
+// bookkeeping from them: the next substep sorts and primes. Options are left
+// as they are.
+hipError_t Sim::snapshot_restore(hipStream_t s) {
+  if (!snap_.f) return hipErrorInvalidValue;
+  const size_t plane_bytes = sizeof(float) * NUM_PLANES * stride_;
+  hipError_t r = hipMemcpyAsync(d_f_[0], snap_.f, plane_bytes, hipMemcpyDeviceToDevice, s);
+  if (r == hipSuccess) r = hipMemcpyAsync(d_mat_[0], snap_.m, (size_t)P_, hipMemcpyDeviceToDevice, s);
+  if (r == hipSuccess) r = hipMemcpyAsync(d_id_[0], snap_.id, sizeof(uint32_t) * (size_t)P_, hipMemcpyDeviceToDevice, s);
+  cur_ = 0;
+  items_valid_ = false;
+  primed_ = false;
+  pending_gather_ = false;
+  steps_since_sort_ = 1 << 30;
+  side_clean_ = false;
+  vel_grid_ = nullptr;
+  f_stale_ = false;
+  f_canonical_ = snap_.canonical;
+  t_ = snap_.t;
+  ++gen_;
+  return r;
+}
+
+void Sim::snapshot_drop() {
+  if (snap_.f) (void)hipFree(snap_.f);
+  if (snap_.m) (void)hipFree(snap_.m);
+  if (snap_.id) (void)hipFree(snap_.id);
+  snap_.f = nullptr;
+  snap_.m = nullptr;
+  snap_.id = nullptr;
+}
+
 int Sim::autotune(int32_t trial_substeps, std::string* report, hipStream_t s) {
   ++gen_;
   if (!uploaded_ || P_ <= 0) return fail(MPM_ERR_STATE, "mpm_autotune: nothing uploaded");
+  if (opt_.interp != MPM_INTERP_QUADRATIC)
+    return fail(MPM_ERR_STATE, "mpm_autotune: tunes the tiled path, which implements the quadratic kernel only");
   if (trial_substeps < 1 || trial_substeps > 100000)
     return fail(MPM_ERR_INVALID, "mpm_autotune: trial_substeps must be 1..100000");
   int rc = check_materials_cover_particles("mpm_autotune");
@@ -1213,38 +1348,14 @@ int Sim::autotune(int32_t trial_substeps, std::string* report, hipStream_t s) {
   rc = ensure_f(s);
   if (rc) return rc;
   // snapshot of the current particle copy
-  const size_t plane_bytes = sizeof(float) * NUM_PLANES * stride_;
-  float* snap_f = nullptr;
-  uint8_t* snap_m = nullptr;
-  uint32_t* snap_i = nullptr;
-  HIPCHK(hipMalloc(&snap_f, plane_bytes), "hipMalloc(autotune snapshot)");
-  hipError_t e = hipMalloc(&snap_m, (size_t)P_);
-  if (e == hipSuccess) e = hipMalloc(&snap_i, sizeof(uint32_t) * (size_t)P_);
-  if (e == hipSuccess) e = hipMemcpyAsync(snap_f, d_f_[cur_], plane_bytes, hipMemcpyDeviceToDevice, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(snap_m, d_mat_[cur_], (size_t)P_, hipMemcpyDeviceToDevice, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(snap_i, d_id_[cur_], sizeof(uint32_t) * (size_t)P_, hipMemcpyDeviceToDevice, s);
+  rc = snapshot_take(s);
+  if (rc) return rc;
+  hipError_t e = hipSuccess;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
   const Options saved = opt_;
-  const double t0 = t_;
-  const bool canon = f_canonical_;
-  auto restore = [&]() -> hipError_t {
-    hipError_t r = hipMemcpyAsync(d_f_[0], snap_f, plane_bytes, hipMemcpyDeviceToDevice, s);
-    if (r == hipSuccess) r = hipMemcpyAsync(d_mat_[0], snap_m, (size_t)P_, hipMemcpyDeviceToDevice, s);
-    if (r == hipSuccess) r = hipMemcpyAsync(d_id_[0], snap_i, sizeof(uint32_t) * (size_t)P_, hipMemcpyDeviceToDevice, s);
-    cur_ = 0;
-    items_valid_ = false;
-    primed_ = false;
-    pending_gather_ = false;
-    steps_since_sort_ = 1 << 30;
-    side_clean_ = false;
-    vel_grid_ = nullptr;
-    f_stale_ = false;
-    f_canonical_ = canon;
-    t_ = t0;
-    return r;
-  };
+  auto restore = [&]() -> hipError_t { return snapshot_restore(s); };
   struct Trial { int64_t sort_every, chunk, radix_bits; double us; };
   std::vector<Trial> trials;
   auto run_trial = [&](int64_t se, int64_t ch, int64_t rb) -> bool {
@@ -1291,9 +1402,7 @@ int Sim::autotune(int32_t trial_substeps, std::string* report, hipStream_t s) {
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  (void)hipFree(snap_f);
-  (void)hipFree(snap_m);
-  (void)hipFree(snap_i);
+  snapshot_drop();
   opt_ = saved;
   if (rc) return rc;
   if (e != hipSuccess) return hip_fail(e, "mpm_autotune");
@@ -1333,6 +1442,14 @@ int Sim::set_option(const char* name, int64_t v) {
     opt_.p2g_mode = v;
     return MPM_OK;
   }
+  if (n == "interpolation") {
+    if (v != MPM_INTERP_QUADRATIC && v != MPM_INTERP_CUBIC)
+      return fail(MPM_ERR_INVALID, "interpolation must be %d (quadratic) or %d (cubic)", MPM_INTERP_QUADRATIC,
+                  MPM_INTERP_CUBIC);
+    if (v != opt_.interp) primed_ = false;
+    opt_.interp = v;
+    return MPM_OK;
+  }
   if (n == "sort_every") { if (v < 0) return fail(MPM_ERR_INVALID, "sort_every must be >= 0"); opt_.sort_every = v; return MPM_OK; }
   if (n == "derive_f") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "derive_f must be 0 or 1"); opt_.derive_f = v; return MPM_OK; }
   if (n == "reduce_mlp") { if (v < -1 || v > 1) return fail(MPM_ERR_INVALID, "reduce_mlp must be -1, 0 or 1"); opt_.reduce_mlp = v; return MPM_OK; }
@@ -1366,6 +1483,7 @@ int Sim::get_option(const char* name, int64_t* v) {
   const std::string n = name;
   if (n == "fused") *v = opt_.fused;
   else if (n == "p2g_mode") *v = opt_.p2g_mode;
+  else if (n == "interpolation") *v = opt_.interp;
   else if (n == "sort_every") *v = opt_.sort_every;
   else if (n == "radix_bits") *v = opt_.radix_bits;
   else if (n == "chunk") *v = opt_.chunk > 0 ? opt_.chunk : mpmk::kMaxItemParticles;

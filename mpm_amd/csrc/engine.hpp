@@ -23,6 +23,8 @@ MaterialDev make_material_dev(const mpm_material& m, const StepConsts& k);
 struct Options {
   int64_t fused = 1;        // "fused": G2P of step n fused with P2G of step n+1
   int64_t p2g_mode = 1;     // "p2g_mode": 0 direct atomics, 1 sorted + LDS tile
+  int64_t interp = MPM_INTERP_QUADRATIC;  // "interpolation": kernel of the TransferScheme policy
+                                          // (the cubic one runs on the direct path)
   int64_t sort_every = 16;  // "sort_every": re-sort particles every k substeps (0: never)
   int64_t radix_bits = 8;   // "radix_bits": digit width of the radix sort (4, 6 or 8)
   int64_t chunk = 0;        // "chunk": max particles per work item (64..192, multiple of 64; 0 = 192)
@@ -57,6 +59,8 @@ class Sim {
   int clear_staging();
   int upload(hipStream_t s);
   int append(hipStream_t s);
+  // removes the particles with original index in [first, first + count)
+  int remove(int64_t first, int64_t count, void* removed, hipStream_t s);
   // surface mesh of the device particles (mesh.hip)
   int mesh_build(int side, int radius, int64_t* ntri, hipStream_t s);
   int mesh_triangles(float* xyz, uint64_t* keys);
@@ -80,6 +84,17 @@ class Sim {
   // a snapshot of the current state, keeps the fastest, restores the state;
   // *report receives the trials as JSON
   int autotune(int32_t trial_substeps, std::string* report, hipStream_t s);
+  // a device copy of the particle state to try settings from (the batch
+  // tuner); restore puts it back and restarts the step bookkeeping
+  int snapshot_take(hipStream_t s);
+  hipError_t snapshot_restore(hipStream_t s);
+  void snapshot_drop();
+  const Options& options() const { return opt_; }
+  void set_options(const Options& o) { opt_ = o; ++gen_; }
+
+  // the substep runs the tiled path: p2g_mode 1 with the quadratic kernel
+  // (the tiles are sized for its 3-node stencil; other kernels run direct)
+  bool tiled() const { return opt_.p2g_mode == 1 && opt_.interp == MPM_INTERP_QUADRATIC; }
 
   // ---- replay of captured substeps (batch.cpp) ----
   // A capture cycle is 2 sort intervals: it starts when the next substep sorts
@@ -87,7 +102,7 @@ class Sim {
   int cycle_substeps() const { return 2 * (int)opt_.sort_every; }
   bool at_cycle_start() const {
     return uploaded_ && !mats_dirty_ && primed_ && items_valid_ && !pending_gather_ &&
-           opt_.p2g_mode == 1 && opt_.fused == 1 && opt_.sort_every > 0 && !opt_.timing &&
+           tiled() && opt_.fused == 1 && opt_.sort_every > 0 && !opt_.timing &&
            steps_since_sort_ == opt_.sort_every;
   }
   // identifies the buffers and settings a captured cycle was recorded with
@@ -102,7 +117,7 @@ class Sim {
   // the scene is in the steady state of the tiled fused mode (primed, items
   // built, materials uploaded, timing off)
   bool fused_ready() const {
-    return uploaded_ && P_ > 0 && !mats_dirty_ && primed_ && items_valid_ && opt_.p2g_mode == 1 &&
+    return uploaded_ && P_ > 0 && !mats_dirty_ && primed_ && items_valid_ && tiled() &&
            opt_.fused == 1 && opt_.sort_every > 0 && !opt_.timing;
   }
   // part 1: the tile reduction with the grid update into the velocity grid
@@ -180,6 +195,15 @@ class Sim {
   int64_t mesh_ntri_ = -1;
   int ensure_f(hipStream_t s);
   int64_t timed_subs_ = 0;   // substeps since timing_reset
+
+  struct Snapshot {
+    float* f = nullptr;
+    uint8_t* m = nullptr;
+    uint32_t* id = nullptr;
+    double t = 0.0;
+    bool canonical = false;
+    Options opt;
+  } snap_;
 
   std::mutex mu_;
   std::string err_;

@@ -9,13 +9,17 @@
 namespace mpmk {
 
 hipError_t fill_zero(float4* g, size_t n, hipStream_t s);
+// The direct kernels over the TransferScheme policy of interpolation kernel
+// `interp` (MPM_INTERP_QUADRATIC: the reference's MLS-APIC with the quadratic
+// B-spline; MPM_INTERP_CUBIC: the cubic B-spline with the generic D^-1).
 hipError_t p2g_direct(const ParticlesDev& P, const MaterialDev* mats, const StepConsts& k,
-                      float4* grid, hipStream_t s);
+                      float4* grid, hipStream_t s, int interp = MPM_INTERP_QUADRATIC);
 hipError_t grid_update(float4* g, float4* clear, const StepConsts& k, hipStream_t s);
 hipError_t g2p(const ParticlesDev& P, const MaterialDev* mats, const StepConsts& k,
-               const float4* grid, hipStream_t s);
+               const float4* grid, hipStream_t s, int interp = MPM_INTERP_QUADRATIC);
 hipError_t g2p2g_direct(const ParticlesDev& P, const MaterialDev* mats, const StepConsts& k,
-                        const float4* grid_in, float4* grid_out, hipStream_t s);
+                        const float4* grid_in, float4* grid_out, hipStream_t s,
+                        int interp = MPM_INTERP_QUADRATIC);
 hipError_t svd_batch(const float* A, float* U, float* S, float* V, int64_t n, hipStream_t s);
 hipError_t probe_kernel_image(int* nregs);
 
@@ -57,6 +61,14 @@ hipError_t build_items(const uint32_t* keys, const uint32_t* perm, int64_t n, in
                        int nbx, uint32_t* flag, uint32_t* pos, uint32_t* scratch, uint32_t* total,
                        uint32_t* item_start, uint32_t* item_block, uint32_t* item_waves,
                        uint32_t* gather, int64_t max_items, hipStream_t s);
+// Removes the particles with original index in [lo, hi): the others packed
+// into dst in their order, indices above the range lowered by hi - lo; the
+// removed ones (when `gone` is non-null) into planes of hi - lo floats at
+// slot index - lo, their materials into gone_mat. flag, pos: src.n entries;
+// scratch: exclusive_scan's tile sums; total: one word.
+hipError_t remove_range(const ParticlesDev& src, const ParticlesDev& dst, uint32_t lo, uint32_t hi,
+                        uint32_t* flag, uint32_t* pos, uint32_t* scratch, uint32_t* total, float* gone,
+                        uint8_t* gone_mat, hipStream_t s);
 // dst[i] = src[perm[i]]
 hipError_t reorder(const ParticlesDev& src, const ParticlesDev& dst, const uint32_t* perm,
                    hipStream_t s);
@@ -209,7 +221,9 @@ hipError_t mesh_emit(const uint32_t* sdf, int side, double rv, double vdx, const
                      uint64_t* keys, hipStream_t s);
 
 // adjoint.hip: the differentiable substep (24 state planes of n floats) and
-// its adjoint; grid is 4 N^3 floats of scratch, tape 8 N^3 floats.
+// its adjoint; grid is 4 N^3 floats of scratch, tape 8 N^3 floats (N^3
+// float4 velocity and mass after the update, then N^3 float4 velocity
+// before the walls and wall flags).
 hipError_t diff_forward(const StepConsts& k, float mass, float volume, int64_t n, const float* st,
                         float* out, const float* mu, const float* lam, float4* grid, float* tape,
                         hipStream_t s);
@@ -220,6 +234,20 @@ constexpr int kNoAllocMaxN = 64;
 hipError_t diff_forward_noalloc(const StepConsts& k, float mass, float volume, int64_t n,
                                 const float* st, float* out, const float* mu, const float* lam,
                                 float* tape, hipStream_t s);
+// `steps` substeps from st0: the state after substep j+1 at states + j
+// state_stride, its tape at tapes + j tape_stride (tape_stride 0: one tape)
+hipError_t diff_forward_steps(const StepConsts& k, float mass, float volume, int64_t n, int32_t steps,
+                              const float* st0, float* states, int64_t state_stride, const float* mu,
+                              const float* lam, float4* grid, float* tapes, int64_t tape_stride,
+                              hipStream_t s);
+// the adjoint of `steps` substeps: states + j state_stride is the state
+// before substep j, tapes + j tape_stride its tape; grid2 is 8 N^3 floats of
+// scratch, work 24 n floats (adj_in may be work)
+hipError_t diff_backward_steps(const StepConsts& k, float mass, float volume, int64_t n, int32_t steps,
+                               const float* states, int64_t state_stride, const float* mu,
+                               const float* lam, const float* tapes, int64_t tape_stride,
+                               const float* adj_out, float* adj_in, float* adj_mu, float* adj_lam,
+                               float4* grid2, float* work, hipStream_t s);
 hipError_t diff_backward(const StepConsts& k, float mass, float volume, int64_t n, const float* st,
                          const float* mu, const float* lam, const float* tape, const float* adj_out,
                          float* adj_in, float* adj_mu, float* adj_lam, float4* grid_adj,

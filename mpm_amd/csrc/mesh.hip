@@ -41,21 +41,23 @@ __global__ void __launch_bounds__(256)
     beg[a] = max(0, (int)(xg - rg));
     end[a] = min(side, (int)(xg + (rg + 1.0f)));
   }
-  // |x - x_node| as ((dx^2 + dy^2) + dz^2), x_node = i * voxel_dx
+  // |x - x_node| = sqrt(dx^2 + (dy^2 + dz^2)): Eigen's norm() sums the three
+  // squares of a fixed-size vector as e0 + (e1 + e2); x_node = i * voxel_dx
   for (int i = beg[0]; i < end[0]; ++i) {
     const float dx = x[0] - (float)(uint32_t)i * vdx;
     const float dx2 = dx * dx;
     // rows and planes whose nearest voxel is at the cut-off or beyond change
-    // nothing: d >= sqrt(dx^2 + dy^2) >= sqrt(dx^2) in float as well
+    // nothing: rounding is monotonic, so d^2 >= fl(dx^2 + dy^2) >= dx^2 in
+    // float as well
     if (sqrtf(dx2) >= cutoff) continue;
     for (int j = beg[1]; j < end[1]; ++j) {
       const float dy = x[1] - (float)(uint32_t)j * vdx;
-      const float dxy = dx2 + dy * dy;
-      if (sqrtf(dxy) >= cutoff) continue;
+      const float dy2 = dy * dy;
+      if (sqrtf(dx2 + dy2) >= cutoff) continue;
       const size_t row = ((size_t)i * side + (size_t)j) * side;
       for (int k = beg[2]; k < end[2]; ++k) {
         const float dz = x[2] - (float)(uint32_t)k * vdx;
-        const float d = sqrtf(dxy + dz * dz);
+        const float d = sqrtf(dx2 + (dy2 + dz * dz));
         const uint32_t bits = __float_as_uint(d);
         uint32_t* const cell = sdf + row + (size_t)k;
         if (bits < *cell) atomicMin(cell, bits);

@@ -452,11 +452,16 @@ __device__ __forceinline__ void g2p2g_item(const mpmk::ItemDesc& d, const int it
         mul_ab(A, s.F, s.F);
       }
       if (KS == KINDS_LIQUID) {
-        // F <- J^(1/3) I, Jp <- J (the only kind this instantiation handles)
+        // F <- J^(1/3) I, Jp <- J (the only kind this instantiation handles).
+        // With F derived from J the new F is neither stored nor used (the
+        // liquid's stress reads J): its cube root is left to the next pass,
+        // which rebuilds F_old = cbrt(Jp) I with the same bits.
         const float J = det3(s.F);
-        const float cj = cbrtf(J);
+        if (!df) {
+          const float cj = cbrtf(J);
 #pragma unroll
-        for (int a = 0; a < 9; ++a) s.F[a] = (a % 4 == 0) ? cj : 0.0f;
+          for (int a = 0; a < 9; ++a) s.F[a] = (a % 4 == 0) ? cj : 0.0f;
+        }
         s.Jp = J;
       } else {
         plasticity(s.F, s.Jp, m, R, has_R);

@@ -526,6 +526,39 @@ __global__ void __launch_bounds__(256)
   dst.id[i] = src.id[j];
 }
 
+// Removal of the particles with original index in [lo, hi): keep flags, then
+// the kept ones packed in their order into dst (indices above the range move
+// down by hi - lo) and the removed ones into `gone` (planes of hi - lo
+// floats, slot id - lo) with their material indices.
+__global__ void __launch_bounds__(256)
+    k_keep_flags(ParticlesDev P, uint32_t lo, uint32_t hi, uint32_t* __restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const uint32_t id = P.id[i];
+  flag[i] = (id >= lo && id < hi) ? 0u : 1u;
+}
+
+__global__ void __launch_bounds__(256)
+    k_compact(ParticlesDev src, ParticlesDev dst, const uint32_t* __restrict__ pos, uint32_t lo, uint32_t hi,
+              float* __restrict__ gone, uint8_t* __restrict__ gone_mat) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= src.n) return;
+  const uint32_t id = src.id[i];
+  if (id >= lo && id < hi) {
+    if (!gone) return;
+    const uint32_t j = id - lo, m = hi - lo;
+#pragma unroll
+    for (int p = 0; p < NUM_PLANES; ++p) gone[(size_t)p * m + j] = src.f[p * src.stride + i];
+    gone_mat[j] = src.mat[i];
+    return;
+  }
+  const uint32_t j = pos[i];
+#pragma unroll
+  for (int p = 0; p < NUM_PLANES; ++p) dst.f[p * dst.stride + j] = src.f[p * src.stride + i];
+  dst.mat[j] = src.mat[i];
+  dst.id[j] = id >= hi ? id - (hi - lo) : id;
+}
+
 template <int BITS>
 hipError_t radix_sort_t(uint32_t* keys[2], uint32_t* vals[2], int64_t n, int start_bit, int key_bits,
                         uint32_t* scratch, int* out_idx, hipStream_t s) {
@@ -719,6 +752,18 @@ hipError_t radix_sort(uint32_t* keys[2], uint32_t* vals[2], int64_t n, int start
     case 8: return radix_sort_t<8>(keys, vals, n, start_bit, key_bits, scratch, out_idx, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t remove_range(const ParticlesDev& src, const ParticlesDev& dst, uint32_t lo, uint32_t hi,
+                        uint32_t* flag, uint32_t* pos, uint32_t* scratch, uint32_t* total, float* gone,
+                        uint8_t* gone_mat, hipStream_t s) {
+  if (src.n <= 0) return hipSuccess;
+  const unsigned g = (unsigned)((src.n + 255) / 256);
+  hipLaunchKernelGGL(k_keep_flags, dim3(g), dim3(256), 0, s, src, lo, hi, flag);
+  hipError_t e = exclusive_scan(flag, pos, src.n, scratch, total, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_compact, dim3(g), dim3(256), 0, s, src, dst, pos, lo, hi, gone, gone_mat);
+  return hipGetLastError();
 }
 
 hipError_t reorder(const ParticlesDev& src, const ParticlesDev& dst, const uint32_t* perm,

@@ -25,8 +25,11 @@
 #include "device_math.hpp"
 #include "engine_types.hpp"
 #include "kernels.hpp"
+#include "plugins.hpp"
 
 using namespace mpmdev;
+using mpmplug::CubicScheme;
+using mpmplug::ReferenceScheme;
 
 namespace {
 
@@ -71,42 +74,39 @@ __device__ __forceinline__ void atomic_add_f32(float* p, float v) {
   unsafeAtomicAdd(p, v);
 }
 
-// affine = stress_scale * tau + m C (TransferScheme.h:83-85)
-__device__ __forceinline__ void make_affine(const float tau[9], const float C[9],
-                                            const MaterialDev& m, float aff[9]) {
-#pragma unroll
-  for (int a = 0; a < 9; ++a) aff[a] = m.stress_scale * tau[a] + m.mass * C[a];
-}
-
-// Scatter one particle's 27 node contributions with global atomics.
-// Reference node bounds (mpm.cu:39-44): begin max(0, -base); end min(3, N+base)
-// in compat REFERENCE (j, k >= N alias into the next row of the linear index;
-// indices >= N^3 are dropped), min(3, N-base) in compat FIXED.
+// Scatter one particle's stencil contributions (p2g_node_contribution,
+// TransferScheme.h:88-100) with global atomics. Reference node bounds
+// (mpm.cu:39-44): begin max(0, -base); end min(S, N+base) in compat REFERENCE
+// (j, k >= N alias into the next row of the linear index; indices >= N^3 are
+// dropped), min(S, N-base) in compat FIXED.
+template <class Sch>
 __device__ __forceinline__ void scatter_direct(const float x[3], const float v[3],
-                                               const float aff[9], float mass, const int base[3],
-                                               const float w[3][3], const StepConsts& k,
-                                               float* __restrict__ grid) {
+                                               const float aff[9], float mass, const Sch& sch,
+                                               const StepConsts& k, float* __restrict__ grid) {
+  constexpr int S = Sch::S;
+  const int* base = sch.base;
+  const auto& w = sch.w;
   const int N = k.N;
   int beg[3], end[3];
-  float d[3][3];
+  float d[3][S];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     beg[a] = max(0, -base[a]);
-    end[a] = min(3, (k.compat == MPM_COMPAT_FIXED) ? N - base[a] : N + base[a]);
+    end[a] = min(S, (k.compat == MPM_COMPAT_FIXED) ? N - base[a] : N + base[a]);
 #pragma unroll
-    for (int o = 0; o < 3; ++o) d[a][o] = (float)(uint32_t)(base[a] + o) * k.dx - x[a];
+    for (int o = 0; o < S; ++o) d[a][o] = (float)(uint32_t)(base[a] + o) * k.dx - x[a];
   }
   const float mv0 = v[0] * mass, mv1 = v[1] * mass, mv2 = v[2] * mass;
   const uint64_t n3 = (uint64_t)N * N * N;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
+  for (int i = 0; i < S; ++i) {
     if (i < beg[0] || i >= end[0]) continue;
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
+    for (int j = 0; j < S; ++j) {
       if (j < beg[1] || j >= end[1]) continue;
       const float wij = w[0][i] * w[1][j];
 #pragma unroll
-      for (int kk = 0; kk < 3; ++kk) {
+      for (int kk = 0; kk < S; ++kk) {
         if (kk < beg[2] || kk >= end[2]) continue;
         const uint32_t idx = (uint32_t)N * (uint32_t)N * (uint32_t)(base[0] + i) +
                              (uint32_t)N * (uint32_t)(base[1] + j) + (uint32_t)(base[2] + kk);
@@ -125,33 +125,40 @@ __device__ __forceinline__ void scatter_direct(const float x[3], const float v[3
   }
 }
 
-// G2P gather (mpm.cu:135-167, TransferScheme.h:120-132) into v and C.
-__device__ __forceinline__ void gather(const float x[3], const int base[3], const float w[3][3],
-                                       const StepConsts& k, const float4* __restrict__ grid,
-                                       float v[3], float C[9]) {
+// G2P gather (mpm.cu:135-167; g2p_node_contribution, TransferScheme.h:120-132)
+// into v and C: v += w v_i, C += (w v_i)(d^T D^-1).
+template <class Sch>
+__device__ __forceinline__ void gather(const float x[3], const Sch& sch, const StepConsts& k,
+                                       const float4* __restrict__ grid, float v[3], float C[9]) {
+  constexpr int S = Sch::S;
+  const int* base = sch.base;
+  const auto& w = sch.w;
   const int N = k.N;
-  float dd[3][3];
-  bool ok[3][3];
+  float dd[3][S];  // constant D^-1: (d * dinv) per axis; generic: the offset d
+  bool ok[3][S];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int o = 0; o < 3; ++o) {
+    for (int o = 0; o < S; ++o) {
       const int g = base[a] + o;
       ok[a][o] = g >= 0 && g < N;
-      dd[a][o] = ((float)(uint32_t)g * k.dx - x[a]) * k.dinv;
+      if constexpr (Sch::Kernel::d_is_const())
+        dd[a][o] = ((float)(uint32_t)g * k.dx - x[a]) * k.dinv;
+      else
+        dd[a][o] = (float)(uint32_t)g * k.dx - x[a];
     }
 #pragma unroll
   for (int a = 0; a < 3; ++a) v[a] = 0.0f;
 #pragma unroll
   for (int a = 0; a < 9; ++a) C[a] = 0.0f;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
+  for (int i = 0; i < S; ++i) {
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
+    for (int j = 0; j < S; ++j) {
       const float wij = w[0][i] * w[1][j];
       const size_t row = ((size_t)(base[0] + i) * N + (size_t)(base[1] + j)) * N;
 #pragma unroll
-      for (int kk = 0; kk < 3; ++kk) {
+      for (int kk = 0; kk < S; ++kk) {
         if (!(ok[0][i] && ok[1][j] && ok[2][kk])) continue;
         const float4 g = grid[row + (size_t)(base[2] + kk)];
         const float wt = wij * w[2][kk];
@@ -159,26 +166,38 @@ __device__ __forceinline__ void gather(const float x[3], const int base[3], cons
         v[0] += wv0;
         v[1] += wv1;
         v[2] += wv2;
-        C[0] += wv0 * dd[0][i]; C[1] += wv0 * dd[1][j]; C[2] += wv0 * dd[2][kk];
-        C[3] += wv1 * dd[0][i]; C[4] += wv1 * dd[1][j]; C[5] += wv1 * dd[2][kk];
-        C[6] += wv2 * dd[0][i]; C[7] += wv2 * dd[1][j]; C[8] += wv2 * dd[2][kk];
+        float r0, r1, r2;  // d^T D^-1
+        if constexpr (Sch::Kernel::d_is_const()) {
+          r0 = dd[0][i];
+          r1 = dd[1][j];
+          r2 = dd[2][kk];
+        } else {
+          float r[3];
+          sch.dinv_rows(dd[0][i], dd[1][j], dd[2][kk], k, r);
+          r0 = r[0];
+          r1 = r[1];
+          r2 = r[2];
+        }
+        C[0] += wv0 * r0; C[1] += wv0 * r1; C[2] += wv0 * r2;
+        C[3] += wv1 * r0; C[4] += wv1 * r1; C[5] += wv1 * r2;
+        C[6] += wv2 * r0; C[7] += wv2 * r1; C[8] += wv2 * r2;
       }
     }
   }
 }
 
-// G2P of one particle already loaded: gather, F update, plasticity, advection.
-// Returns false when the particle is outside the domain (left untouched).
+// G2P of one particle already loaded: gather, F update (g2p_finish_particle),
+// plasticity, advection. Returns false when the particle is outside the
+// domain (left untouched).
+template <class Sch>
 __device__ __forceinline__ bool g2p_update(float x[3], float v[3], float C[9], float F[9],
                                            float& Jp, const MaterialDev& m, const StepConsts& k,
                                            const float4* __restrict__ grid, float R[9],
                                            bool& has_R) {
-  int base[3];
-  float w[3][3];
-  quad_weights(x, k.dx_inv, base, w);
+  Sch sch;
   has_R = false;
-  if (outside_domain(base, k.N)) return false;
-  gather(x, base, w, k, grid, v, C);
+  if (!sch.prepare(x, k)) return false;
+  gather(x, sch, k, grid, v, C);
   // F = (I + dt C) F (TransferScheme.h:141)
   float A[9];
 #pragma unroll
@@ -191,14 +210,13 @@ __device__ __forceinline__ bool g2p_update(float x[3], float v[3], float C[9], f
 }
 
 // P2G of one particle (p2g_prepare_particle + node loop).
+template <class Sch>
 __device__ __forceinline__ void p2g_particle(const float x[3], const float v[3], const float C[9],
                                              const float F[9], float Jp, const MaterialDev& m,
                                              const StepConsts& k, const float* Rin, bool has_R,
                                              float* __restrict__ grid) {
-  int base[3];
-  float w[3][3];
-  quad_weights(x, k.dx_inv, base, w);
-  if (outside_domain(base, k.N)) return;
+  Sch sch;
+  if (!sch.prepare(x, k)) return;
   float R[9];
   if (m.kind != MPM_MATERIAL_WC_LIQUID) {
     if (has_R) {
@@ -210,8 +228,8 @@ __device__ __forceinline__ void p2g_particle(const float x[3], const float v[3],
   }
   float tau[9], aff[9];
   kirchhoff_stress(F, Jp, R, m, k.compat, tau);
-  make_affine(tau, C, m, aff);
-  scatter_direct(x, v, aff, m.mass, base, w, k, grid);
+  sch.affine(tau, C, m, k, aff);
+  scatter_direct(x, v, aff, m.mass, sch, k, grid);
 }
 
 }  // namespace
@@ -225,6 +243,7 @@ __global__ void __launch_bounds__(256) k_fill_zero(float4* __restrict__ g, size_
   for (size_t t = i; t < n; t += step) g[t] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+template <class Sch>
 __global__ void __launch_bounds__(256)
     k_p2g_direct(ParticlesDev P, const MaterialDev* __restrict__ mats, StepConsts k,
                  float* __restrict__ grid) {
@@ -233,7 +252,7 @@ __global__ void __launch_bounds__(256)
   float x[3], v[3], C[9], F[9], Jp;
   load_state(P, i, x, v, C, F, Jp);
   const MaterialDev m = mats[P.mat[i]];
-  p2g_particle(x, v, C, F, Jp, m, k, nullptr, false, grid);
+  p2g_particle<Sch>(x, v, C, F, Jp, m, k, nullptr, false, grid);
 }
 
 // gridOpKernel: normalise, gravity, walls. Optionally zeroes `clear` (the
@@ -255,6 +274,7 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+template <class Sch>
 __global__ void __launch_bounds__(256)
     k_g2p(ParticlesDev P, const MaterialDev* __restrict__ mats, StepConsts k,
           const float4* __restrict__ grid) {
@@ -266,11 +286,12 @@ __global__ void __launch_bounds__(256)
   const MaterialDev m = mats[P.mat[i]];
   float R[9];
   bool has_R;
-  if (!g2p_update(x, v, C, F, Jp, m, k, grid, R, has_R)) return;
+  if (!g2p_update<Sch>(x, v, C, F, Jp, m, k, grid, R, has_R)) return;
   store_state(P, i, x, v, C, F, Jp);
 }
 
 // Fused G2P(n) + P2G(n+1) with direct global atomics.
+template <class Sch>
 __global__ void __launch_bounds__(256)
     k_g2p2g_direct(ParticlesDev P, const MaterialDev* __restrict__ mats, StepConsts k,
                    const float4* __restrict__ grid_in, float* __restrict__ grid_out) {
@@ -282,9 +303,9 @@ __global__ void __launch_bounds__(256)
   float R[9];
   bool has_R = false;
   if (P.id[i] <= k.guard) {
-    if (g2p_update(x, v, C, F, Jp, m, k, grid_in, R, has_R)) store_state(P, i, x, v, C, F, Jp);
+    if (g2p_update<Sch>(x, v, C, F, Jp, m, k, grid_in, R, has_R)) store_state(P, i, x, v, C, F, Jp);
   }
-  p2g_particle(x, v, C, F, Jp, m, k, R, has_R, grid_out);
+  p2g_particle<Sch>(x, v, C, F, Jp, m, k, R, has_R, grid_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -305,10 +326,14 @@ hipError_t fill_zero(float4* g, size_t n, hipStream_t s) {
 }
 
 hipError_t p2g_direct(const ParticlesDev& P, const MaterialDev* mats, const StepConsts& k,
-                      float4* grid, hipStream_t s) {
+                      float4* grid, hipStream_t s, int interp) {
   if (P.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_p2g_direct, dim3((P.n + 255) / 256), dim3(256), 0, s, P, mats, k,
-                     (float*)grid);
+  if (interp == MPM_INTERP_CUBIC)
+    hipLaunchKernelGGL(k_p2g_direct<CubicScheme>, dim3((P.n + 255) / 256), dim3(256), 0, s, P, mats, k,
+                       (float*)grid);
+  else
+    hipLaunchKernelGGL(k_p2g_direct<ReferenceScheme>, dim3((P.n + 255) / 256), dim3(256), 0, s, P, mats, k,
+                       (float*)grid);
   return hipGetLastError();
 }
 
@@ -319,17 +344,24 @@ hipError_t grid_update(float4* g, float4* clear, const StepConsts& k, hipStream_
 }
 
 hipError_t g2p(const ParticlesDev& P, const MaterialDev* mats, const StepConsts& k,
-               const float4* grid, hipStream_t s) {
+               const float4* grid, hipStream_t s, int interp) {
   if (P.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_g2p, dim3((P.n + 255) / 256), dim3(256), 0, s, P, mats, k, grid);
+  if (interp == MPM_INTERP_CUBIC)
+    hipLaunchKernelGGL(k_g2p<CubicScheme>, dim3((P.n + 255) / 256), dim3(256), 0, s, P, mats, k, grid);
+  else
+    hipLaunchKernelGGL(k_g2p<ReferenceScheme>, dim3((P.n + 255) / 256), dim3(256), 0, s, P, mats, k, grid);
   return hipGetLastError();
 }
 
 hipError_t g2p2g_direct(const ParticlesDev& P, const MaterialDev* mats, const StepConsts& k,
-                        const float4* grid_in, float4* grid_out, hipStream_t s) {
+                        const float4* grid_in, float4* grid_out, hipStream_t s, int interp) {
   if (P.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_g2p2g_direct, dim3((P.n + 255) / 256), dim3(256), 0, s, P, mats, k,
-                     grid_in, (float*)grid_out);
+  if (interp == MPM_INTERP_CUBIC)
+    hipLaunchKernelGGL(k_g2p2g_direct<CubicScheme>, dim3((P.n + 255) / 256), dim3(256), 0, s, P, mats, k,
+                       grid_in, (float*)grid_out);
+  else
+    hipLaunchKernelGGL(k_g2p2g_direct<ReferenceScheme>, dim3((P.n + 255) / 256), dim3(256), 0, s, P, mats,
+                       k, grid_in, (float*)grid_out);
   return hipGetLastError();
 }
 

@@ -1,13 +1,19 @@
 """Differentiable rollouts of the MLS-MPM substep on the GPU.
 
 `rollout()` advances a particle state by k substeps of the library's
-differentiable substep (mpm_diff_substep: fixed-corotated with J = det F,
-per-particle Lame parameters, compat FIXED) inside a torch.autograd.Function
-whose backward runs the hand-written adjoint kernels
-(mpm_diff_substep_backward). Memory stays bounded by checkpointing: the
-forward keeps the state every `checkpoint_every` substeps; the backward
-recomputes each segment's states and grid tapes from its checkpoint, then
-runs the segment's adjoints in reverse.
+differentiable substep (fixed-corotated with J = det F, per-particle Lame
+parameters, compat FIXED) inside a torch.autograd.Function whose backward
+runs the hand-written adjoint kernels. Memory stays bounded by
+checkpointing: the forward keeps the state every `checkpoint_every`
+substeps; the backward recomputes each segment's states and grid tapes from
+its checkpoint (mpm_diff_forward_steps), then runs the segment's adjoints in
+reverse (mpm_diff_backward_steps). Each segment is one call into the
+library, which enqueues all of its kernels (two per substep each way).
+
+Inside the Function the particles are ordered by the grid cell of their
+initial position (a stable sort, undone on the way out, gradients included):
+the kernels sum the contributions of particles that share a cell before
+their global atomics, so cell order makes those sums long.
 
 The reference has no gradients; the step itself follows src/mpm.cu:14-178
 with the MMFixedCorotated plugin (MaterialModel.cuh:56-61).
@@ -15,7 +21,7 @@ with the MMFixedCorotated plugin (MaterialModel.cuh:56-61).
 State layout: one float32 tensor of shape (24, n) on the GPU, planes
 x0 x1 x2, v0 v1 v2, C (9, row-major), F (9, row-major). `pack_state` and
 `unpack_state` convert from and to per-field tensors. All work goes on
-torch's current HIP stream. The forward scatters with FP32 atomics, so the
+torch's current HIP stream. The scatters add with FP32 atomics, so a
 recomputed segment can differ from the first forward in the last bits.
 """
 from __future__ import annotations
@@ -83,29 +89,69 @@ class DiffStepper:
         check(lib().mpm_diff_sizes(C.byref(self._cp), C.byref(gf), C.byref(tf)))
         self.grid_floats, self.tape_floats = gf.value, tf.value
         self.grid = torch.empty(self.grid_floats, device=self.device, dtype=torch.float32)
+        self._grid2 = None
+        self._work = None
+
+    def _stream(self):
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
     def new_tape(self) -> torch.Tensor:
         return torch.empty(self.tape_floats, device=self.device, dtype=torch.float32)
+
+    def new_tapes(self, k: int) -> torch.Tensor:
+        return torch.empty((k, self.tape_floats), device=self.device, dtype=torch.float32)
 
     def _check(self, *ts):
         for t in ts:
             if t.device != self.device or t.dtype != torch.float32 or not t.is_contiguous():
                 raise ValueError("expected contiguous float32 tensors on " + str(self.device))
 
+    # one substep (mpm_diff_substep / mpm_diff_substep_backward)
     def forward(self, st, out, mu, lam, tape):
         self._check(st, out, mu, lam, tape)
-        s = torch.cuda.current_stream(self.device).cuda_stream
         check(lib().mpm_diff_substep(C.byref(self._cp), self.mass, self.volume, self.n,
                                      _ptr(st), _ptr(out), _ptr(mu), _ptr(lam), _ptr(self.grid),
-                                     _ptr(tape), C.c_void_p(s)))
+                                     _ptr(tape), self._stream()))
 
     def backward(self, st, mu, lam, tape, adj_out, adj_in, adj_mu, adj_lam):
         self._check(st, mu, lam, tape, adj_out, adj_in, adj_mu, adj_lam)
-        s = torch.cuda.current_stream(self.device).cuda_stream
         check(lib().mpm_diff_substep_backward(
             C.byref(self._cp), self.mass, self.volume, self.n, _ptr(st), _ptr(mu), _ptr(lam),
             _ptr(tape), _ptr(adj_out), _ptr(adj_in), _ptr(adj_mu), _ptr(adj_lam), _ptr(self.grid),
-            C.c_void_p(s)))
+            self._stream()))
+
+    # a run of substeps in one call (mpm_diff_forward_steps / _backward_steps)
+    def forward_steps(self, st0, states, mu, lam, tapes=None):
+        """len(states) substeps from st0: states[j] (a (k, 24, n) tensor) is
+        the state after substep j + 1; with `tapes` ((k, tape_floats)) the
+        grid tape of each substep is kept (else one scratch tape is reused)."""
+        k = states.shape[0]
+        self._check(st0, states, mu, lam)
+        if tapes is None:
+            tapes, stride = self.new_tape(), 0
+        else:
+            self._check(tapes)
+            if tapes.shape[0] < k:
+                raise ValueError("one tape per substep")
+            stride = tapes.shape[1]
+        check(lib().mpm_diff_forward_steps(
+            C.byref(self._cp), self.mass, self.volume, self.n, int(k), _ptr(st0), _ptr(states),
+            PLANES * self.n, _ptr(mu), _ptr(lam), _ptr(self.grid), _ptr(tapes), int(stride),
+            self._stream()))
+
+    def backward_steps(self, states, mu, lam, tapes, adj_out, adj_in, adj_mu, adj_lam):
+        """The adjoint of len(states) substeps: states[j] is the state before
+        substep j, tapes[j] its tape; adj_out the adjoint of the state after
+        the last one. Writes adj_in, adds into adj_mu and adj_lam."""
+        k = states.shape[0]
+        self._check(states, mu, lam, tapes, adj_out, adj_in, adj_mu, adj_lam)
+        if self._grid2 is None:
+            self._grid2 = torch.empty(2 * self.grid_floats, device=self.device, dtype=torch.float32)
+            self._work = torch.empty((PLANES, self.n), device=self.device, dtype=torch.float32)
+        check(lib().mpm_diff_backward_steps(
+            C.byref(self._cp), self.mass, self.volume, self.n, int(k), _ptr(states), PLANES * self.n,
+            _ptr(mu), _ptr(lam), _ptr(tapes), int(tapes.shape[1]), _ptr(adj_out), _ptr(adj_in),
+            _ptr(adj_mu), _ptr(adj_lam), _ptr(self._grid2), _ptr(self._work), self._stream()))
 
 
 class _Rollout(torch.autograd.Function):
@@ -114,14 +160,16 @@ class _Rollout(torch.autograd.Function):
         st0 = st0.detach().contiguous()
         mu = mu.detach().contiguous()
         lam = lam.detach().contiguous()
-        tape = stepper.new_tape()
+        seg = torch.empty((min(every, steps), PLANES, stepper.n), device=st0.device, dtype=torch.float32)
         cur = st0
         checkpoints = [st0]
-        for k in range(steps):
-            nxt = torch.empty_like(cur)
-            stepper.forward(cur, nxt, mu, lam, tape)
-            cur = nxt
-            if (k + 1) % every == 0 and k + 1 < steps:
+        done = 0
+        while done < steps:
+            k = min(every, steps - done)
+            stepper.forward_steps(cur, seg[:k], mu, lam)
+            cur = seg[k - 1].clone()
+            done += k
+            if done < steps:
                 checkpoints.append(cur)
         ctx.stepper, ctx.steps, ctx.every = stepper, steps, every
         ctx.checkpoints = checkpoints
@@ -136,21 +184,31 @@ class _Rollout(torch.autograd.Function):
         adj_next = torch.empty_like(adj)
         adj_mu = torch.zeros_like(mu)
         adj_lam = torch.zeros_like(lam)
-        tapes = [stepper.new_tape() for _ in range(min(every, steps))]
+        k_max = min(every, steps)
+        states = torch.empty((k_max + 1, PLANES, stepper.n), device=adj.device, dtype=torch.float32)
+        tapes = stepper.new_tapes(k_max)
         nseg = len(ctx.checkpoints)
-        for seg in range(nseg - 1, -1, -1):
-            k0 = seg * every
-            k1 = min(steps, k0 + every)
-            states = [ctx.checkpoints[seg]]
-            for j in range(k1 - k0):  # recompute the segment and its tapes
-                nxt = torch.empty_like(states[-1])
-                stepper.forward(states[-1], nxt, mu, lam, tapes[j])
-                states.append(nxt)
-            for j in range(k1 - k0 - 1, -1, -1):
-                stepper.backward(states[j], mu, lam, tapes[j], adj, adj_next, adj_mu, adj_lam)
-                adj, adj_next = adj_next, adj
+        for s in range(nseg - 1, -1, -1):
+            k0 = s * every
+            k = min(steps, k0 + every) - k0
+            # recompute the segment: states[j] is the state before substep j
+            states[0].copy_(ctx.checkpoints[s])
+            stepper.forward_steps(states[0], states[1:k + 1], mu, lam, tapes)
+            stepper.backward_steps(states[:k], mu, lam, tapes, adj, adj_next, adj_mu, adj_lam)
+            adj, adj_next = adj_next, adj
         ctx.checkpoints = None
         return None, None, None, adj, adj_mu, adj_lam
+
+
+def cell_order(x: torch.Tensor, N: int) -> torch.Tensor:
+    """Stable order of the particles (positions x: (3, n)) by the base cell
+    of their quadratic stencil, int(x / dx - 0.5) per axis."""
+    dx_inv = float(torch.tensor(1.0 / float(torch.tensor(1.0 / N, dtype=torch.float32)),
+                                dtype=torch.float32))
+    b = torch.trunc(x * dx_inv - 0.5).to(torch.int64) + 3
+    M = N + 3
+    key = (b[0] * M + b[1]) * M + b[2]
+    return torch.argsort(key, stable=True)
 
 
 def rollout(stepper: DiffStepper, state0: torch.Tensor, mu: torch.Tensor, lam: torch.Tensor,
@@ -161,4 +219,9 @@ def rollout(stepper: DiffStepper, state0: torch.Tensor, mu: torch.Tensor, lam: t
         raise ValueError("steps and checkpoint_every must be >= 1")
     if state0.shape != (PLANES, stepper.n) or mu.shape != (stepper.n,) or lam.shape != (stepper.n,):
         raise ValueError("state0 must be (24, n) and mu, lam (n,)")
-    return _Rollout.apply(stepper, int(steps), int(checkpoint_every), state0, mu, lam)
+    with torch.no_grad():
+        perm = cell_order(state0[0:3].detach(), stepper.params.N)
+        inv = torch.empty_like(perm)
+        inv[perm] = torch.arange(perm.numel(), device=perm.device)
+    out = _Rollout.apply(stepper, int(steps), int(checkpoint_every), state0[:, perm], mu[perm], lam[perm])
+    return out[:, inv]

@@ -161,6 +161,16 @@ class Engine:
         self._chk(lib().mpm_append_particles(self._h, _stream(stream)))
         self._n = self.count()
 
+    def remove(self, first: int, count: int, stream=None) -> np.ndarray:
+        """Remove the device particles with original index in [first, first +
+        count) (mpm_remove_particles); returns their last state as 104-byte
+        records (uint8, (count * 104,)), in index order."""
+        out = np.empty(int(count) * 104, np.uint8)
+        self._chk(lib().mpm_remove_particles(self._h, int(first), int(count),
+                                             out.ctypes.data if count else None, _stream(stream)))
+        self._n = self.count()
+        return out
+
     def mesh(self, grid_side: int = 250, radius: int = 5, stream=None) -> dict:
         """Surface mesh of the device particles (mpm_mesh_build): 'triangles'
         (n, 3, 3) float32 vertices, normals out by the right-hand rule, and
@@ -306,6 +316,17 @@ class Batch:
 
     def advance(self, substeps: int = 1, stream=None) -> None:
         self._chk(lib().mpm_batch_advance(self._b, int(substeps), _stream(stream)))
+
+    def autotune(self, trial_substeps: int = 64, stream=None) -> dict:
+        """Pick the batch's settings by timing candidates through the batch
+        itself (mpm_batch_autotune): every scene's own tuned choice, and common
+        sort intervals and work-item sizes for all scenes, each from a snapshot
+        of every scene that is put back afterwards. Returns the choice and the
+        trials (microseconds per batch substep)."""
+        import json
+        buf = C.create_string_buffer(1 << 16)
+        self._chk(lib().mpm_batch_autotune(self._b, int(trial_substeps), buf, len(buf), _stream(stream)))
+        return json.loads(buf.value.decode())
 
     def set_option(self, name: str, value: int) -> None:
         self._chk(lib().mpm_batch_set_option(self._b, name.encode(), int(value)))

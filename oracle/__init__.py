@@ -77,10 +77,16 @@ def _state_arrays(state, dtype):
     return out
 
 
+INTERP_QUADRATIC = 0
+INTERP_CUBIC = 1
+
+
 def advance(params, materials, state, steps: int, dtype=np.float32, threads: int = 1,
-            return_grid: bool = False):
+            return_grid: bool = False, interp: int = INTERP_QUADRATIC):
     """Run `steps` reference substeps on the CPU; returns the new state dict
-    (and the last grid, shape (N,N,N,4), after its grid update)."""
+    (and the last grid, shape (N,N,N,4), after its grid update). `interp`
+    selects the interpolation kernel (quadratic: the reference's; cubic:
+    generic per-particle D^-1)."""
     s = _state_arrays(state, dtype)
     n = s["x"].shape[0]
     ct = C.c_float if dtype == np.float32 else C.c_double
@@ -91,7 +97,7 @@ def advance(params, materials, state, steps: int, dtype=np.float32, threads: int
     cm = _conv_mats(materials)
     rc = fn(C.byref(cp), cm, len(materials), C.c_int64(n), _ptr(s["x"], ct), _ptr(s["v"], ct),
             _ptr(s["C"], ct), _ptr(s["F"], ct), _ptr(s["Jp"], ct), _ptr(s["material"], C.c_uint8),
-            int(steps), int(threads), _ptr(grid, ct))
+            int(steps), int(threads), _ptr(grid, ct), int(interp))
     if rc != 0:
         raise RuntimeError("oracle advance rejected its arguments")
     return (s, grid) if return_grid else s
@@ -117,7 +123,7 @@ def advance_lame(params, material, state, mu, lam, steps: int, dtype=np.float32,
     return s
 
 
-def p2g(params, materials, state, dtype=np.float32):
+def p2g(params, materials, state, dtype=np.float32, interp: int = INTERP_QUADRATIC):
     s = _state_arrays(state, dtype)
     n = s["x"].shape[0]
     ct = C.c_float if dtype == np.float32 else C.c_double
@@ -127,10 +133,28 @@ def p2g(params, materials, state, dtype=np.float32):
     cp = _conv_params(params)
     rc = fn(C.byref(cp), _conv_mats(materials), len(materials), C.c_int64(n), _ptr(s["x"], ct),
             _ptr(s["v"], ct), _ptr(s["C"], ct), _ptr(s["F"], ct), _ptr(s["Jp"], ct),
-            _ptr(s["material"], C.c_uint8), _ptr(grid, ct))
+            _ptr(s["material"], C.c_uint8), _ptr(grid, ct), int(interp))
     if rc != 0:
         raise RuntimeError("oracle p2g rejected its arguments")
     return grid
+
+
+def kernel(params, x, interp: int = INTERP_QUADRATIC, generic: bool = False):
+    """The selected kernel at positions x (n, 3), in FP64: base (n, 3) int32,
+    weights (n, 3, 4) (entries past the kernel's size are 0) and D^-1 (n, 3, 3)
+    (the generic inverse for the cubic kernel, or for either with generic)."""
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    n = x.shape[0]
+    base = np.empty((n, 3), np.int32)
+    w = np.empty((n, 3, 4), np.float64)
+    D = np.empty((n, 3, 3), np.float64)
+    cp = _conv_params(params)
+    rc = lib().oracle_kernel_f64(C.byref(cp), int(interp), int(bool(generic)), C.c_int64(n),
+                                 _ptr(x, C.c_double), _ptr(base, C.c_int32), _ptr(w, C.c_double),
+                                 _ptr(D, C.c_double))
+    if rc != 0:
+        raise RuntimeError("oracle kernel rejected its arguments")
+    return base, w, D
 
 
 def grid_update(params, grid, dtype=np.float32):

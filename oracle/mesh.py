@@ -39,7 +39,9 @@ def distance_field(x: np.ndarray, N: int, side: int, radius: int) -> np.ndarray:
         nx = ii.astype(np.float32) * vdx                # x_node = i * dx (:139-145)
         ny = jj.astype(np.float32) * vdx
         nz = kk.astype(np.float32) * vdx
-        d2 = ((p[0] - nx)[:, None, None] ** 2 + (p[1] - ny)[None, :, None] ** 2) + (p[2] - nz)[None, None, :] ** 2
+        # (particle.x - x_node).norm() (:144): Eigen sums the squares of a
+        # 3-vector as e0 + (e1 + e2)
+        d2 = (p[0] - nx)[:, None, None] ** 2 + ((p[1] - ny)[None, :, None] ** 2 + (p[2] - nz)[None, None, :] ** 2)
         d = np.sqrt(d2.astype(np.float32))
         sub = g[beg[0]:end[0], beg[1]:end[1], beg[2]:end[2]]
         np.minimum(sub, d, out=sub)                     # grid = min(d, grid) (:146)

@@ -407,16 +407,23 @@ template <class S> struct Particle {
   uint8_t mat;
 };
 
+// Interpolation kernels (the InterpolationKernel plugin, InterpolationKernel.cuh):
+// the reference's quadratic B-spline with its constant D^-1, and a cubic
+// B-spline whose D^-1 goes through the generic per-particle inverse.
+enum Interp : int { INTERP_QUADRATIC = 0, INTERP_CUBIC = 1 };
+
 template <class S> struct StepConsts {
   S dt, dx, dx_inv, dinv, gravity, bfrac;
   uint32_t N;
   int boundary_mode;
   int compat;
   int bcells;
+  int interp;  // Interp
 };
 
-template <class S> inline StepConsts<S> make_consts(const mpm_params& p) {
+template <class S> inline StepConsts<S> make_consts(const mpm_params& p, int interp = INTERP_QUADRATIC) {
   StepConsts<S> k;
+  k.interp = interp;
   k.N = p.N;
   k.dt = S(p.dt);
   // TransferScheme.h:26-27: dx = 1.0/N (double) stored as real; dx_inv = 1.0/dx
@@ -447,6 +454,89 @@ template <class S> inline void quad_weights(const S x[3], S dx_inv, int base[3],
     w[a][1] = S(0.75) - t1 * t1;
     w[a][2] = S(0.5) * (t2 * t2);
   }
+}
+
+// Cubic B-spline N(t), t = |node - particle| / dx: (1/2) t^3 - t^2 + 2/3 on
+// [0, 1), (2 - t)^3 / 6 on [1, 2), 0 beyond.
+template <class S> inline S cubic_bspline(S t) {
+  if (t < S(1)) {
+    const S t2 = t * t;
+    return ((S(0.5) * t2) * t - t2) + S(2.0 / 3.0);
+  }
+  if (t < S(2)) {
+    const S u = S(2) - t;
+    return ((u * u) * u) * S(1.0 / 6.0);
+  }
+  return S(0);
+}
+
+// Cubic kernel weights over 4 nodes per direction: base = int(x/dx - 1)
+// (truncated toward zero, as the quadratic kernel's .cast<int>()), node o at
+// distance |x/dx - base - o|.
+template <class S> inline void cubic_weights(const S x[3], S dx_inv, int base[3], S w[3][4]) {
+  for (int a = 0; a < 3; ++a) {
+    const S xg = x[a] * dx_inv;
+    base[a] = (int)(xg - S(1.0));
+    const S fx = xg - S(base[a]);
+    for (int o = 0; o < 4; ++o) w[a][o] = cubic_bspline<S>(std::fabs(fx - S(o)));
+  }
+}
+
+inline int kernel_size(int interp) { return interp == INTERP_CUBIC ? 4 : 3; }
+
+// InterpolationKernel::weights_per_direction of the selected kernel; w[a][o]
+// for o < kernel_size.
+template <class S>
+inline void kernel_weights(const StepConsts<S>& k, const S x[3], int base[3], S w[3][4]) {
+  if (k.interp == INTERP_CUBIC) {
+    cubic_weights<S>(x, k.dx_inv, base, w);
+    return;
+  }
+  S q[3][3];
+  quad_weights<S>(x, k.dx_inv, base, q);
+  for (int a = 0; a < 3; ++a) {
+    for (int o = 0; o < 3; ++o) w[a][o] = q[a][o];
+    w[a][3] = S(0);
+  }
+}
+
+// Inverse of a 3x3 matrix by its adjugate: cofactors of column 0 give the
+// determinant, every entry is cofactor / det.
+template <class S> inline void inverse3(const S m[3][3], S out[3][3]) {
+  S cof[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+      cof[r][c] = m[r1][c1] * m[r2][c2] - m[r1][c2] * m[r2][c1];
+    }
+  const S det = dot3<S>(cof[0][0], m[0][0], cof[1][0], m[1][0], cof[2][0], m[2][0]);
+  const S inv = S(1) / det;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) out[r][c] = cof[c][r] * inv;
+}
+
+// InterpolationKernelBase::D_inv (InterpolationKernel.cuh:21-50): D = sum over
+// the stencil of w_i w_j w_k (x_node - x_p)(x_node - x_p)^T, inverted. The
+// reference evaluates it when the kernel's d_is_const() is false.
+template <class S>
+inline void generic_dinv(const S x[3], const int base[3], const S w[3][4], int ns, S dx, S Dinv[3][3]) {
+  S D[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  S d[3];
+  for (int i = 0; i < ns; ++i) {
+    d[0] = S(base[0] + i) * dx - x[0];
+    for (int j = 0; j < ns; ++j) {
+      d[1] = S(base[1] + j) * dx - x[1];
+      for (int kk = 0; kk < ns; ++kk) {
+        d[2] = S(base[2] + kk) * dx - x[2];
+        const S wt = (w[0][i] * w[1][j]) * w[2][kk];
+        for (int r = 0; r < 3; ++r) {
+          const S wd = wt * d[r];
+          for (int c = 0; c < 3; ++c) D[r][c] = D[r][c] + wd * d[c];
+        }
+      }
+    }
+  }
+  inverse3<S>(D, Dinv);
 }
 
 // The Kirchhoff stress the MaterialModel plugin returns (computePF).
@@ -487,13 +577,27 @@ inline void material_stress(const Particle<S>& p, const mpm_material& m, int com
 
 // Per-particle P2G preparation (MLS_APIC_Scheme::p2g_prepare_particle,
 // TransferScheme.h:66-86): affine = -Dinv dt V PF + m C.
+// Dinv: the particle's D^-1 when the kernel's is not constant (nullptr: the
+// quadratic kernel's constant k.dinv I).
 template <class S>
 inline void p2g_affine(const Particle<S>& p, const mpm_material& m, const StepConsts<S>& k,
-                       S affine[3][3], const S* lame = nullptr) {
+                       S affine[3][3], const S* lame = nullptr, const S (*Dinv)[3] = nullptr) {
   S R[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
   if (m.kind != MPM_MATERIAL_WC_LIQUID) polar_R<S>(p.F, R);
   S PF[3][3];
   material_stress<S>(p, m, k.compat, R, PF, lame);
+  if (Dinv) {
+    // stress = ((-Dinv) dt V) PF, a full matrix product (TransferScheme.h:83)
+    S sD[3][3];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) sD[r][c] = ((-Dinv[r][c]) * k.dt) * S(m.particle_volume);
+    S st[3][3];
+    matmul<S>(sD, PF, st);
+    const S mass = S(m.particle_mass);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) affine[r][c] = st[r][c] + mass * p.C[r][c];
+    return;
+  }
   // stress = ((-Dinv) * dt) * V * PF with diagonal Dinv
   const S d = ((-k.dinv) * k.dt) * S(m.particle_volume);
   const S mass = S(m.particle_mass);
@@ -506,14 +610,21 @@ template <class S, class Add>
 inline void p2g_particle(const Particle<S>& p, const mpm_material& m, const StepConsts<S>& k,
                          Add&& add, const S* lame = nullptr) {
   const uint32_t N = k.N;
+  const int ns = kernel_size(k.interp);
   int base[3];
-  S w[3][3];
-  quad_weights<S>(p.x, k.dx_inv, base, w);
+  S w[3][4];
+  kernel_weights<S>(k, p.x, base, w);
   // out-of-domain particles are skipped (mpm.cu:31-35)
   for (int a = 0; a < 3; ++a)
-    if (base[a] + 3 < 0 || base[a] >= (int)N) return;
+    if (base[a] + ns < 0 || base[a] >= (int)N) return;
   S affine[3][3];
-  p2g_affine<S>(p, m, k, affine, lame);
+  if (k.interp == INTERP_QUADRATIC) {
+    p2g_affine<S>(p, m, k, affine, lame);
+  } else {
+    S Dinv[3][3];
+    generic_dinv<S>(p.x, base, w, ns, k.dx, Dinv);
+    p2g_affine<S>(p, m, k, affine, lame, Dinv);
+  }
   const S mass = S(m.particle_mass);
   int beg[3], end[3];
   for (int a = 0; a < 3; ++a) {
@@ -521,7 +632,7 @@ inline void p2g_particle(const Particle<S>& p, const mpm_material& m, const Step
     // mpm.cu:42-44 bounds the upper end by N + base (compat REFERENCE); the
     // correct bound, used by G2P (mpm.cu:140-142) and compat FIXED, is N - base
     const int lim = (k.compat == MPM_COMPAT_FIXED) ? (int)N - base[a] : (int)N + base[a];
-    end[a] = std::min(3, lim);
+    end[a] = std::min(ns, lim);
   }
   for (int i = beg[0]; i < end[0]; ++i) {
     const uint32_t ig = (uint32_t)(base[0] + i);
@@ -617,17 +728,21 @@ template <class S>
 inline void g2p_particle(Particle<S>& p, const mpm_material& m, const StepConsts<S>& k,
                          const S* grid) {
   const uint32_t N = k.N;
+  const int ns = kernel_size(k.interp);
   int base[3];
-  S w[3][3];
-  quad_weights<S>(p.x, k.dx_inv, base, w);
+  S w[3][4];
+  kernel_weights<S>(k, p.x, base, w);
   for (int a = 0; a < 3; ++a)
-    if (base[a] + 3 < 0 || base[a] >= (int)N) return;  // mpm.cu:129-133: left untouched
+    if (base[a] + ns < 0 || base[a] >= (int)N) return;  // mpm.cu:129-133: left untouched
+  S Dinv[3][3];
+  const bool generic = k.interp != INTERP_QUADRATIC;
+  if (generic) generic_dinv<S>(p.x, base, w, ns, k.dx, Dinv);
   S v[3] = {0, 0, 0};
   S C[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   int beg[3], end[3];
   for (int a = 0; a < 3; ++a) {
     beg[a] = std::max(0, -base[a]);
-    end[a] = std::min(3, (int)N - base[a]);
+    end[a] = std::min(ns, (int)N - base[a]);
   }
   for (int i = beg[0]; i < end[0]; ++i) {
     const uint32_t ig = (uint32_t)(base[0] + i);
@@ -641,7 +756,9 @@ inline void g2p_particle(Particle<S>& p, const mpm_material& m, const StepConsts
         const S* cell = grid + 4 * ((size_t)N * N * ig + (size_t)N * jg + kg);
         const S wt = (w[0][i] * w[1][j]) * w[2][kk];
         // TransferScheme.h:128,131: v += w vg; C += (w vg)(d^T Dinv)
-        const S dd[3] = {d0 * k.dinv, d1 * k.dinv, d2 * k.dinv};
+        S dd[3] = {d0 * k.dinv, d1 * k.dinv, d2 * k.dinv};
+        if (generic)
+          for (int c = 0; c < 3; ++c) dd[c] = dot3<S>(d0, Dinv[0][c], d1, Dinv[1][c], d2, Dinv[2][c]);
         for (int r = 0; r < 3; ++r) {
           const S wv = wt * cell[r];
           v[r] = v[r] + wv;

@@ -51,13 +51,14 @@ int check_mats(const uint8_t* mat, int64_t n, int nmat) {
 
 template <class S>
 int advance(const mpm_params* prm, const mpm_material* mats, int nmat, int64_t n, S* x, S* v, S* C,
-            S* F, S* Jp, const uint8_t* mat, int steps, int threads, S* grid_out) {
+            S* F, S* Jp, const uint8_t* mat, int steps, int threads, S* grid_out, int interp) {
   if (!prm || (n > 0 && (!x || !v || !C || !F || !Jp || !mat)) || prm->N < 4 || prm->N > 1024)
     return 1;
+  if (interp != INTERP_QUADRATIC && interp != INTERP_CUBIC) return 1;
   if (check_mats<S>(mat, n, nmat)) return 1;
   std::vector<Particle<S>> ps;
   load<S>(ps, n, x, v, C, F, Jp, mat);
-  const StepConsts<S> k = make_consts<S>(*prm);
+  const StepConsts<S> k = make_consts<S>(*prm, interp);
   std::vector<S> grid;
   for (int s = 0; s < steps; ++s) step<S>(ps, mats, nmat, k, grid, threads);
   store<S>(ps, x, v, C, F, Jp);
@@ -85,12 +86,14 @@ int advance_lame(const mpm_params* prm, const mpm_material* m, int64_t n, S* x, 
 
 template <class S>
 int p2g_only(const mpm_params* prm, const mpm_material* mats, int nmat, int64_t n, const S* x,
-             const S* v, const S* C, const S* F, const S* Jp, const uint8_t* mat, S* grid_out) {
+             const S* v, const S* C, const S* F, const S* Jp, const uint8_t* mat, S* grid_out,
+             int interp) {
   if (!prm || !grid_out || prm->N < 4 || prm->N > 1024) return 1;
+  if (interp != INTERP_QUADRATIC && interp != INTERP_CUBIC) return 1;
   if (check_mats<S>(mat, n, nmat)) return 1;
   std::vector<Particle<S>> ps;
   load<S>(ps, n, x, v, C, F, Jp, mat);
-  const StepConsts<S> k = make_consts<S>(*prm);
+  const StepConsts<S> k = make_consts<S>(*prm, interp);
   const size_t nodes = (size_t)prm->N * prm->N * prm->N;
   for (size_t i = 0; i < 4 * nodes; ++i) grid_out[i] = S(0);
   for (int64_t pi = 0; pi < n; ++pi)
@@ -178,26 +181,53 @@ int oracle_plasticity_f64(const mpm_material* m, int64_t n, double* F, double* J
 
 int oracle_p2g_f32(const mpm_params* p, const mpm_material* m, int nm, int64_t n, const float* x,
                    const float* v, const float* C, const float* F, const float* Jp, const uint8_t* mat,
-                   float* grid) {
-  return p2g_only<float>(p, m, nm, n, x, v, C, F, Jp, mat, grid);
+                   float* grid, int interp) {
+  return p2g_only<float>(p, m, nm, n, x, v, C, F, Jp, mat, grid, interp);
 }
 int oracle_p2g_f64(const mpm_params* p, const mpm_material* m, int nm, int64_t n, const double* x,
                    const double* v, const double* C, const double* F, const double* Jp,
-                   const uint8_t* mat, double* grid) {
-  return p2g_only<double>(p, m, nm, n, x, v, C, F, Jp, mat, grid);
+                   const uint8_t* mat, double* grid, int interp) {
+  return p2g_only<double>(p, m, nm, n, x, v, C, F, Jp, mat, grid, interp);
 }
 int oracle_grid_update_f32(const mpm_params* p, float* grid) { return grid_update<float>(p, grid); }
 int oracle_grid_update_f64(const mpm_params* p, double* grid) { return grid_update<double>(p, grid); }
 
 int oracle_advance_f32(const mpm_params* p, const mpm_material* m, int nm, int64_t n, float* x, float* v,
                        float* C, float* F, float* Jp, const uint8_t* mat, int steps, int threads,
-                       float* grid_out) {
-  return advance<float>(p, m, nm, n, x, v, C, F, Jp, mat, steps, threads, grid_out);
+                       float* grid_out, int interp) {
+  return advance<float>(p, m, nm, n, x, v, C, F, Jp, mat, steps, threads, grid_out, interp);
 }
 int oracle_advance_f64(const mpm_params* p, const mpm_material* m, int nm, int64_t n, double* x,
                        double* v, double* C, double* F, double* Jp, const uint8_t* mat, int steps,
-                       int threads, double* grid_out) {
-  return advance<double>(p, m, nm, n, x, v, C, F, Jp, mat, steps, threads, grid_out);
+                       int threads, double* grid_out, int interp) {
+  return advance<double>(p, m, nm, n, x, v, C, F, Jp, mat, steps, threads, grid_out, interp);
+}
+
+// Weights and D^-1 of the selected kernel at n positions (x[n][3]): base[n][3],
+// w[n][3][4] (entries past the kernel's size are 0), Dinv[n][3][3] (generic
+// inverse for the cubic kernel, the constant 4/dx^2 I for the quadratic one,
+// or the generic inverse of either with generic = 1).
+int oracle_kernel_f64(const mpm_params* p, int interp, int generic, int64_t n, const double* x,
+                      int32_t* base, double* w, double* Dinv) {
+  if (!p || p->N < 4 || p->N > 1024 || (interp != INTERP_QUADRATIC && interp != INTERP_CUBIC)) return 1;
+  const StepConsts<double> k = make_consts<double>(*p, interp);
+  for (int64_t i = 0; i < n; ++i) {
+    int b[3];
+    double ww[3][4], D[3][3];
+    kernel_weights<double>(k, x + 3 * i, b, ww);
+    if (generic || interp != INTERP_QUADRATIC) {
+      generic_dinv<double>(x + 3 * i, b, ww, kernel_size(interp), k.dx, D);
+    } else {
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) D[r][c] = (r == c) ? k.dinv : 0.0;
+    }
+    for (int a = 0; a < 3; ++a) {
+      base[3 * i + a] = b[a];
+      for (int o = 0; o < 4; ++o) w[12 * i + 4 * a + o] = ww[a][o];
+      for (int c = 0; c < 3; ++c) Dinv[9 * i + 3 * a + c] = D[a][c];
+    }
+  }
+  return 0;
 }
 
 int oracle_advance_lame_f32(const mpm_params* p, const mpm_material* m, int64_t n, float* x, float* v,

@@ -20,10 +20,10 @@
 extern "C" {
 int oracle_advance_f32(const mpm_params* p, const mpm_material* m, int nm, int64_t n, float* x, float* v,
                        float* C, float* F, float* Jp, const uint8_t* mat, int steps, int threads,
-                       float* grid_out);
+                       float* grid_out, int interp);
 int oracle_advance_f64(const mpm_params* p, const mpm_material* m, int nm, int64_t n, double* x,
                        double* v, double* C, double* F, double* Jp, const uint8_t* mat, int steps,
-                       int threads, double* grid_out);
+                       int threads, double* grid_out, int interp);
 int oracle_svd_f32(int64_t n, const float* A, float* U, float* s, float* V);
 }
 
@@ -39,7 +39,8 @@ static int failures = 0;
 
 template <typename T>
 static void run_oracle(const std::vector<mpm_material>& mats, const std::vector<float>& x32,
-                       const std::vector<float>& v32, const std::vector<uint8_t>& mat, int threads) {
+                       const std::vector<float>& v32, const std::vector<uint8_t>& mat, int threads,
+                       int interp) {
   const int64_t n = (int64_t)mat.size();
   std::vector<T> x(x32.begin(), x32.end()), v(v32.begin(), v32.end());
   std::vector<T> C(9 * n, T(0)), F(9 * n, T(0)), Jp(n, T(1));
@@ -55,10 +56,10 @@ static void run_oracle(const std::vector<mpm_material>& mats, const std::vector<
   int rc;
   if constexpr (sizeof(T) == 4)
     rc = oracle_advance_f32(&p, mats.data(), (int)mats.size(), n, x.data(), v.data(), C.data(), F.data(),
-                            Jp.data(), mat.data(), 3, threads, grid.data());
+                            Jp.data(), mat.data(), 3, threads, grid.data(), interp);
   else
     rc = oracle_advance_f64(&p, mats.data(), (int)mats.size(), n, x.data(), v.data(), C.data(), F.data(),
-                            Jp.data(), mat.data(), 3, threads, grid.data());
+                            Jp.data(), mat.data(), 3, threads, grid.data(), interp);
   EXPECT(rc == 0, "oracle advance (%zu-byte floats) returned %d", sizeof(T), rc);
   bool finite = true;
   for (const T& a : x) finite = finite && std::isfinite((double)a);
@@ -98,9 +99,11 @@ int main(int argc, char** argv) {
     std::vector<float> x(3 * np), v(3 * np);
     std::vector<uint8_t> mat(np);
     EXPECT(mpm_scene_particles_soa(sc, x.data(), v.data(), mat.data()) == 0, "scene particles");
-    run_oracle<float>(mats, x, v, mat, 1);
-    run_oracle<float>(mats, x, v, mat, 2);
-    run_oracle<double>(mats, x, v, mat, 1);
+    for (int interp = 0; interp < 2; ++interp) {  // quadratic, cubic
+      run_oracle<float>(mats, x, v, mat, 1, interp);
+      run_oracle<float>(mats, x, v, mat, 2, interp);
+      run_oracle<double>(mats, x, v, mat, 1, interp);
+    }
     EXPECT(mpm_scene_destroy(sc) == 0, "scene destroy");
   }
   // ---- the sampler on the good mesh, twice on one stream

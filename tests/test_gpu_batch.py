@@ -43,7 +43,10 @@ def run_alone(scs, k):
 @pytest.mark.parametrize("graph,fuse", [(1, 1), (1, 0), (0, 1)])
 def test_batch_equals_individual_runs(gpu_lib, graph, fuse):
     scs = small_batch()
-    k = 100  # 16 eager substeps, two recorded cycles of 32, 20 eager
+    # 16 eager substeps, two recorded cycles of 32, 20 eager; with "fuse" the
+    # eager substeps after the first (which primes each scene) run as
+    # multi-scene launches too
+    k = 100
     ref = run_alone(scs, k)
     engines = [make(sc) for sc in scs]
     b = Batch(engines)
@@ -53,8 +56,15 @@ def test_batch_equals_individual_runs(gpu_lib, graph, fuse):
     outs = [e.download() for e in engines]
     launches = b.get_option("graph_launches")
     fused = b.get_option("fused_captures")
+    eager = b.get_option("fused_eager_substeps")
     b.close()
     assert fused == (1 if graph and fuse else 0)
+    if not fuse:
+        assert eager == 0
+    elif graph:
+        assert 16 + 20 - 2 <= eager <= 16 + 20 - 1, eager
+    else:
+        assert k - 2 <= eager <= k - 1, eager
     for e, r in zip(engines, ref):
         assert e.time() == r["t"]
     for o, r in zip(outs, ref):
@@ -216,3 +226,70 @@ def test_full_batch_against_oracle(gpu_lib):
         tol = {0: 1e-4, 1: 1e-3, 2: 1e-4}[kind]
         compare_with_oracle(sc, sc.state(), outs[j], k, tol, fp32_noise=kind != 0, threads=threads,
                             label=f"{sc.name} ({sc.n} particles) in the batch")
+
+
+def test_batch_with_different_sort_intervals(gpu_lib):
+    """Scenes sorting every 16, 24 and 8 substeps share one recorded cycle (the
+    least common multiple of their two-interval cycles, 96 substeps) of
+    multi-scene launches; each equals its own run bit for bit."""
+    scs = small_batch()[:3]
+    every = (16, 24, 8)
+    k = 48 + 96 + 20  # to the first common cycle start, one cycle, 20 more
+
+    def with_se(sc, se):
+        e = make(sc)
+        e.set_option("sort_every", se)
+        return e
+
+    ref = []
+    for sc, se in zip(scs, every):
+        e = with_se(sc, se)
+        e.advance(k)
+        ref.append(e.download())
+        e.close()
+    engines = [with_se(sc, se) for sc, se in zip(scs, every)]
+    b = Batch(engines)
+    b.advance(k)
+    outs = [e.download() for e in engines]
+    assert b.get_option("fused_captures") == 1 and b.get_option("graph_launches") == 1
+    b.close()
+    for o, r in zip(outs, ref):
+        for f in ("x", "v", "C", "F", "Jp"):
+            assert np.array_equal(o[f], r[f]), f
+    for e in engines:
+        e.close()
+
+
+def test_batch_autotune_restores_and_keeps_bits(gpu_lib):
+    """The batch tuner times its candidates from a snapshot of every scene and
+    puts the scenes back (same particles, time 0); then the batch under the
+    chosen settings equals each scene run alone with the same settings."""
+    scs = small_batch()
+    engines = [make(sc) for sc in scs]
+    b = Batch(engines)
+    choice = b.autotune(trial_substeps=64)
+    print("batch tuner:", choice)
+    assert choice["choice"] == "own" or set(choice["choice"]) == {"sort_every", "chunk"}
+    assert len(choice["trials"]) >= 2
+    for e, sc in zip(engines, scs):
+        assert e.time() == 0.0
+        st = sc.state()
+        d = e.download()
+        for f in ("x", "v", "C", "F", "Jp"):
+            assert np.array_equal(d[f], st[f].astype(np.float32)), f
+    opts = [{k: e.get_option(k) for k in ("sort_every", "chunk", "radix_bits")} for e in engines]
+    k = 150
+    b.advance(k)
+    outs = [e.download() for e in engines]
+    b.close()
+    for e in engines:
+        e.close()
+    for sc, o, out in zip(scs, opts, outs):
+        e = make(sc)
+        for name, v in o.items():
+            e.set_option(name, v)
+        e.advance(k)
+        r = e.download()
+        e.close()
+        for f in ("x", "v", "C", "F", "Jp"):
+            assert np.array_equal(out[f], r[f]), (sc.name, f)

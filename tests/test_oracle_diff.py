@@ -103,3 +103,84 @@ def test_diff_cube_workload():
     assert c.x.min() > 0.35 - 0.3 / 128 - 1e-6 and c.x.max() < 0.65 + 0.3 / 128 + 1e-6
     assert abs(float(c.logE.mean()) - np.log(1e4)) < 0.01
     assert c.params.compat == COMPAT_FIXED and c.params.N == 32 and c.substeps == 200
+
+
+# ---------------------------------------------------------------- FP64 reference gradients
+def dropped_cube(side):
+    # a jittered cube just above the floor walls, so the stiffness matters
+    return scenes.diff_cube(side=side, lo=(0.45, 0.08, 0.45), spacing=0.009375)
+
+
+def oracle_loss(c, x0, v0, logE, steps):
+    E = np.exp(np.asarray(logE, np.float64))
+    mu, lam = E / (2.0 * (1.0 + c.nu)), E * c.nu / ((1.0 + c.nu) * (1.0 - 2.0 * c.nu))
+    n = x0.shape[0]
+    st = {"x": np.asarray(x0, np.float64), "v": np.tile(np.asarray(v0, np.float64), (n, 1)),
+          "C": np.zeros((n, 3, 3)), "F": np.tile(np.eye(3), (n, 1, 1)), "Jp": np.ones(n),
+          "material": np.zeros(n, np.uint8)}
+    m = material_as_numpy(MATERIAL_FIXED_COROTATED, c.particle_volume, 1000.0, E=1e4, nu=c.nu)
+    m.particle_mass = c.particle_mass
+    m.particle_volume = c.particle_volume
+    out = oracle.advance_lame(c.params, m, st, mu, lam, steps, dtype=np.float64)
+    return out, float(((out["x"].mean(0) - np.asarray(c.target)) ** 2).sum())
+
+
+def test_torch_fp64_forward_matches_oracle(oracle_lib):
+    """The autograd restatement computes the C++ FP64 oracle's step (a cube
+    that reaches the floor walls within the 30 substeps), to FP64 roundoff."""
+    torch = pytest.importorskip("torch")
+    from oracle import diff_torch
+
+    c = dropped_cube(6)
+    v0 = np.array([0.3, -1.5, 0.2])
+    steps = 30
+    n = c.x.shape[0]
+    E = np.exp(c.logE.astype(np.float64))
+    mu, lam = E / 2.6, E * 0.3 / (1.3 * 0.4)
+    T = lambda a: torch.tensor(np.asarray(a, np.float64))
+    xT, vT, CT, FT = diff_torch.rollout(T(c.x), T(np.tile(v0, (n, 1))), torch.zeros((n, 3, 3), dtype=torch.float64),
+                                        torch.eye(3, dtype=torch.float64).expand(n, 3, 3).clone(), T(mu), T(lam),
+                                        c.particle_mass, c.particle_volume, c.params, steps)
+    ref, _ = oracle_loss(c, c.x, v0, c.logE, steps)
+    got = {"x": xT, "v": vT, "C": CT, "F": FT}
+    for k in ("x", "v", "C", "F"):
+        a = got[k].detach().numpy()
+        err = np.abs(a - ref[k]).max() / np.abs(ref[k]).max()
+        # FP64 roundoff (C carries D^-1 = 4 / dx^2); a wrong term shows at 1e-3 or more
+        assert err < 1e-9, (k, err)
+
+
+def test_torch_fp64_gradients_match_central_differences(oracle_lib):
+    """dL/dv0, dL/dlogE and dL/dx0 of the autograd restatement against FP64
+    central differences of the C++ oracle (eps 1e-3): the v0 components and
+    the directional derivatives along random log E and x0 directions within
+    1e-5 of the gradient's norm."""
+    pytest.importorskip("torch")
+    from oracle import diff_torch
+
+    c = dropped_cube(5)
+    v0 = np.array([0.3, -3.0, 0.2])  # reaches the floor walls, so the stiffness matters
+    steps = 80
+    loss, g_v0, g_logE, g_x0 = diff_torch.com_loss_gradients(c.x, c.logE, v0, c.nu, c.particle_mass,
+                                                            c.particle_volume, c.params, steps, c.target,
+                                                            every=15)
+    _, l_ref = oracle_loss(c, c.x, v0, c.logE, steps)
+    assert abs(loss - l_ref) < 1e-12 * l_ref
+    eps = 1e-3
+    logE = c.logE.astype(np.float64)
+    x0 = c.x.astype(np.float64)
+    L = lambda x, v, le: oracle_loss(c, x, v, le, steps)[1]
+    fd_v0 = np.array([(L(x0, v0 + eps * e, logE) - L(x0, v0 - eps * e, logE)) / (2 * eps) for e in np.eye(3)])
+    assert np.abs(g_v0 - fd_v0).max() < 1e-5 * np.linalg.norm(fd_v0), (g_v0, fd_v0)
+    rng = np.random.default_rng(2)
+    for _ in range(2):
+        d = rng.standard_normal(logE.shape[0])
+        d /= np.linalg.norm(d)
+        fd = (L(x0, v0, logE + eps * d) - L(x0, v0, logE - eps * d)) / (2 * eps)
+        assert abs(g_logE @ d - fd) < 1e-5 * np.linalg.norm(g_logE), (g_logE @ d, fd)
+    d = rng.standard_normal(x0.shape)
+    d /= np.linalg.norm(d)
+    fd = (L(x0 + eps * d, v0, logE) - L(x0 - eps * d, v0, logE)) / (2 * eps)
+    assert abs((g_x0 * d).sum() - fd) < 1e-5 * np.linalg.norm(g_x0), ((g_x0 * d).sum(), fd)
+    # the stiffness matters here (the cube lands on the floor)
+    assert np.linalg.norm(g_logE) > 1e-4 * np.linalg.norm(g_v0)

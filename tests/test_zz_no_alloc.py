@@ -66,13 +66,17 @@ def run_both(c, dev):
     assert rc == 0, lib.mpm_last_error(None)
     torch.cuda.synchronize()
     e_state = max(rel(out_b[q], out_a[q]) for q in range(24))
-    ta = tape_a.view(-1, 8)
-    tb = tape_b.view(-1, 8)
+    # the tape: N^3 (velocity, mass) after the update, then N^3 (velocity
+    # before the walls, wall flags); as 8 columns per node
+    nodes = c.params.N ** 3
+    ta = torch.cat([tape_a[:4 * nodes].view(-1, 4), tape_a[4 * nodes:].view(-1, 4)], 1)
+    tb = torch.cat([tape_b[:4 * nodes].view(-1, 4), tape_b[4 * nodes:].view(-1, 4)], 1)
     e_tape = max(rel(tb[:, q], ta[:, q]) for q in range(8) if float(ta[:, q].abs().max()) > 0)
     print(f"N={c.params.N} n={n}: state {e_state:.2e} tape {e_tape:.2e}")
     assert e_state < 1e-5 and e_tape < 1e-5, (e_state, e_tape)
-    # the flags column is exact
+    # the flags column is exact, and so is which nodes hold mass
     assert torch.equal(ta[:, 7], tb[:, 7]) and torch.equal(ta[:, 3] > 0, tb[:, 3] > 0)
+    assert bool((ta[:, 3] > 0).any())
 
 
 def test_noalloc_matches_workspace_entry_point(gpu_lib):
