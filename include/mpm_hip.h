@@ -383,7 +383,7 @@ int mpm_debug_glibc_rand(uint32_t seed, int32_t n, int32_t* out);
  * p at state[q*n + p]. params->compat must be MPM_COMPAT_FIXED.
  * Scratch sizes (floats) come from mpm_diff_sizes: `grid` (4 N^3) is scratch
  * of one call; `tape` (8 N^3) is written by the forward and read by the
- * backward of the same substep. */
+ * backward of the same substep. Grids and tapes must be 16-byte aligned. */
 int mpm_diff_sizes(const mpm_params* params, int64_t* grid_floats, int64_t* tape_floats);
 /* state_out = substep(state); state_out must not alias state. */
 int mpm_diff_substep(const mpm_params* params, float particle_mass, float particle_volume,
@@ -414,7 +414,7 @@ int mpm_diff_substep_backward(const mpm_params* params, float particle_mass,
  * mpm_diff_substep, two launches per substep, all enqueued here. From
  * state0, the state after substep j+1 goes to states + j*state_stride
  * (state_stride >= 24 n floats; state0 must not lie in that output) and its
- * grid tape to tapes + j*tape_stride (tape_stride >= 8 N^3 floats, or 0:
+ * grid tape to tapes + j*tape_stride (tape_stride a multiple of 4 >= 8 N^3 floats, or 0:
  * every substep writes the same tape, when no backward follows). grid: 4 N^3
  * floats of scratch. No reference counterpart. */
 int mpm_diff_forward_steps(const mpm_params* params, float particle_mass, float particle_volume,
@@ -423,7 +423,8 @@ int mpm_diff_forward_steps(const mpm_params* params, float particle_mass, float 
                            float* tapes, int64_t tape_stride, void* stream);
 /* The adjoint of a run of `steps` substeps: states + j*state_stride is the
  * state before substep j (j = 0..steps-1) and tapes + j*tape_stride its tape,
- * as mpm_diff_forward_steps wrote them; adj_out is dL/d(state after the last
+ * as mpm_diff_forward_steps wrote them (tape_stride a multiple of 4 >= 8 N^3
+ * floats when steps > 1); adj_out is dL/d(state after the last
  * substep). Writes adj_in = dL/d(state before substep 0) and ADDS dL/dmu,
  * dL/dlambda into adj_mu[n], adj_lambda[n]. Scratch: grid 8 N^3 floats, work
  * 24 n floats (adj_in may be work; neither may alias adj_out). Two launches

@@ -30,13 +30,13 @@
 // Fbar += 2 U N V^T, N_ij = M_ij / (s_i + s_j).
 //
 // Scatters (P2G: 27 nodes x 4 values; the grid adjoint of G2P: 27 x 3): each
-// lane writes its particle's values into a row of an LDS table; the lanes
-// then take (run, value) pairs, where a run is a stretch of lanes whose
-// particles share a base cell (particles sorted by cell make long runs:
-// ~37 per cell for a dense lattice), sum the run's column in lane order and
-// add the sum to the grid with one FP32 atomic. A dense scene issues ~3 runs
-// x 108 atomics per wavefront instead of 64 x 108; unsorted particles only
-// give shorter runs.
+// lane writes its particle's values into a row of an LDS table; then each
+// lane takes a column and walks the rows in lane order, summing runs -- a run
+// is a stretch of lanes whose particles share a base cell (particles sorted by
+// cell make long runs: ~37 per cell for a dense lattice) -- and adding each
+// run's sum to the grid with one FP32 atomic. A dense scene issues ~3 runs x
+// 108 atomics per wavefront instead of 64 x 108; unsorted particles only give
+// shorter runs.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -180,52 +180,51 @@ __device__ __forceinline__ float stress_scale(const DiffK& d) {
 }
 
 // ---------------------------------------------------------------- run scatter
-// Runs of lanes with equal keys (key < 0: the lane adds nothing) and their
-// base cells, as the head lanes wrote them.
-struct Runs {
-  int start[kWave + 1];
-  int key[kWave];
-  int base[3 * kWave];
-};
-
 // Adds the LDS table rows (NV values per lane, row stride STRIDE; VPN values
 // per node, stencil node o = 9a + 3b + c at columns VPN o .. VPN o + VPN - 1)
 // to `grid` (4 floats per node): one sum per (run, column), in lane order,
-// and one FP32 atomic per non-zero sum of a node inside the grid. Every lane
-// of the (one-wavefront) workgroup calls it.
+// and one FP32 atomic per non-zero sum of a node inside the grid. A run is a
+// stretch of lanes with the same key (key < 0: the lane adds nothing). Each
+// lane takes a column and walks the rows in lane order, 16 rows loaded at a
+// time; the run ends are the same for every column, so the walk branches
+// uniformly and a run's atomics for 64 columns are one instruction. Every
+// lane of the (one-wavefront) workgroup calls it.
 template <int NV, int STRIDE, int VPN>
-__device__ __forceinline__ void run_scatter(const float* table, Runs& runs, int key, const int base[3],
-                                            int N, float* __restrict__ grid) {
+__device__ __forceinline__ void run_scatter(const float* table, int key, const int base[3], int N,
+                                            float* __restrict__ grid) {
   const int lane = threadIdx.x;
-  const int prev = __shfl_up(key, 1);
-  const bool head = lane == 0 || key != prev;
-  const unsigned long long heads = __ballot(head);
-  const int nruns = __popcll(heads);
-  if (head) {
-    const int r = __popcll(heads & ((1ull << lane) - 1ull));
-    runs.start[r] = lane;
-    runs.key[r] = key;
-    runs.base[3 * r] = base[0];
-    runs.base[3 * r + 1] = base[1];
-    runs.base[3 * r + 2] = base[2];
-  }
-  if (lane == 0) runs.start[nruns] = kWave;
-  __syncthreads();  // the table rows and the run list are visible to every lane
-  const int total = nruns * NV;
-  for (int t = lane; t < total; t += kWave) {
-    const int r = t / NV;
-    const int q = t - r * NV;
-    if (runs.key[r] < 0) continue;
-    const int p1 = runs.start[r + 1];
+  const int next = __shfl_down(key, 1);
+  const uint64_t ends = __ballot(lane == kWave - 1 || key != next);
+  const uint64_t live = __ballot(key >= 0);
+  __syncthreads();  // the table rows are visible to every lane
+  for (int c0 = 0; c0 < NV; c0 += kWave) {
+    const int q = c0 + lane;
+    const bool qok = q < NV;
+    const int qc = qok ? q : 0;  // (a column past the table reads column 0, unused)
+    const int o = qc / VPN, comp = qc - (qc / VPN) * VPN;
+    const int da = o / 9, db = (o / 3) % 3, dc = o % 3;
     float sum = 0.0f;
-    for (int p = runs.start[r]; p < p1; ++p) sum += table[p * STRIDE + q];
-    if (sum == 0.0f) continue;
-    const int o = q / VPN;
-    const int comp = q - o * VPN;
-    const int gi = runs.base[3 * r] + o / 9, gj = runs.base[3 * r + 1] + (o / 3) % 3,
-              gk = runs.base[3 * r + 2] + o % 3;
-    if ((unsigned)gi >= (unsigned)N || (unsigned)gj >= (unsigned)N || (unsigned)gk >= (unsigned)N) continue;
-    unsafeAtomicAdd(grid + 4 * (((size_t)gi * N + (size_t)gj) * N + (size_t)gk) + comp, sum);
+#pragma unroll
+    for (int p0 = 0; p0 < kWave; p0 += 16) {
+      float v[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) v[i] = table[(p0 + i) * STRIDE + qc];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int p = p0 + i;
+        if ((live >> p) & 1ull) sum += v[i];  // (a row of no particle is never read)
+        if ((ends >> p) & 1ull) {
+          if (((live >> p) & 1ull) && qok && sum != 0.0f) {
+            const int gi = __builtin_amdgcn_readlane(base[0], p) + da;
+            const int gj = __builtin_amdgcn_readlane(base[1], p) + db;
+            const int gk = __builtin_amdgcn_readlane(base[2], p) + dc;
+            if ((unsigned)gi < (unsigned)N && (unsigned)gj < (unsigned)N && (unsigned)gk < (unsigned)N)
+              unsafeAtomicAdd(grid + 4 * (((size_t)gi * N + (size_t)gj) * N + (size_t)gk) + comp, sum);
+          }
+          sum = 0.0f;
+        }
+      }
+    }
   }
 }
 
@@ -268,7 +267,6 @@ __global__ void __launch_bounds__(kWave)
                 const float* __restrict__ mu, const float* __restrict__ lam, DiffK d,
                 float* __restrict__ grid) {
   __shared__ float table[P2G ? kWave * kP2GStride : 1];
-  __shared__ Runs runs;
   const int lane = threadIdx.x;
   const int64_t p = (int64_t)blockIdx.x * kWave + lane;
   const StepConsts& k = d.k;
@@ -331,7 +329,7 @@ __global__ void __launch_bounds__(kWave)
           }
       key = cell_key(sn, k.N);
     }
-    run_scatter<kP2GVals, kP2GStride, 4>(table, runs, key, sn.base, k.N, grid);
+    run_scatter<kP2GVals, kP2GStride, 4>(table, key, sn.base, k.N, grid);
   }
 }
 
@@ -380,7 +378,6 @@ __global__ void __launch_bounds__(kWave)
                const float4* __restrict__ vel_prev, float* adj_dst, float* __restrict__ adj_mu,
                float* __restrict__ adj_lam, float* __restrict__ gv_out, DiffK d) {
   __shared__ float table[G2PA ? kWave * kAdjStride : 1];
-  __shared__ Runs runs;
   const int lane = threadIdx.x;
   const int64_t p = (int64_t)blockIdx.x * kWave + lane;
   const StepConsts& k = d.k;
@@ -578,7 +575,7 @@ __global__ void __launch_bounds__(kWave)
       key = cell_key(sn, k.N);
     }
   }
-  run_scatter<kAdjVals, kAdjStride, 3>(table, runs, key, sn.base, k.N, gv_out);
+  run_scatter<kAdjVals, kAdjStride, 3>(table, key, sn.base, k.N, gv_out);
 }
 
 // adjoint of the updated velocity (gadj.xyz) -> adjoints of the P2G momentum

@@ -338,6 +338,9 @@ extern "C" int mpm_debug_selftest(char* buf, int32_t len) {
 // ---------------------------------------------------------------------------
 namespace {
 
+// grids and tapes are read as float4
+bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
 int diff_check(const char* fn, const mpm_params* p, float mass, float volume, int64_t n,
                StepConsts* k) {
   if (!p) {
@@ -389,6 +392,10 @@ extern "C" int mpm_diff_substep(const mpm_params* params, float particle_mass, f
     set_global_error("mpm_diff_substep: state_out must not alias state");
     return MPM_ERR_INVALID;
   }
+  if (!aligned16(grid) || !aligned16(tape)) {
+    set_global_error("mpm_diff_substep: grid and tape must be 16-byte aligned");
+    return MPM_ERR_INVALID;
+  }
   const hipError_t e = mpmk::diff_forward(k, particle_mass, particle_volume, n, state, state_out, mu,
                                           lambda, (float4*)grid, tape, (hipStream_t)stream);
   if (e != hipSuccess) {
@@ -418,6 +425,10 @@ extern "C" int mpm_diff_substep_noalloc(const mpm_params* params, float particle
     set_global_error("mpm_diff_substep_noalloc: state_out must not alias state");
     return MPM_ERR_INVALID;
   }
+  if (!aligned16(tape)) {
+    set_global_error("mpm_diff_substep_noalloc: tape must be 16-byte aligned");
+    return MPM_ERR_INVALID;
+  }
   const hipError_t e = mpmk::diff_forward_noalloc(k, particle_mass, particle_volume, n, state,
                                                   state_out, mu, lambda, tape, (hipStream_t)stream);
   if (e != hipSuccess) {
@@ -436,8 +447,9 @@ extern "C" int mpm_diff_forward_steps(const mpm_params* params, float particle_m
   int rc = diff_check(__func__, params, particle_mass, particle_volume, n, &k);
   if (rc) return rc;
   const int64_t nodes = (int64_t)params->N * params->N * params->N;
-  if (steps < 1 || state_stride < 24 * n || (tape_stride != 0 && tape_stride < 8 * nodes)) {
-    set_global_error("mpm_diff_forward_steps: need steps >= 1, state_stride >= 24 n and tape_stride 0 or >= 8 N^3");
+  if (steps < 1 || state_stride < 24 * n || (tape_stride != 0 && (tape_stride < 8 * nodes || tape_stride % 4))) {
+    set_global_error("mpm_diff_forward_steps: need steps >= 1, state_stride >= 24 n and tape_stride 0 or a "
+                     "multiple of 4 >= 8 N^3");
     return MPM_ERR_INVALID;
   }
   if (!grid || !tapes || (n > 0 && (!state0 || !states || !mu || !lambda))) {
@@ -446,6 +458,10 @@ extern "C" int mpm_diff_forward_steps(const mpm_params* params, float particle_m
   }
   if (n > 0 && state0 >= states && state0 < states + (size_t)steps * (size_t)state_stride) {
     set_global_error("mpm_diff_forward_steps: state0 must not lie in the output states");
+    return MPM_ERR_INVALID;
+  }
+  if (!aligned16(grid) || !aligned16(tapes)) {
+    set_global_error("mpm_diff_forward_steps: grid and tapes must be 16-byte aligned");
     return MPM_ERR_INVALID;
   }
   const hipError_t e = mpmk::diff_forward_steps(k, particle_mass, particle_volume, n, steps, state0, states,
@@ -468,8 +484,9 @@ extern "C" int mpm_diff_backward_steps(const mpm_params* params, float particle_
   int rc = diff_check(__func__, params, particle_mass, particle_volume, n, &k);
   if (rc) return rc;
   const int64_t nodes = (int64_t)params->N * params->N * params->N;
-  if (steps < 1 || state_stride < 24 * n || tape_stride < 8 * nodes) {
-    set_global_error("mpm_diff_backward_steps: need steps >= 1, state_stride >= 24 n and tape_stride >= 8 N^3");
+  if (steps < 1 || state_stride < 24 * n || (steps > 1 && (tape_stride < 8 * nodes || tape_stride % 4))) {
+    set_global_error("mpm_diff_backward_steps: need steps >= 1, state_stride >= 24 n and tape_stride a "
+                     "multiple of 4 >= 8 N^3");
     return MPM_ERR_INVALID;
   }
   if (!grid || !tapes ||
@@ -479,6 +496,10 @@ extern "C" int mpm_diff_backward_steps(const mpm_params* params, float particle_
   }
   if (n > 0 && (adj_in == adj_out || work == adj_out)) {
     set_global_error("mpm_diff_backward_steps: adj_in and work must not alias adj_out");
+    return MPM_ERR_INVALID;
+  }
+  if (!aligned16(grid) || !aligned16(tapes)) {
+    set_global_error("mpm_diff_backward_steps: grid and tapes must be 16-byte aligned");
     return MPM_ERR_INVALID;
   }
   const hipError_t e = mpmk::diff_backward_steps(k, particle_mass, particle_volume, n, steps, states,
@@ -507,6 +528,10 @@ extern "C" int mpm_diff_substep_backward(const mpm_params* params, float particl
   }
   if (n > 0 && adj_in == adj_out) {
     set_global_error("mpm_diff_substep_backward: adj_in must not alias adj_out");
+    return MPM_ERR_INVALID;
+  }
+  if (!aligned16(grid) || !aligned16(tape)) {
+    set_global_error("mpm_diff_substep_backward: grid and tape must be 16-byte aligned");
     return MPM_ERR_INVALID;
   }
   const hipError_t e = mpmk::diff_backward(k, particle_mass, particle_volume, n, state, mu, lambda,

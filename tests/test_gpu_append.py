@@ -6,7 +6,8 @@ the same physics to the step's parity tolerance (the two runs hold the
 particles in different device orders, so the P2G sums add in a different
 order), the simulated time goes on instead of restarting, downloads list the
 appended particles last, and the device arrays grow past their first
-capacity. A liquid whose F was implicit (cbrt(J) I) keeps the first
+capacity. Against the oracle run on the same split (the first half for k1
+substeps, then both halves for k2): the snow parity rule of the step tests. A liquid whose F was implicit (cbrt(J) I) keeps the first
 particles' F when particles with another F arrive.
 """
 import numpy as np
@@ -67,6 +68,36 @@ def test_append_matches_round_trip(gpu_lib):
     assert np.array_equal(app["material"], ref["material"])
     errs = state_errors(app, ref, sc.params.N)
     assert all(e < 1e-4 for e in errs.values()), errs
+
+
+def test_append_matches_oracle_on_the_same_split(gpu_lib):
+    """The oracle runs the same split in FP32 and in FP64: the first half for
+    k1 substeps, the second half joined behind it, both for k2. The engine
+    passes the snow rule of the step tests: within 1e-3 of the FP32 oracle,
+    or within twice the FP32 oracle's own distance from FP64."""
+    import oracle
+    from conftest import state_elem_errors
+
+    sc = scenes.snow_spheres(n=40_000, N=32)
+    k1, k2 = 25, 30
+    app, _, _ = split_run(sc, k1, k2)
+    st = sc.state()
+    n, N = sc.n, sc.params.N
+    threads = oracle.check_threads()
+    refs = {}
+    for dt in (np.float32, np.float64):
+        first = {f: st[f][: n // 2] for f in st}
+        mid = oracle.advance(sc.params, sc.materials, first, k1, dt, threads=threads)
+        joined = {f: np.concatenate([np.asarray(mid[f]), np.asarray(st[f][n // 2:], np.asarray(mid[f]).dtype)])
+                  for f in mid}
+        refs[dt] = oracle.advance(sc.params, sc.materials, joined, k2, dt, threads=threads)
+    e32 = state_errors(app, refs[np.float32], N)
+    e64 = state_errors(app, refs[np.float64], N)
+    o64 = state_errors(refs[np.float32], refs[np.float64], N)
+    print("append vs FP32 oracle", e32, "vs FP64", e64, "FP32 oracle vs FP64", o64,
+          "element-wise vs FP32", state_elem_errors(app, refs[np.float32], N))
+    for f in ("x", "v", "F", "C"):
+        assert e32[f] < 1e-3 or (e64[f] <= 2.0 * o64[f] and e32[f] < 1e-2), (f, e32[f], e64[f], o64[f])
 
 
 def test_append_to_implicit_liquid_F(gpu_lib):

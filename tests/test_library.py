@@ -147,3 +147,13 @@ def test_diff_substep_rejects_aliased_output():
     rc = mpm_amd.lib().mpm_diff_substep(C.byref(p), 1.0, 1.0, 4, d, d, d, d, d, d, None)
     assert rc == _lib.MPM_ERR_INVALID
     assert b"alias" in mpm_amd.lib().mpm_last_error(None)
+
+
+def test_diff_substep_rejects_misaligned_grid_and_tape():
+    p = mpm_amd.Params(dt=1e-4, N=16, compat=_lib.COMPAT_FIXED).to_c()
+    a, b, odd = C.c_void_p(256), C.c_void_p(512), C.c_void_p(1032)
+    lib = mpm_amd.lib()
+    assert lib.mpm_diff_substep(C.byref(p), 1.0, 1.0, 4, a, b, a, a, odd, a, None) == _lib.MPM_ERR_INVALID
+    assert b"16-byte aligned" in lib.mpm_last_error(None)
+    assert lib.mpm_diff_substep_noalloc(C.byref(p), 1.0, 1.0, 4, a, b, a, a, odd, None) == _lib.MPM_ERR_INVALID
+    assert b"16-byte aligned" in lib.mpm_last_error(None)
