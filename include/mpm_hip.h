@@ -182,10 +182,9 @@ int mpm_download_grid(mpm_handle h, float* grid, void* stream);
  * mpm_mesh_build: distance field on grid_side^3 voxels (voxel_dx = N dx /
  * grid_side; each voxel the distance to the nearest particle whose box of
  * voxels, (particle_radius + 1) voxels around it, reaches it, else that
- * cut-off), then the iso-surface at particle_radius voxels by marching
- * tetrahedra (the reference runs libigl's marching cubes on the same field;
- * its optional smoothing and decimation, mesh_builder.h:201-210, are not
- * done). Returns the triangle count. Blocking.
+ * cut-off), then the iso-surface at particle_radius voxels by marching cubes
+ * (a closed surface; the case table, mpm_mesh_case_table, is this library's,
+ * not libigl's). Returns the triangle count. Blocking.
  * mpm_mesh_triangles: the triangles, 9 floats each (3 vertices, normal
  * pointing out by the right-hand rule), and for each vertex the grid edge it
  * lies on (lower node index << 32 | upper node index; NULL skips them).
@@ -197,6 +196,47 @@ int mpm_mesh_build(mpm_handle h, int32_t grid_side, int32_t particle_radius, int
 int mpm_mesh_triangles(mpm_handle h, float* xyz, uint64_t* edges);
 int mpm_mesh_distance_field(mpm_handle h, int32_t grid_side, float* out);
 int mpm_mesh_write_obj(mpm_handle h, const char* path);
+/* The last mesh indexed: one vertex per crossed grid edge (xyz doubles, in
+ * edge-key order), faces as 0-based vertex triples. Pass V = F = NULL to get
+ * the counts *nv, *nf first. */
+int mpm_mesh_indexed(mpm_handle h, int64_t* nv, int64_t* nf, double* V, int32_t* F);
+/* The marching-cubes case table: for case c (bit q: cube corner q = (q>>2&1,
+ * q>>1&1, q&1) inside), ntri[c] triangles, tri[15 c + 3 t + k] the cube edge
+ * of vertex k of triangle t; edges 0-3 along x from corners 0-3, 4-7 along y
+ * from corners 0,1,4,5, 8-11 along z from corners 0,2,4,6. */
+int mpm_mesh_case_table(uint8_t ntri[256], int8_t tri[256 * 15]);
+
+/* ---- debug view (replaces Renderer, include/renderer.h:7-59 and
+ *      src/renderer.cpp, which draw to a window; no reference C function) ----
+ * The device particles as the reference draws them (points of 2 pixels at
+ * clip position (x - 0.5) * 1.8, colour (0.59, 0.54, 0.2) on 0.1 grey, no
+ * depth test) into `rgb`, a host buffer of width * height RGB8 pixels, rows
+ * top to bottom (the reference's window: 800 x 600). A point covers the
+ * pixels whose centres lie within one pixel of its window position (right
+ * and top edges included). Blocking. */
+int mpm_render_frame(mpm_handle h, int32_t width, int32_t height, uint8_t* rgb, void* stream);
+
+/* ---- host-side mesh steps and particle files (replace MeshBuilder's
+ *      libigl calls, mesh_builder.h:196-246, and ParticleWriter's Partio
+ *      call, particle_writer.h:13-44; neither library is used). Vertices are
+ *      double xyz triples, faces 0-based int32 triples. ----
+ * mpm_mesh_smooth: MeshBuilder::smoothMesh in place: one implicit step
+ * (M - 0.001 L) U = M V (L the cotangent Laplacian, M the barycentric mass
+ * matrix of the input), then U /= sqrt(total area of U) (mesh_builder.h:
+ * 215-246).
+ * mpm_mesh_decimate: igl::decimate with its default cost and placement
+ * (mesh_builder.h:203-207): shortest edge first, merged at its midpoint, only
+ * where the link condition holds, until at most max_faces faces remain; U
+ * (capacity nv vertices) and G (capacity nf faces) receive the result.
+ * mpm_obj_write: igl::writeOBJ (v lines, then 1-based f lines).
+ * mpm_bgeo_write: the particles (x, v: n xyz float triples) as the reference
+ * writes them (id, position, velocity, radius 0.1) in Houdini's classic
+ * .bgeo format (version 5, gzip-compressed; particle_writer.h:13-44). */
+int mpm_mesh_smooth(int64_t nv, double* V, int64_t nf, const int32_t* F);
+int mpm_mesh_decimate(int64_t nv, const double* V, int64_t nf, const int32_t* F, int64_t max_faces,
+                      double* U, int64_t* nu, int32_t* G, int64_t* ng);
+int mpm_obj_write(const char* path, int64_t nv, const double* V, int64_t nf, const int32_t* F);
+int mpm_bgeo_write(const char* path, int64_t n, const float* x, const float* v);
 
 /* ---- full-state checkpoint (no reference counterpart: the reference writes
  *      only x and v, particle_writer.h:32-37). Binary, little-endian. ---- */

@@ -71,6 +71,7 @@ struct CLIOptions {
   int32_t mesh_face_count = -1;
   int laplacian_smooth = 0;
   bool no_mesh = false;  // (no counterpart: the reference always writes meshes with --save-dir)
+  std::string frames_dir;  // (no counterpart: the reference shows its view in a window)
   // not reference flags
   float gravity = -9.81f;
   float boundary = 0.05f;
@@ -131,6 +132,7 @@ struct CLIOptions {
       else if (a == "seed") seed = (uint32_t)to_i(a, val, 0);
       else if (a == "steps") steps = to_i(a, val, 0);
       else if (a == "sync-every") sync_every = (uint32_t)to_i(a, val, 1);
+      else if (a == "frames-dir") frames_dir = val;
       else die("unknown option '" + a + "'");
     }
     dx = (float)(1.0 / N);  // options.h:41
@@ -147,15 +149,16 @@ struct CLIOptions {
         "  --particle-count arg        Particles per unit volume (default 500000)\n"
         "  --mesh-grid arg             Grid size for computing mesh (default 250)\n"
         "  --mesh-particle-radius arg  Particle radius for the mesh, in grid points (default 5)\n"
-        "  --mesh-face-count arg       accepted, unused (no decimation)\n"
-        "  --laplacian_smooth arg      accepted, unused (no smoothing)\n"
+        "  --mesh-face-count arg       Approximate the mesh with this many faces (default -1: all)\n"
+        "  --laplacian_smooth arg      Apply Laplacian smoothing to the mesh (default 0: off)\n"
         "  --no-mesh                   with --save-dir, write particles only\n"
         "  --steps arg                 substeps to run (default 1000)\n"
         "  --gravity arg               y acceleration (default -9.81)\n"
         "  --boundary arg              wall fraction (default 0.05)\n"
         "  --compat reference|fixed    reference quirks on or off (default reference)\n"
         "  --seed arg                  sampler seed (default 1, the reference's unseeded rand())\n"
-        "  --sync-every arg            substeps between device syncs (default 20)\n");
+        "  --sync-every arg            substeps between device syncs (default 20)\n"
+        "  --frames-dir arg            write the debug view every 20 substeps as PPM images\n");
   }
 
  private:
@@ -364,24 +367,87 @@ class Simulation {
 };
 
 // MeshBuilder (include/mesh_builder.h:19-35,181-211): the surface mesh of the
-// simulation's particles to an OBJ file, built on the GPU from the device
-// particles (mpm_mesh_build; no smoothing or decimation).
+// simulation's particles to an OBJ file: distance field and marching cubes on
+// the GPU from the device particles (mpm_mesh_build), then on the host the
+// optional smoothing (--laplacian_smooth != 0, mesh_builder.h:196-200) and
+// decimation (--mesh-face-count != -1, mesh_builder.h:202-207).
 class MeshBuilder {
  public:
   MeshBuilder(Simulation& sim, const CLIOptions& flags, uint32_t grid_size)
-      : sim_(sim), radius_(flags.mesh_particle_radius), side_(grid_size) {}
-  // the triangle count; throws on failure
+      : sim_(sim), radius_(flags.mesh_particle_radius), side_(grid_size), smooth_(flags.laplacian_smooth != 0),
+        faces_(flags.mesh_face_count) {}
+  // the face count written; throws on failure
   int64_t computeMesh(const std::string& filename) {
-    int64_t n = 0;
+    int64_t n = 0, nv = 0, nf = 0;
     check(mpm_mesh_build(sim_.handle(), (int32_t)side_, (int32_t)radius_, &n, nullptr), sim_.handle(),
           "mpm_mesh_build");
-    check(mpm_mesh_write_obj(sim_.handle(), filename.c_str()), sim_.handle(), "mpm_mesh_write_obj");
-    return n;
+    check(mpm_mesh_indexed(sim_.handle(), &nv, &nf, nullptr, nullptr), sim_.handle(), "mpm_mesh_indexed");
+    std::vector<double> V(3 * (size_t)nv);
+    std::vector<int32_t> F(3 * (size_t)nf);
+    check(mpm_mesh_indexed(sim_.handle(), &nv, &nf, V.data(), F.data()), sim_.handle(), "mpm_mesh_indexed");
+    if (smooth_) check(mpm_mesh_smooth(nv, V.data(), nf, F.data()), nullptr, "mpm_mesh_smooth");
+    if (faces_ != -1) {
+      std::vector<double> U(V.size());
+      std::vector<int32_t> G(F.size());
+      int64_t nu = 0, ng = 0;
+      check(mpm_mesh_decimate(nv, V.data(), nf, F.data(), faces_ < 0 ? 0 : faces_, U.data(), &nu, G.data(), &ng),
+            nullptr, "mpm_mesh_decimate");
+      V.swap(U);
+      F.swap(G);
+      nv = nu;
+      nf = ng;
+    }
+    check(mpm_obj_write(filename.c_str(), nv, V.data(), nf, F.data()), nullptr, "mpm_obj_write");
+    return nf;
   }
 
  private:
   Simulation& sim_;
   uint32_t radius_, side_;
+  bool smooth_;
+  int32_t faces_;
+};
+
+// Renderer (include/renderer.h:7-59): the reference's debug view of the
+// device particles (mpm_render_frame), to PPM images instead of a window.
+class Renderer {
+ public:
+  Renderer(Simulation& sim, const std::string& save_dir, uint32_t width = 800, uint32_t height = 600)
+      : sim_(sim), dir_(save_dir), w_(width), h_(height), rgb_((size_t)3 * width * height) {}
+  // frame_<id>.ppm in the directory; throws on failure
+  void render(uint32_t frame_id) {
+    check(mpm_render_frame(sim_.handle(), (int32_t)w_, (int32_t)h_, rgb_.data(), nullptr), sim_.handle(),
+          "mpm_render_frame");
+    char name[64];
+    std::snprintf(name, sizeof(name), "/frame_%05u.ppm", frame_id);
+    FILE* f = std::fopen((dir_ + name).c_str(), "wb");
+    if (!f) throw Error(MPM_ERR_IO, "cannot write " + dir_ + name);
+    std::fprintf(f, "P6\n%u %u\n255\n", w_, h_);
+    const bool ok = std::fwrite(rgb_.data(), 1, rgb_.size(), f) == rgb_.size();
+    if (std::fclose(f) != 0 || !ok) throw Error(MPM_ERR_IO, "cannot write " + dir_ + name);
+  }
+
+ private:
+  Simulation& sim_;
+  std::string dir_;
+  uint32_t w_, h_;
+  std::vector<uint8_t> rgb_;
+};
+
+// ParticleWriter (include/particle_writer.h:7-44): id, position, velocity
+// and radius 0.1 of each particle to a .bgeo file (mpm_bgeo_write).
+class ParticleWriter {
+ public:
+  void writeParticles(const std::string& filepath, const std::vector<Particle>& particles) {
+    std::vector<float> x(3 * particles.size()), v(3 * particles.size());
+    for (size_t i = 0; i < particles.size(); ++i)
+      for (int a = 0; a < 3; ++a) {
+        x[3 * i + (size_t)a] = particles[i].x[a];
+        v[3 * i + (size_t)a] = particles[i].v[a];
+      }
+    check(mpm_bgeo_write(filepath.c_str(), (int64_t)particles.size(), x.data(), v.data()), nullptr,
+          "mpm_bgeo_write");
+  }
 };
 
 }  // namespace mpm

@@ -106,6 +106,13 @@ SIGNATURES = {
     "mpm_mesh_distance_field": (C.c_int, [_P, C.c_int32, _P]),
     "mpm_mesh_write_obj": (C.c_int, [_P, C.c_char_p]),
     "mpm_count": (C.c_int, [_P, _i64p]),
+    "mpm_mesh_indexed": (C.c_int, [_P, _i64p, _i64p, _P, _P]),
+    "mpm_mesh_case_table": (C.c_int, [_P, _P]),
+    "mpm_render_frame": (C.c_int, [_P, C.c_int32, C.c_int32, _P, _P]),
+    "mpm_mesh_smooth": (C.c_int, [C.c_int64, _P, C.c_int64, _P]),
+    "mpm_mesh_decimate": (C.c_int, [C.c_int64, _P, C.c_int64, _P, C.c_int64, _P, _i64p, _P, _i64p]),
+    "mpm_obj_write": (C.c_int, [C.c_char_p, C.c_int64, _P, C.c_int64, _P]),
+    "mpm_bgeo_write": (C.c_int, [C.c_char_p, C.c_int64, _P, _P]),
     "mpm_remove_particles": (C.c_int, [_P, C.c_int64, C.c_int64, _P, _P]),
     "mpm_advance": (C.c_int, [_P, C.c_int32, _P]),
     "mpm_time": (C.c_int, [_P, C.POINTER(C.c_double)]),

@@ -39,7 +39,7 @@ COMMON = [
 ]
 
 SOURCES = ["capi.cpp", "material.cpp", "engine.cpp", "batch.cpp", "substep.hip", "p2g_block.hip", "sort.hip",
-           "linalg.hip", "adjoint.hip", "scene.cpp", "mesh.hip"]
+           "linalg.hip", "adjoint.hip", "scene.cpp", "mesh.hip", "mesh_post.cpp", "render.hip"]
 
 
 def _compile(src: Path) -> Path:
@@ -70,7 +70,7 @@ def build(verbose: bool = False) -> Path:
     # --no-undefined: a declaration without a definition fails the link here,
     # not at load time on the GPU box
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-mcode-object-version=5", "-shared", "-fPIC",
-           "-Wl,--no-undefined", "-o", str(tmp), *map(str, objs), "-lrocprofiler-sdk-roctx"]
+           "-Wl,--no-undefined", "-o", str(tmp), *map(str, objs), "-lrocprofiler-sdk-roctx", "-lz"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

@@ -18,6 +18,8 @@
 
 #include "../../include/mpm_hip.h"
 #include "engine.hpp"
+#include "mc_table.hpp"
+#include "mesh_post.hpp"
 #include "host_error.hpp"
 #include "kernels.hpp"
 
@@ -118,6 +120,96 @@ extern "C" int mpm_mesh_triangles(mpm_handle h, float* xyz, uint64_t* edges) {
 extern "C" int mpm_mesh_distance_field(mpm_handle h, int32_t grid_side, float* out) {
   MPM_GET(h);
   return s->mesh_distance(grid_side, out);
+}
+
+extern "C" int mpm_mesh_indexed(mpm_handle h, int64_t* nv, int64_t* nf, double* V, int32_t* F) {
+  MPM_GET(h);
+  if (!nv || !nf) return s->fail(MPM_ERR_INVALID, "mpm_mesh_indexed: null count");
+  return s->mesh_indexed(nv, nf, V, F);
+}
+
+extern "C" int mpm_render_frame(mpm_handle h, int32_t width, int32_t height, uint8_t* rgb, void* stream) {
+  MPM_GET(h);
+  return s->render(width, height, rgb, (hipStream_t)stream);
+}
+
+extern "C" int mpm_mesh_case_table(uint8_t* ntri, int8_t* tri) {
+  if (!ntri || !tri) {
+    set_global_error("mpm_mesh_case_table: null argument");
+    return MPM_ERR_INVALID;
+  }
+  static constexpr mc::Table t = mc::make_table();
+  std::memcpy(ntri, t.ntri, sizeof(t.ntri));
+  std::memcpy(tri, t.tri, sizeof(t.tri));
+  return MPM_OK;
+}
+
+namespace {
+int mesh_args(const char* fn, int64_t nv, const void* V, int64_t nf, const int32_t* F) {
+  std::string why;
+  if (nv < 0 || nf < 0 || nv > INT32_MAX || (nv > 0 && !V) || (nf > 0 && !F)) {
+    set_global_error("%s: bad mesh arguments", fn);
+    return MPM_ERR_INVALID;
+  }
+  if (!mpm_mesh::check_faces(nv, nf, F, &why)) {
+    set_global_error("%s: %s", fn, why.c_str());
+    return MPM_ERR_INVALID;
+  }
+  return MPM_OK;
+}
+}  // namespace
+
+extern "C" int mpm_mesh_smooth(int64_t nv, double* V, int64_t nf, const int32_t* F) {
+  int rc = mesh_args(__func__, nv, V, nf, F);
+  if (rc) return rc;
+  std::string why;
+  if (mpm_mesh::smooth(nv, V, nf, F, &why) != 0) {
+    set_global_error("%s", why.c_str());
+    return MPM_ERR_INVALID;
+  }
+  return MPM_OK;
+}
+
+extern "C" int mpm_mesh_decimate(int64_t nv, const double* V, int64_t nf, const int32_t* F, int64_t max_faces,
+                                 double* U, int64_t* nu, int32_t* G, int64_t* ng) {
+  int rc = mesh_args(__func__, nv, V, nf, F);
+  if (rc) return rc;
+  if (max_faces < 0 || !nu || !ng || (nv > 0 && !U) || (nf > 0 && !G)) {
+    set_global_error("mpm_mesh_decimate: need max_faces >= 0 and output buffers");
+    return MPM_ERR_INVALID;
+  }
+  std::string why;
+  if (mpm_mesh::decimate(nv, V, nf, F, max_faces, U, nu, G, ng, &why) != 0) {
+    set_global_error("mpm_mesh_decimate: %s", why.c_str());
+    return MPM_ERR_INVALID;
+  }
+  return MPM_OK;
+}
+
+extern "C" int mpm_obj_write(const char* path, int64_t nv, const double* V, int64_t nf, const int32_t* F) {
+  int rc = mesh_args(__func__, nv, V, nf, F);
+  if (rc) return rc;
+  if (!path) {
+    set_global_error("mpm_obj_write: null path");
+    return MPM_ERR_INVALID;
+  }
+  if (!mpm_mesh::write_obj(path, nv, V, nf, F)) {
+    set_global_error("mpm_obj_write: cannot write %s", path);
+    return MPM_ERR_IO;
+  }
+  return MPM_OK;
+}
+
+extern "C" int mpm_bgeo_write(const char* path, int64_t n, const float* x, const float* v) {
+  if (!path || n < 0 || n > INT32_MAX || (n > 0 && (!x || !v))) {
+    set_global_error("mpm_bgeo_write: bad arguments");
+    return MPM_ERR_INVALID;
+  }
+  if (!mpm_mesh::write_bgeo(path, n, x, v)) {
+    set_global_error("mpm_bgeo_write: cannot write %s", path);
+    return MPM_ERR_IO;
+  }
+  return MPM_OK;
 }
 
 extern "C" int mpm_mesh_write_obj(mpm_handle h, const char* path) {

@@ -8,6 +8,7 @@
 //   make_material_dev    MaterialModel fields + p2g_prepare_particle's stress factor,
 //                        include/TransferScheme.h:83
 #include "engine.hpp"
+#include "mesh_post.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -94,6 +95,7 @@ Sim::~Sim() {
                       d_sdf_, d_mcount_, d_moff_, d_mscratch_})
     if (p) (void)hipFree(p);
   if (d_tiles_) (void)hipFree(d_tiles_);
+  if (d_img_) (void)hipFree(d_img_);
   if (d_mxyz_) (void)hipFree(d_mxyz_);
   if (d_mkeys_) (void)hipFree(d_mkeys_);
   for (auto& p : ev_) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -687,11 +689,11 @@ int Sim::mesh_distance(int side, float* out) {
   return MPM_OK;
 }
 
-// OBJ of the last mesh: one vertex per grid edge the surface crosses (the
-// triangles that share an edge share its vertex bits), 1-based faces.
-int Sim::mesh_write_obj(const char* path) {
-  if (mesh_ntri_ < 0) return fail(MPM_ERR_STATE, "mpm_mesh_write_obj: no mesh built");
-  if (!path) return fail(MPM_ERR_INVALID, "mpm_mesh_write_obj: null path");
+// The last mesh indexed: one vertex per grid edge the surface crosses (the
+// triangles that share an edge share its vertex bits), in the order of the
+// edge keys; faces 0-based.
+int Sim::mesh_indexed(int64_t* nv, int64_t* nf, double* V, int32_t* F) {
+  if (mesh_ntri_ < 0) return fail(MPM_ERR_STATE, "mpm_mesh_indexed: no mesh built");
   const size_t n = (size_t)mesh_ntri_;
   std::vector<float> xyz(9 * n);
   std::vector<uint64_t> keys(3 * n);
@@ -700,23 +702,52 @@ int Sim::mesh_write_obj(const char* path) {
   std::vector<uint64_t> order(keys);
   std::sort(order.begin(), order.end());
   order.erase(std::unique(order.begin(), order.end()), order.end());
-  std::vector<int64_t> first(order.size(), -1);
-  std::vector<int64_t> face(3 * n);
+  if (order.size() > (size_t)INT32_MAX) return fail(MPM_ERR_CAPACITY, "mpm_mesh_indexed: too many vertices");
+  if (nv) *nv = (int64_t)order.size();
+  if (nf) *nf = (int64_t)n;
+  if (!V && !F) return MPM_OK;
   for (size_t t = 0; t < 3 * n; ++t) {
     const size_t v = (size_t)(std::lower_bound(order.begin(), order.end(), keys[t]) - order.begin());
-    if (first[v] < 0) first[v] = (int64_t)t;
-    face[t] = (int64_t)v + 1;
+    if (F) F[t] = (int32_t)v;
+    if (V)
+      for (int a = 0; a < 3; ++a) V[3 * v + a] = (double)xyz[3 * t + a];
   }
-  FILE* f = std::fopen(path, "w");
-  if (!f) return fail(MPM_ERR_IO, "mpm_mesh_write_obj: cannot open %s", path);
-  for (size_t v = 0; v < order.size(); ++v) {
-    const float* p = &xyz[3 * (size_t)first[v]];
-    std::fprintf(f, "v %.9g %.9g %.9g\n", p[0], p[1], p[2]);
+  return MPM_OK;
+}
+
+// The reference's debug view (Renderer::render, renderer.h:48-59) of the
+// device particles, without a copy of them to the host. Blocking.
+int Sim::render(int W, int H, uint8_t* rgb, hipStream_t s) {
+  if (!uploaded_) return fail(MPM_ERR_STATE, "mpm_render_frame: nothing uploaded");
+  if (W < 1 || H < 1 || W > 16384 || H > 16384 || !rgb)
+    return fail(MPM_ERR_INVALID, "mpm_render_frame: need 1 <= width, height <= 16384 and an image buffer");
+  const size_t bytes = (size_t)3 * (size_t)W * (size_t)H;
+  if (bytes > img_cap_) {
+    if (d_img_) (void)hipFree(d_img_);
+    d_img_ = nullptr;
+    img_cap_ = 0;
+    HIPCHK(hipMalloc(&d_img_, bytes), "hipMalloc(image)");
+    img_cap_ = bytes;
   }
-  for (size_t t = 0; t < n; ++t)
-    std::fprintf(f, "f %lld %lld %lld\n", (long long)face[3 * t], (long long)face[3 * t + 1],
-                 (long long)face[3 * t + 2]);
-  if (std::fclose(f) != 0) return fail(MPM_ERR_IO, "mpm_mesh_write_obj: cannot write %s", path);
+  HIPCHK(mpmk::render_points(particles(), W, H, d_img_, s), "render");
+  HIPCHK(hipMemcpyAsync(rgb, d_img_, bytes, hipMemcpyDeviceToHost, s), "image copy");
+  HIPCHK(hipStreamSynchronize(s), "image copy");
+  return MPM_OK;
+}
+
+// OBJ of the last mesh as built (no smoothing or decimation)
+int Sim::mesh_write_obj(const char* path) {
+  if (mesh_ntri_ < 0) return fail(MPM_ERR_STATE, "mpm_mesh_write_obj: no mesh built");
+  if (!path) return fail(MPM_ERR_INVALID, "mpm_mesh_write_obj: null path");
+  int64_t nv = 0, nf = 0;
+  int rc = mesh_indexed(&nv, &nf, nullptr, nullptr);
+  if (rc) return rc;
+  std::vector<double> V(3 * (size_t)nv);
+  std::vector<int32_t> F(3 * (size_t)nf);
+  rc = mesh_indexed(&nv, &nf, V.data(), F.data());
+  if (rc) return rc;
+  if (!mpm_mesh::write_obj(path, nv, V.data(), nf, F.data()))
+    return fail(MPM_ERR_IO, "mpm_mesh_write_obj: cannot write %s", path);
   return MPM_OK;
 }
 

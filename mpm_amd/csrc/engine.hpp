@@ -66,6 +66,11 @@ class Sim {
   int mesh_triangles(float* xyz, uint64_t* keys);
   int mesh_distance(int side, float* out);
   int mesh_write_obj(const char* path);
+  // the last mesh with one vertex per crossed grid edge (in edge-key order)
+  // and 0-based faces; V, F may be null to ask for the counts
+  int mesh_indexed(int64_t* nv, int64_t* nf, double* V, int32_t* F);
+  // the debug view of the device particles into a host RGB8 image (render.hip)
+  int render(int W, int H, uint8_t* rgb, hipStream_t s);
   int64_t count() const { return P_; }
   double time() const { return t_; }
   int advance(int32_t substeps, hipStream_t s);
@@ -191,6 +196,8 @@ class Sim {
   float* d_mxyz_ = nullptr;
   uint64_t* d_mkeys_ = nullptr;
   size_t sdf_cap_ = 0, mcube_cap_ = 0, mtri_cap_ = 0;
+  uint8_t* d_img_ = nullptr;  // the last rendered image
+  size_t img_cap_ = 0;
   int mesh_side_ = 0;
   int64_t mesh_ntri_ = -1;
   int ensure_f(hipStream_t s);

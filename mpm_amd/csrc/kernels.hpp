@@ -253,4 +253,8 @@ hipError_t diff_backward(const StepConsts& k, float mass, float volume, int64_t 
                          float* adj_in, float* adj_mu, float* adj_lam, float4* grid_adj,
                          hipStream_t s);
 
+// render.hip: the device particles as the reference's debug view draws them,
+// into a W x H RGB8 image (rows top to bottom) on the device
+hipError_t render_points(const ParticlesDev& P, int W, int H, uint8_t* img, hipStream_t s);
+
 }  // namespace mpmk

@@ -4,16 +4,15 @@
 // simulation, syncing every --sync-every substeps (main.cu:99-101) and, with
 // --save-dir, writing 240 times per simulated second (main.cu:7,93,102-113)
 // the surface mesh to <save-dir>/meshes/mesh_<frame>.obj (MeshBuilder,
-// main.cu:78,104-106: --mesh-grid voxels, --mesh-particle-radius) and the
-// active particles to <save-dir>/particles/particles_<frame>.ply.
+// main.cu:78,104-106: --mesh-grid voxels, --mesh-particle-radius,
+// --laplacian_smooth, --mesh-face-count) and the active particles to
+// <save-dir>/particles/particles_<frame>.bgeo (ParticleWriter, main.cu:
+// 107-111).
 //
 // Differences: the run stops after --steps substeps (the reference loops
-// until killed, rendering to a window); particles are written as binary PLY
-// (x, y, z, vx, vy, vz, id) because the reference's .bgeo writer needs the
-// Partio library (particle_writer.h:1-3, absent here); the mesh is built on
-// the GPU by marching tetrahedra, without libigl's smoothing or decimation
-// (--laplacian_smooth, --mesh-face-count are accepted and ignored);
-// --no-mesh skips it.
+// until killed, rendering to a window every 20 substeps; with --frames-dir
+// the same view goes to PPM images instead, mpm_render_frame); --no-mesh
+// skips the mesh.
 #include <sys/stat.h>
 
 #include <chrono>
@@ -25,24 +24,6 @@
 #include "../../include/mpm_simulation.hpp"
 
 namespace {
-
-bool write_ply(const std::string& path, const std::vector<mpm::Particle>& ps) {
-  FILE* f = std::fopen(path.c_str(), "wb");
-  if (!f) return false;
-  std::fprintf(f,
-               "ply\nformat binary_little_endian 1.0\nelement vertex %zu\n"
-               "property float x\nproperty float y\nproperty float z\n"
-               "property float vx\nproperty float vy\nproperty float vz\n"
-               "property int id\nend_header\n",
-               ps.size());
-  for (size_t i = 0; i < ps.size(); ++i) {
-    float rec[6] = {ps[i].x[0], ps[i].x[1], ps[i].x[2], ps[i].v[0], ps[i].v[1], ps[i].v[2]};
-    const int32_t id = (int32_t)i;
-    std::fwrite(rec, sizeof(rec), 1, f);
-    std::fwrite(&id, sizeof(id), 1, f);
-  }
-  return std::fclose(f) == 0;
-}
 
 void make_dir(const std::string& d) { ::mkdir(d.c_str(), 0755); }
 
@@ -93,6 +74,12 @@ int main(int argc, char* argv[]) {
       if (mesh) make_dir(flags.save_dir + "/meshes");
     }
     mpm::MeshBuilder mesher(simulation, flags, flags.mesh_grid);
+    mpm::ParticleWriter writer;
+    const bool frames = !flags.frames_dir.empty();
+    if (frames) make_dir(flags.frames_dir);
+    mpm::Renderer renderer(simulation, flags.frames_dir);
+    uint32_t view_id = 0;
+    if (frames && simulation.active_particle_count > 0) renderer.render(view_id++);  // main.cu:76
     const uint32_t save_every = std::max<uint32_t>(1, (uint32_t)(1.0 / 240.0 / flags.dt));
     uint32_t frame_id = 0;
     const auto t0 = std::chrono::steady_clock::now();
@@ -103,6 +90,7 @@ int main(int argc, char* argv[]) {
       }
       simulation.advance();
       if (i % flags.sync_every == 0) simulation.syncDevice();
+      if (frames && i % 20 == 0 && simulation.active_particle_count > 0) renderer.render(view_id++);  // main.cu:99-102
       if (save && i % save_every == 0) {
         if (i % flags.sync_every != 0) simulation.syncDevice();
         char name[64];
@@ -110,11 +98,8 @@ int main(int argc, char* argv[]) {
           std::snprintf(name, sizeof(name), "/meshes/mesh_%05u.obj", frame_id);
           mesher.computeMesh(flags.save_dir + name);
         }
-        std::snprintf(name, sizeof(name), "/particles/particles_%05u.ply", frame_id++);
-        if (!write_ply(flags.save_dir + name, simulation.getActiveParticleList())) {
-          std::fprintf(stderr, "cannot write %s%s\n", flags.save_dir.c_str(), name);
-          return 1;
-        }
+        std::snprintf(name, sizeof(name), "/particles/particles_%u.bgeo", frame_id++);
+        writer.writeParticles(flags.save_dir + name, simulation.getActiveParticleList());
       }
     }
     simulation.syncDevice();

@@ -191,6 +191,23 @@ class Engine:
     def write_mesh_obj(self, path: str) -> None:
         self._chk(lib().mpm_mesh_write_obj(self._h, str(path).encode()))
 
+    def render(self, width: int = 800, height: int = 600, stream=None) -> np.ndarray:
+        """The reference's debug view of the device particles (mpm_render_frame):
+        (height, width, 3) uint8, rows top to bottom."""
+        img = np.empty((height, width, 3), np.uint8)
+        self._chk(lib().mpm_render_frame(self._h, width, height, img.ctypes.data, _stream(stream)))
+        return img
+
+    def mesh_indexed(self):
+        """The last mesh as (V (nv, 3) float64, F (nf, 3) int32): one vertex
+        per crossed grid edge (mpm_mesh_indexed)."""
+        nv, nf = C.c_int64(), C.c_int64()
+        self._chk(lib().mpm_mesh_indexed(self._h, C.byref(nv), C.byref(nf), None, None))
+        V = np.empty((nv.value, 3), np.float64)
+        F = np.empty((nf.value, 3), np.int32)
+        self._chk(lib().mpm_mesh_indexed(self._h, C.byref(nv), C.byref(nf), V.ctypes.data, F.ctypes.data))
+        return V, F
+
     def count(self) -> int:
         v = C.c_int64()
         self._chk(lib().mpm_count(self._h, C.byref(v)))

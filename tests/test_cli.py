@@ -144,7 +144,7 @@ def test_reference_readme_command_line(rubber_duck, tmp_path, gpu_lib):
             "--save-dir", out, "--no-mesh")
     assert r.returncode == 0, r.stdout + r.stderr
     assert "grid: N=16," in r.stdout
-    frames = sorted((out / "particles").glob("particles_*.ply"))
+    frames = particle_frames(out)
     assert len(frames) == 1  # 240 frames per second: substep 0 only
     sc = load_scene(str(rubber_duck), 10000, seed=1)
     a = sc.objects[0]
@@ -156,8 +156,8 @@ def test_reference_readme_command_line(rubber_duck, tmp_path, gpu_lib):
     eng.advance(1)
     ref = eng.download()
     eng.close()
-    x0, v0, _ = read_ply(frames[0])
-    assert x0.shape[0] == a.count
+    x0, v0, ids = read_bgeo(frames[0])
+    assert x0.shape[0] == a.count and np.array_equal(ids, np.arange(a.count))
     assert np.array_equal(x0, ref["x"]) and np.array_equal(v0, ref["v"])
 
 
@@ -166,12 +166,28 @@ def test_missing_scene_fails_cleanly(tmp_path):
     assert r.returncode == 1 and "cannot open" in r.stderr
 
 
-def read_ply(path):
-    data = Path(path).read_bytes()
-    head, _, body = data.partition(b"end_header\n")
-    n = int([l for l in head.decode().splitlines() if l.startswith("element vertex")][0].split()[-1])
-    rec = np.frombuffer(body, dtype=np.dtype([("xv", "<f4", 6), ("id", "<i4")]), count=n)
-    return rec["xv"][:, :3].copy(), rec["xv"][:, 3:].copy(), rec["id"].copy()
+def particle_frames(out):
+    """particles_<frame>.bgeo files (frame numbers not zero-padded, as the
+    reference names them, main.cu:109) in frame order"""
+    return sorted((out / "particles").glob("particles_*.bgeo"), key=lambda p: int(p.stem.split("_")[1]))
+
+
+def read_bgeo(path):
+    """x, v, id of a .bgeo the program wrote (id, velocity, radius attributes)."""
+    import gzip
+    import struct
+    raw = gzip.decompress(Path(path).read_bytes())
+    assert raw[:5] == b"BgeoV"
+    n = struct.unpack(">i", raw[9:13])[0]
+    at = 41
+    for _ in range(3):  # id, velocity, radius
+        (ln,) = struct.unpack(">H", raw[at:at + 2])
+        size = struct.unpack(">H", raw[at + 2 + ln:at + 4 + ln])[0]
+        at += 8 + ln + 4 * size
+    rec = np.frombuffer(raw[at:at + 36 * n], dtype=np.dtype([("p", ">f4", 4), ("id", ">i4"), ("v", ">f4", 3),
+                                                              ("r", ">f4")]))
+    assert raw[at + 36 * n:] == b"\x00\xff" and np.all(rec["r"] == np.float32(0.1))
+    return rec["p"][:, :3].astype(np.float32), rec["v"].astype(np.float32), rec["id"].astype(np.int32)
 
 
 @pytest.mark.gpu
@@ -182,24 +198,32 @@ def test_lifetime_activation_matches_python_binding(scene, tmp_path, gpu_lib):
     out = tmp_path / "out"
     pc, dt, N, steps = 60000, 1e-3, 32, 30
     r = run("--scene", scene, "--particle-count", pc, "--dt", dt, "-N", N, "--steps", steps,
-            "--save-dir", out, "--sync-every", 20, "--mesh-grid", 64, "--mesh-particle-radius", 2)
+            "--save-dir", out, "--sync-every", 20, "--mesh-grid", 64, "--mesh-particle-radius", 2,
+            "--laplacian_smooth", 1, "--mesh-face-count", 500, "--frames-dir", tmp_path / "view")
     assert r.returncode == 0, r.stdout + r.stderr
-    frames = sorted((out / "particles").glob("particles_*.ply"))
+    frames = particle_frames(out)
     save_every = int(1.0 / 240.0 / np.float32(dt))
     assert len(frames) == (steps + save_every - 1) // save_every
     # a surface mesh per frame (MeshBuilder): OBJ with vertices and faces
     meshes = sorted((out / "meshes").glob("mesh_*.obj"))
     assert len(meshes) == len(frames)
+    # smoothed (area 1) and decimated to at most 500 faces
     text = meshes[-1].read_text().splitlines()
-    nv = sum(1 for l in text if l.startswith("v "))
-    faces = np.array([[int(v) for v in l.split()[1:]] for l in text if l.startswith("f ")])
-    assert nv > 100 and len(faces) > 100 and faces.min() == 1 and faces.max() == nv
+    V = np.array([[float(v) for v in l.split()[1:]] for l in text if l.startswith("v ")])
+    faces = np.array([[int(v) for v in l.split()[1:]] for l in text if l.startswith("f ")]) - 1
+    assert len(V) > 50 and 490 <= len(faces) <= 500 and faces.min() == 0 and faces.max() == len(V) - 1
+    area = 0.5 * np.linalg.norm(np.cross(V[faces[:, 1]] - V[faces[:, 0]], V[faces[:, 2]] - V[faces[:, 0]]), axis=1).sum()
+    assert 0.5 < area < 1.5  # the smoothed mesh has area 1; decimation changes it a little
+    # the debug view every 20 substeps and once before the first (main.cu:76, 99-102)
+    views = sorted((tmp_path / "view").glob("frame_*.ppm"))
+    assert len(views) == 1 + (steps + 19) // 20
+    assert views[0].read_bytes().startswith(b"P6\n800 600\n255\n")
     sc = load_scene(str(scene), pc, seed=1)
     a, b = sc.objects
     # frame k is written after substep i = k * save_every, i.e. after i + 1 substeps
-    x0, _, _ = read_ply(frames[0])
+    x0, _, _ = read_bgeo(frames[0])
     assert x0.shape[0] == a.count  # the second object starts at t = 0.01
-    xl, vl, _ = read_ply(frames[-1])
+    xl, vl, _ = read_bgeo(frames[-1])
     assert xl.shape[0] == a.count + b.count
     nsub = (len(frames) - 1) * save_every + 1
     # the same sequence through the Python binding

@@ -2,16 +2,21 @@
 
   * the distance field equals the oracle's restatement of the reference's
     fillVoxelGrid_distance bit for bit (oracle/mesh.py);
-  * the marching-tetrahedra surface of one particle is a closed, consistently
-    oriented sphere: every edge in two triangles, once each way, Euler
-    characteristic 2, vertices near the iso distance r voxel_dx, enclosed volume
-    (divergence theorem, positive: normals out) near 4/3 pi (r voxel_dx)^3;
+  * the marching-cubes triangles equal the oracle's restatement
+    (oracle.mesh.marching_cubes on the downloaded field) bit for bit, in the
+    same order;
+  * the surface of one particle is a closed, consistently oriented sphere:
+    every edge in two triangles, once each way, Euler characteristic 2,
+    vertices near the iso distance r voxel_dx, enclosed volume (divergence
+    theorem, positive: normals out) near 4/3 pi (r voxel_dx)^3;
   * every vertex is the zero of S = r voxel_dx - d interpolated along its grid
     edge, from the downloaded field;
   * a scene's mesh is closed, and its OBJ file lists one vertex per crossed
-    grid edge and the same faces.
-The reference meshes with libigl's marching cubes (absent here): its
-triangles are not reproduced, the surface is (parity by these properties).
+    grid edge and the same faces; smoothed and decimated as MeshBuilder does
+    (--laplacian_smooth, --mesh-face-count) it stays closed.
+The reference meshes with libigl's marching cubes (absent here, with its own
+case table): its triangles are not reproduced, the surface is (parity by
+these properties).
 """
 import math
 from collections import Counter
@@ -63,6 +68,23 @@ def test_distance_field_matches_oracle(gpu_lib):
     e.close()
     want = distance_field(x, N, side, r)
     assert np.array_equal(got, want)
+
+
+def test_marching_cubes_matches_oracle(gpu_lib):
+    from oracle.mesh import marching_cubes
+    N, side, r = 32, 40, 3
+    rng = np.random.default_rng(9)
+    x = (0.25 + 0.5 * rng.random((200, 3))).astype(np.float32)
+    e = engine_with(x, N)
+    m = e.mesh(side, r)
+    d = e.mesh_distance_field(side)
+    e.close()
+    _, _, iso = voxel_geometry(N, side, r)
+    vdx = float(N * np.float32(1.0 / N)) / side
+    tri, keys = marching_cubes(iso - d.astype(np.float64), vdx)
+    assert len(tri) > 500
+    assert np.array_equal(m["edges"], keys)
+    assert np.array_equal(m["triangles"], tri)
 
 
 def test_single_particle_sphere(gpu_lib):
@@ -119,6 +141,29 @@ def test_scene_mesh_is_closed_and_obj_matches(gpu_lib, tmp_path):
     assert len(verts) == len(np.unique(keys)) and len(faces) == len(tri)
     assert faces.min() == 1 and faces.max() == len(verts)
     assert np.allclose(verts[faces - 1], tri, rtol=0, atol=1e-6)
+
+
+def test_mesh_pipeline_smooth_and_decimate(gpu_lib, tmp_path):
+    from mpm_amd import meshing
+    sc = scenes.snow_spheres(n=40_000, N=32)
+    st = sc.state()
+    e = Engine(sc.params)
+    e.set_materials(sc.materials)
+    e.add_particles(st["x"], st["material"], st["v"], st["C"], st["F"], st["Jp"])
+    e.upload()
+    m = e.mesh(64, 2)
+    V, F = e.mesh_indexed()
+    e.close()
+    assert len(F) == len(m["triangles"]) and len(V) == len(np.unique(m["edges"]))
+    assert np.allclose(V[F], m["triangles"], rtol=0, atol=0)
+    check_closed(F)
+    U = meshing.smooth(V, F)
+    area = 0.5 * np.linalg.norm(np.cross(U[F[:, 1]] - U[F[:, 0]], U[F[:, 2]] - U[F[:, 0]]), axis=1).sum()
+    assert abs(area - 1.0) < 1e-9
+    U2, G = meshing.decimate(U, F, len(F) // 4)
+    assert len(F) // 4 - 2 <= len(G) <= len(F) // 4
+    check_closed(G)
+    meshing.write_obj(str(tmp_path / "m.obj"), U2, G)
 
 
 def test_mesh_errors_and_empty_set(gpu_lib):
