@@ -260,7 +260,7 @@ int run_scenes_fused(mpm_batch_s* b, int32_t n, bool eager) {
     BCHK(hipMemcpyAsync(b->d_tables, b->h_tables, b->tables_cap, hipMemcpyHostToDevice, b->origin),
          "hipMemcpyAsync(batch tables)");
   std::vector<mpmk::ReduceDesc> rd(nsc);
-  std::vector<uint32_t> rp(nsc + 1), cp(nsc + 1);
+  std::vector<uint32_t> rp(nsc + 1);
   std::vector<mpmk::ItemDesc> gd[2];
   std::vector<uint32_t> gp[2];
   std::vector<mpmk::ItemDesc> done(nsc);
@@ -277,20 +277,15 @@ int run_scenes_fused(mpm_batch_s* b, int32_t n, bool eager) {
     //    blocks with several items were summed into one (small dense scenes
     //    hold dozens of items per block)
     rp[0] = 0;
-    cp[0] = 0;
     for (size_t j = 0; j < nsc; ++j) {
       rd[j] = b->scenes[j]->reduce_desc();
       rd[j].combined = 1;
       rp[j + 1] = rp[j] + (uint32_t)b->scenes[j]->node_blocks();
-      cp[j + 1] = cp[j] + (uint32_t)rd[j].nblk;
     }
     const void* rdt = tw.put(rd.data(), sizeof(mpmk::ReduceDesc) * nsc);
     const void* rpt = tw.put(rp.data(), sizeof(uint32_t) * (nsc + 1));
-    const void* cpt = tw.put(cp.data(), sizeof(uint32_t) * (nsc + 1));
-    if (!rdt || !rpt || !cpt) return tw.failed();
-    BCHK(mpmk::tile_combine_multi((const mpmk::ReduceDesc*)rdt, (const uint32_t*)cpt, (int)nsc, cp[nsc],
-                                  b->origin),
-         "multi-scene tile combine");
+    if (!rdt || !rpt) return tw.failed();
+    BCHK(mpmk::tile_combine_multi((const mpmk::ReduceDesc*)rdt, (int)nsc, b->origin), "multi-scene tile combine");
     BCHK(mpmk::tile_reduce_multi((const mpmk::ReduceDesc*)rdt, (const uint32_t*)rpt, (int)nsc, rp[nsc],
                                  b->origin),
          "multi-scene grid update");

@@ -498,4 +498,21 @@ __device__ __forceinline__ void polar_rotation(const float* F, float* R) {
   mul_abt(U, V, R);
 }
 
+// ---------------------------------------------------------------------------
+// Multi-scene launches: index g (a workgroup or a block of the launch) lies in
+// scene j when prefix[j] <= g < prefix[j + 1]. Every lane of the wavefront
+// calls this with the same g: lane l tests scene c + l of each chunk of 64
+// scenes, so one round trip of loads finds the scene (a binary search takes
+// log2(scenes) dependent ones). g must be below prefix[nscenes].
+__device__ __forceinline__ int scene_of_wave(const uint32_t* __restrict__ prefix, int nscenes, uint32_t g) {
+  const int lane = (int)(threadIdx.x & 63u);
+  for (int c = 0; c < nscenes; c += 64) {
+    const int j = c + lane;
+    const bool hit = j < nscenes && prefix[j] <= g && g < prefix[j + 1];
+    const uint64_t m = __ballot(hit);
+    if (m != 0ull) return c + __ffsll((long long)m) - 1;
+  }
+  return nscenes - 1;
+}
+
 }  // namespace mpmdev

@@ -90,7 +90,7 @@ Sim::~Sim() {
   }
   if (d_hist_) (void)hipFree(d_hist_);
   if (d_mats_) (void)hipFree(d_mats_);
-  for (uint32_t* p : {d_block_start_, d_block_items_, d_flag_, d_pos_, d_gather_, d_item_start_,
+  for (uint32_t* p : {d_block_start_, d_block_items_, d_crowded_, d_flag_, d_pos_, d_gather_, d_item_start_,
                       d_item_block_, d_item_waves_, d_item_count_, d_dirty_, d_covered_, d_live_,
                       d_sdf_, d_mcount_, d_moff_, d_mscratch_})
     if (p) (void)hipFree(p);
@@ -220,13 +220,15 @@ int Sim::ensure_grids() {
   while ((1 << bits) < (int)nbx) ++bits;
   nblk_ = (int64_t)1 << (3 * bits);
   if ((size_t)nblk_ > block_start_cap_) {
-    for (uint32_t** p : {&d_block_start_, &d_block_items_}) {
+    for (uint32_t** p : {&d_block_start_, &d_block_items_, &d_crowded_}) {
       if (*p) (void)hipFree(*p);
       *p = nullptr;
     }
     HIPCHK(hipMalloc(&d_block_start_, 2 * sizeof(uint32_t) * nblk_), "hipMalloc(block ranges)");
     HIPCHK(hipMalloc(&d_block_items_, 2 * sizeof(uint32_t) * nblk_), "hipMalloc(block items)");
     HIPCHK(hipMemset(d_block_items_, 0, 2 * sizeof(uint32_t) * nblk_), "hipMemset(block items)");
+    HIPCHK(hipMalloc(&d_crowded_, sizeof(uint32_t) * (nblk_ + 1)), "hipMalloc(crowded blocks)");
+    HIPCHK(hipMemset(d_crowded_, 0, sizeof(uint32_t) * (nblk_ + 1)), "hipMemset(crowded blocks)");
     block_start_cap_ = (size_t)nblk_;
   }
   items_valid_ = false;
@@ -773,6 +775,7 @@ int Sim::sort_particles(hipStream_t s) {
   if (P_ <= 0) {
     // no items: every home block has an empty item range, no node block is reached
     HIPCHK(mpmk::fill_u32(d_block_items_, 2 * nblk_, 0u, s), "clear block items");
+    HIPCHK(mpmk::fill_u32(d_crowded_, 1, 0u, s), "clear crowded blocks");
     HIPCHK(mpmk::fill_u32(d_covered_, (int64_t)dirty_cap_, 0u, s), "clear node block flags");
     steps_since_sort_ = 0;
     items_valid_ = true;
@@ -789,7 +792,7 @@ int Sim::sort_particles(hipStream_t s) {
     HIPCHK(mpmk::build_items(d_keys_[out], d_vals_[out], P_, lb_, chunk_, nblk_, d_block_start_,
                              d_block_items_, d_covered_, (k_.N + 3) / 4, d_flag_, d_pos_, d_hist_,
                              d_item_count_, d_item_start_, d_item_block_, d_item_waves_, d_gather_,
-                             items, s),
+                             items, d_crowded_, s),
            "work items");
   });
   // the particles move to their new slots in the next tiled pass
@@ -821,6 +824,7 @@ int Sim::sort_desc(mpmk::SortDesc& d) {
   d.block_range = d_block_start_;
   d.block_items = d_block_items_;
   d.covered = d_covered_;
+  d.crowded = d_crowded_;
   d.nblk = nblk_;
   d.nbx = (k_.N + 3) / 4;
   d.flag = d_flag_;
@@ -947,6 +951,7 @@ mpmk::ReduceDesc Sim::reduce_desc() const {
   r.update = 1;
   r.combined = 0;
   r.nblk = nblk_;
+  r.crowded = d_crowded_;
   r.live = d_live_;
   r.skip_dead = 1;
   return r;

@@ -257,6 +257,7 @@ class Sim {
   int64_t nblk_ = 0;                   // Morton space of 4^3 home blocks
   uint32_t* d_block_start_ = nullptr;  // [first, end) particles of each home block
   uint32_t* d_block_items_ = nullptr;  // [first, end) items of each home block
+  uint32_t* d_crowded_ = nullptr;      // count, then the home blocks with several items
   size_t block_start_cap_ = 0;
   uint32_t* d_flag_ = nullptr;         // cap entries
   uint32_t* d_pos_ = nullptr;          // cap entries

@@ -60,7 +60,7 @@ hipError_t build_items(const uint32_t* keys, const uint32_t* perm, int64_t n, in
                        int64_t nblk, uint32_t* block_range, uint32_t* block_items, uint32_t* covered,
                        int nbx, uint32_t* flag, uint32_t* pos, uint32_t* scratch, uint32_t* total,
                        uint32_t* item_start, uint32_t* item_block, uint32_t* item_waves,
-                       uint32_t* gather, int64_t max_items, hipStream_t s);
+                       uint32_t* gather, int64_t max_items, uint32_t* crowded, hipStream_t s);
 // Removes the particles with original index in [lo, hi): the others packed
 // into dst in their order, indices above the range lowered by hi - lo; the
 // removed ones (when `gone` is non-null) into planes of hi - lo floats at
@@ -126,7 +126,10 @@ struct ReduceDesc {
   int nbx;
   int update;
   int combined;   // the items of each home block were summed into its first tile
-  int64_t nblk;   // home blocks (Morton space): the combine's workgroups
+  int64_t nblk;   // home blocks (Morton space)
+  // crowded[0]: the number of home blocks with several items, crowded[1..]:
+  // those blocks, in no particular order (built at the sort)
+  const uint32_t* crowded;
   // live[nb] = some tile or side-grid scatter reaches node block nb; with
   // skip_dead the other blocks keep their old values instead of zeros (no
   // particle stencil reads them: a particle's G2P reads the nodes its own
@@ -161,8 +164,7 @@ hipError_t zero_dead_blocks(float4* grid, const uint32_t* live, int N, hipStream
 // sums the tiles of every home block holding several items into its first
 // item's tile, in item order (prefix over the scenes' nblk); a reduction with
 // `combined` set then reads one tile per home block
-hipError_t tile_combine_multi(const ReduceDesc* descs, const uint32_t* prefix, int nscenes,
-                              int64_t total_blocks, hipStream_t s);
+hipError_t tile_combine_multi(const ReduceDesc* descs, int nscenes, hipStream_t s);
 // One scene's cell sort and work-item build (the arguments of the sort
 // kernels), for sorts of many scenes in one launch per kernel.
 struct SortDesc {
@@ -187,12 +189,13 @@ struct SortDesc {
   uint32_t* gather;
   int chunk;
   int64_t max_items;
+  uint32_t* crowded;  // the home blocks with several items: count, then the list
 };
 // the sort as a sequence of phases; SORT_HIST/RSCAN/SCATTER run once per
 // 8-bit pass (pass = 0, 1, ...)
 enum SortPhase : int {
   SORT_FILL = 0, SORT_KEYS, SORT_HIST, SORT_RSCAN, SORT_SCATTER, SORT_HEADS, SORT_FLAGS,
-  SORT_SCAN_TILES, SORT_SCAN_SUMS, SORT_SCAN_ADD, SORT_EMIT, SORT_ARRANGE, SORT_NUM_PHASES
+  SORT_SCAN_TILES, SORT_SCAN_SUMS, SORT_SCAN_ADD, SORT_EMIT, SORT_ARRANGE, SORT_CROWDED, SORT_NUM_PHASES
 };
 // workgroups of one scene in a phase (0: the scene sits the phase out)
 int64_t sort_blocks(const SortDesc& d, int phase, int pass);

@@ -692,24 +692,13 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
   g2p2g_item<KS>(d, (int)((d.flags & mpmk::TILED_XCD) ? xcd_order(b, gridDim.x, *d.item_count) : b));
 }
 
-// the scene of global workgroup or block g: the last j with prefix[j] <= g
-__device__ __forceinline__ int scene_of(const uint32_t* __restrict__ prefix, int nscenes, uint32_t g) {
-  int lo = 0, hi = nscenes;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (prefix[mid] <= g) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
 template <int KS>
 __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
     __attribute__((amdgpu_waves_per_eu(KS == KINDS_LIQUID ? 5 : 4)))
     k_g2p2g_multi(const mpmk::ItemDesc* __restrict__ descs, const uint32_t* __restrict__ prefix,
                   int nscenes) {
   const uint32_t g = blockIdx.x;
-  const int j = scene_of(prefix, nscenes, g);
+  const int j = scene_of_wave(prefix, nscenes, g);
   g2p2g_item<KS>(descs[j], (int)(g - prefix[j]));
 }
 
@@ -849,22 +838,23 @@ __device__ __forceinline__ void combine_block(const mpmk::ReduceDesc& d, const i
   const uint32_t r0 = d.block_items[2 * (size_t)b], r1 = d.block_items[2 * (size_t)b + 1];
   if (r1 <= r0 + 1) return;  // uniform for the workgroup
   float4* const t0 = const_cast<float4*>(d.tiles) + (size_t)r0 * TN;  // the tiles are the engine's scratch
-  // each thread sums its two nodes over the block's items in item order, 8
+  // each thread sums its two nodes over the block's items in item order, 16
   // items of both nodes in flight at a time (a crowded block holds dozens)
   static_assert(TN == 2 * kCombineThreads, "two tile nodes per thread");
   const int n0 = threadIdx.x, n1 = threadIdx.x + kCombineThreads;
   float4 a0 = t0[n0], a1 = t0[n1];
   const float4* __restrict__ const tiles = d.tiles;
   uint32_t it = r0 + 1;
-  for (; it + 8 <= r1; it += 8) {
-    float4 v[8], u[8];
+  constexpr int B = 16;  // items of both nodes in flight
+  for (; it + B <= r1; it += B) {
+    float4 v[B], u[B];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < B; ++q) {
       v[q] = tiles[(size_t)(it + q) * TN + n0];
       u[q] = tiles[(size_t)(it + q) * TN + n1];
     }
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < B; ++q) {
       a0.x += v[q].x; a0.y += v[q].y; a0.z += v[q].z; a0.w += v[q].w;
       a1.x += u[q].x; a1.y += u[q].y; a1.z += u[q].z; a1.w += u[q].w;
     }
@@ -878,36 +868,35 @@ __device__ __forceinline__ void combine_block(const mpmk::ReduceDesc& d, const i
   t0[n1] = a1;
 }
 
-// kCombineBlocks home blocks per workgroup: lanes of wavefront 0 look up
-// their blocks together and the workgroup combines the crowded ones in turn
-// (most blocks hold at most one item: one workgroup per block spent its time
-// launching workgroups, many per workgroup serialised crowded neighbours)
-constexpr int kCombineBlocks = 4;
+// The crowded home blocks (several items) of every scene, from the lists the
+// sorts built: the scenes' lists are taken as one concatenation, and
+// workgroup w combines the blocks whose index in it is w modulo the grid.
+// Every wavefront reads the scenes' counts lane by lane (64 scenes per round
+// trip) and their running sum, so all 4 take the same (uniform) loop.
+constexpr int kCombineGrid = 1024;
 __global__ void __launch_bounds__(kCombineThreads)
-    k_tile_combine_multi(const mpmk::ReduceDesc* __restrict__ descs, const uint32_t* __restrict__ prefix,
-                         int nscenes) {
-  static_assert(kCombineBlocks <= 64, "one lane per block");
-  __shared__ int s_scene[kCombineBlocks];
-  __shared__ uint32_t s_block[kCombineBlocks];
-  __shared__ uint32_t s_mask;
-  if (threadIdx.x < 64) {
-    const uint32_t g = blockIdx.x * (uint32_t)kCombineBlocks + threadIdx.x;
-    bool need = false;
-    if (threadIdx.x < kCombineBlocks && g < prefix[nscenes]) {
-      const int j = scene_of(prefix, nscenes, g);
-      const uint32_t b = g - prefix[j];
-      const uint2 r = *reinterpret_cast<const uint2*>(descs[j].block_items + 2 * (size_t)b);
-      need = r.y > r.x + 1;
-      s_scene[threadIdx.x] = j;
-      s_block[threadIdx.x] = b;
+    k_tile_combine_multi(const mpmk::ReduceDesc* __restrict__ descs, int nscenes) {
+  const int lane = (int)(threadIdx.x & 63u);
+  const uint32_t G = gridDim.x, w = blockIdx.x;
+  uint32_t base = 0;  // blocks of the scenes before this chunk of 64
+  for (int c = 0; c < nscenes; c += 64) {
+    const int j = c + lane;
+    const uint32_t cnt = j < nscenes ? descs[j].crowded[0] : 0u;
+    uint32_t incl = cnt;  // inclusive sum over the lanes
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t u = (uint32_t)__shfl_up((int)incl, o);
+      if (lane >= o) incl += u;
     }
-    const uint64_t m = __ballot(need);
-    if (threadIdx.x == 0) s_mask = (uint32_t)m;
-  }
-  __syncthreads();
-  for (uint32_t m = s_mask; m != 0u; m &= m - 1u) {
-    const int q = __builtin_ctz(m);
-    combine_block(descs[s_scene[q]], (int64_t)s_block[q]);
+    const int m = min(64, nscenes - c);
+    for (int q = 0; q < m; ++q) {
+      const uint32_t cq = (uint32_t)__shfl((int)cnt, q);
+      if (cq == 0u) continue;
+      const uint32_t first = base + (uint32_t)__shfl((int)incl, q) - cq;  // index of the scene's first block
+      const mpmk::ReduceDesc& d = descs[c + q];
+      for (uint32_t e = (w + G - first % G) % G; e < cq; e += G) combine_block(d, (int64_t)d.crowded[1 + e]);
+    }
+    base += (uint32_t)__shfl((int)incl, m - 1);
   }
 }
 
@@ -917,7 +906,7 @@ __global__ void __launch_bounds__(256)
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t g = blockIdx.x * 4u + (uint32_t)wave;
   if (g >= prefix[nscenes]) return;  // uniform for the wavefront
-  const int j = scene_of(prefix, nscenes, g);
+  const int j = scene_of_wave(prefix, nscenes, g);
   reduce_block<false>(descs[j], (int64_t)(g - prefix[j]), lane);
 }
 
@@ -976,12 +965,10 @@ hipError_t g2p2g_multi(const ItemDesc* descs, const uint32_t* prefix, int nscene
   return hipGetLastError();
 }
 
-hipError_t tile_combine_multi(const ReduceDesc* descs, const uint32_t* prefix, int nscenes,
-                              int64_t total_blocks, hipStream_t s) {
-  if (nscenes <= 0 || total_blocks <= 0) return hipSuccess;
-  if (!descs || !prefix || total_blocks > 0x7fffffff) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_tile_combine_multi, dim3((unsigned)((total_blocks + kCombineBlocks - 1) / kCombineBlocks)),
-                     dim3(kCombineThreads), 0, s, descs, prefix, nscenes);
+hipError_t tile_combine_multi(const ReduceDesc* descs, int nscenes, hipStream_t s) {
+  if (nscenes <= 0) return hipSuccess;
+  if (!descs) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_tile_combine_multi, dim3(kCombineGrid), dim3(kCombineThreads), 0, s, descs, nscenes);
   return hipGetLastError();
 }
 
@@ -1042,6 +1029,7 @@ hipError_t tile_reduce(const TileSums& in, const StepConsts& k, float4* grid, in
   d.update = update;
   d.combined = 0;
   d.nblk = 0;
+  d.crowded = nullptr;
   d.live = in.live;
   d.skip_dead = update && in.live;
   const int64_t nblocks = (int64_t)d.nbx * d.nbx * d.nbx;

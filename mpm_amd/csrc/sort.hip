@@ -151,6 +151,21 @@ __global__ void __launch_bounds__(256)
                  blockIdx.x);
 }
 
+// the home blocks with several items: crowded[0] counts them (zero
+// beforehand), crowded[1..] lists them (in no particular order)
+__device__ __forceinline__ void crowded_body(const uint32_t* __restrict__ block_items, int64_t nblk,
+                                             uint32_t* __restrict__ crowded, uint32_t blk) {
+  const int64_t b = (int64_t)blk * 256 + threadIdx.x;
+  if (b >= nblk) return;
+  const uint2 r = reinterpret_cast<const uint2*>(block_items)[b];
+  if (r.y > r.x + 1u) crowded[1 + atomicAdd(crowded, 1u)] = (uint32_t)b;
+}
+
+__global__ void __launch_bounds__(256)
+    k_crowded(const uint32_t* __restrict__ block_items, int64_t nblk, uint32_t* __restrict__ crowded) {
+  crowded_body(block_items, nblk, crowded, blockIdx.x);
+}
+
 __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -580,23 +595,13 @@ hipError_t radix_sort_t(uint32_t* keys[2], uint32_t* vals[2], int64_t n, int sta
 }
 
 // ---- many scenes per launch: workgroup g of the launch is workgroup
-// g - prefix[j] of scene j (prefix[j] <= g < prefix[j + 1])
-__device__ __forceinline__ int sort_scene_of(const uint32_t* __restrict__ prefix, int nscenes, uint32_t g) {
-  int lo = 0, hi = nscenes;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (prefix[mid] <= g) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
+// g - prefix[j] of scene j (prefix[j] <= g < prefix[j + 1]; scene_of_wave)
 template <int PH>
 __global__ void __launch_bounds__(256)
     k_sort_multi(const mpmk::SortDesc* __restrict__ descs, const uint32_t* __restrict__ prefix, int nscenes,
                  int pass) {
   const uint32_t g = blockIdx.x;
-  const int j = sort_scene_of(prefix, nscenes, g);
+  const int j = mpmdev::scene_of_wave(prefix, nscenes, g);
   const mpmk::SortDesc& d = descs[j];
   const uint32_t blk = g - prefix[j];
   const int out = d.npasses & 1;
@@ -604,6 +609,7 @@ __global__ void __launch_bounds__(256)
   if constexpr (PH == mpmk::SORT_FILL) {
     const int64_t i = (int64_t)blk * 256 + threadIdx.x;
     const int64_t nnb = (int64_t)d.nbx * d.nbx * d.nbx;
+    if (i == 0) d.crowded[0] = 0u;
     if (i < 2 * d.nblk) d.block_items[i] = 0u;
     else if (i < 2 * d.nblk + nnb) d.covered[i - 2 * d.nblk] = 0u;
   } else if constexpr (PH == mpmk::SORT_KEYS) {
@@ -632,9 +638,11 @@ __global__ void __launch_bounds__(256)
   } else if constexpr (PH == mpmk::SORT_EMIT) {
     item_emit_body(d.keys[out], d.n, shift, d.flag, d.pos, d.item_count, d.item_start, d.item_block,
                    d.block_items, d.covered, d.nbx, blk);
-  } else {
+  } else if constexpr (PH == mpmk::SORT_ARRANGE) {
     item_arrange_body(d.keys[out], d.vals[out], shift, d.item_start, d.item_count, d.gather,
                       d.item_waves, blk);
+  } else {
+    crowded_body(d.block_items, d.nblk, d.crowded, blk);
   }
 }
 
@@ -652,6 +660,7 @@ int64_t sort_blocks(const SortDesc& d, int phase, int pass) {
     case SORT_SCAN_TILES: case SORT_SCAN_ADD: return tiles;
     case SORT_SCAN_SUMS: return 1;
     case SORT_ARRANGE: return d.max_items;
+    case SORT_CROWDED: return (d.nblk + 255) / 256;
     default: return 0;
   }
 }
@@ -676,6 +685,7 @@ hipError_t sort_multi(int phase, int pass, const SortDesc* descs, const uint32_t
     MPM_SORT_CASE(SORT_SCAN_ADD)
     MPM_SORT_CASE(SORT_EMIT)
     MPM_SORT_CASE(SORT_ARRANGE)
+    MPM_SORT_CASE(SORT_CROWDED)
 #undef MPM_SORT_CASE
     default: return hipErrorInvalidValue;
   }
@@ -721,8 +731,9 @@ hipError_t build_items(const uint32_t* keys, const uint32_t* perm, int64_t n, in
                        int64_t nblk, uint32_t* block_range, uint32_t* block_items, uint32_t* covered,
                        int nbx, uint32_t* flag, uint32_t* pos, uint32_t* scratch, uint32_t* total,
                        uint32_t* item_start, uint32_t* item_block, uint32_t* item_waves,
-                       uint32_t* gather, int64_t max_items, hipStream_t s) {
+                       uint32_t* gather, int64_t max_items, uint32_t* crowded, hipStream_t s) {
   if (n <= 0) return hipSuccess;
+  if (!crowded) return hipErrorInvalidValue;
   if (chunk < 1 || chunk > kMaxItemParticles || max_items < 1) return hipErrorInvalidValue;
   const int shift = 3 * lb;
   const unsigned g = (unsigned)((n + 255) / 256);
@@ -730,6 +741,7 @@ hipError_t build_items(const uint32_t* keys, const uint32_t* perm, int64_t n, in
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((2 * nblk + 255) / 256)), dim3(256), 0, s,
                      block_items, 2 * nblk, 0u);
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nnb + 255) / 256)), dim3(256), 0, s, covered, nnb, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, crowded, (int64_t)1, 0u);
   hipLaunchKernelGGL(k_block_heads, dim3(g), dim3(256), 0, s, keys, n, shift, block_range);
   hipLaunchKernelGGL(k_item_flags, dim3(g), dim3(256), 0, s, keys, n, shift, chunk, block_range, flag);
   hipError_t e = exclusive_scan(flag, pos, n, scratch, total, s);
@@ -739,6 +751,7 @@ hipError_t build_items(const uint32_t* keys, const uint32_t* perm, int64_t n, in
   if (!perm || !gather || !item_waves) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_item_arrange, dim3((unsigned)max_items), dim3(256), 0, s, keys, perm, shift,
                      item_start, total, gather, item_waves);
+  hipLaunchKernelGGL(k_crowded, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, s, block_items, nblk, crowded);
   return hipGetLastError();
 }
 
