@@ -274,6 +274,12 @@ int mpm_load_state(mpm_handle h, const char* path);
  *                blocks are; 1: the grid update reads the further work items
  *                of crowded blocks 8 tiles at a time (more registers); 0: one
  *                at a time (more wavefronts resident). Same result either way.
+ *   "combine"    -1 (default): chosen at upload as for reduce_mlp; 1: before
+ *                the grid update a pass sums the tiles of every home block
+ *                holding several work items into its first (one workgroup per
+ *                block, 16 items in flight), and the grid update reads one
+ *                tile per block; 0: the grid update reads every item's tile.
+ *                Same result either way (both sum in item order).
  *   "timing"    k > 0: record HIP events around the kernel phases of one
  *                substep in every k (k = 1: every substep); 0 (default): off
  *   "markers"   1: roctx ranges (rocprofiler-sdk) named "mpm_advance" around

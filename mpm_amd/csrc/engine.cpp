@@ -870,6 +870,16 @@ bool Sim::dense_blocks() const {
 }
 
 bool Sim::use_mlp() const { return opt_.reduce_mlp < 0 ? dense_ : opt_.reduce_mlp != 0; }
+bool Sim::use_combine() const { return opt_.combine < 0 ? dense_ : opt_.combine != 0; }
+
+int Sim::reduce_tiles(float4* grid, int update, hipStream_t s) {
+  const mpmk::TileSums t = tile_sums();
+  const bool comb = use_combine();
+  if (comb) HIPCHK(mpmk::tile_combine(t, nblk_, s), "tile combine");
+  // (combined: one tile per block, which the plain variant reads best)
+  HIPCHK(mpmk::tile_reduce(t, k_, grid, update, comb ? 0 : use_mlp(), comb ? 1 : 0, s), "grid update");
+  return MPM_OK;
+}
 
 bool Sim::derives_f() const {
   // only the fused tiled path: the unfused one's G2P kernel reads the F planes
@@ -998,6 +1008,7 @@ mpmk::TileSums Sim::tile_sums() const {
   t.block_items = d_block_items_;
   t.covered = d_covered_;
   t.live = d_live_;
+  t.crowded = d_crowded_;
   return t;
 }
 
@@ -1037,7 +1048,8 @@ int Sim::substep(hipStream_t s) {
       // reference order: P2G, grid update, G2P
       int rc = tiled_p2g(s);
       if (rc) return rc;
-      PHASE(PH_GRID_UPDATE, HIPCHK(mpmk::tile_reduce(tile_sums(), k_, d_grid_[0], 1, use_mlp(), s), "grid update"));
+      PHASE(PH_GRID_UPDATE, rc = reduce_tiles(d_grid_[0], 1, s));
+      if (rc) return rc;
       PHASE(PH_G2P, HIPCHK(mpmk::g2p(particles(), d_mats_, k_, d_grid_[0], s), "g2p"));
       vel_grid_ = d_grid_[0];
       grid_dead_ = true;
@@ -1051,7 +1063,11 @@ int Sim::substep(hipStream_t s) {
       primed_ = true;
     }
     // the tiles hold the P2G of this substep (items of the current order)
-    PHASE(PH_GRID_UPDATE, HIPCHK(mpmk::tile_reduce(tile_sums(), k_, d_grid_[0], 1, use_mlp(), s), "grid update"));
+    {
+      int rc = MPM_OK;
+      PHASE(PH_GRID_UPDATE, rc = reduce_tiles(d_grid_[0], 1, s));
+      if (rc) return rc;
+    }
     if (!items_valid_ || (opt_.sort_every > 0 && steps_since_sort_ >= opt_.sort_every)) {
       int rc = sort_particles(s);
       if (rc) return rc;
@@ -1218,7 +1234,8 @@ int Sim::debug_p2g(float* grid, hipStream_t s) {
     rc = tiled_p2g(s);
     if (rc) return rc;
     g = d_grid_[0];
-    HIPCHK(mpmk::tile_reduce(tile_sums(), k_, g, 0, use_mlp(), s), "tile sums");
+    rc = reduce_tiles(g, 0, s);
+    if (rc) return rc;
   } else {
     // the non-current grid, so a primed direct-path grid survives
     g = d_grid_[gcur_ ^ 1];
@@ -1489,6 +1506,7 @@ int Sim::set_option(const char* name, int64_t v) {
   if (n == "sort_every") { if (v < 0) return fail(MPM_ERR_INVALID, "sort_every must be >= 0"); opt_.sort_every = v; return MPM_OK; }
   if (n == "derive_f") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "derive_f must be 0 or 1"); opt_.derive_f = v; return MPM_OK; }
   if (n == "reduce_mlp") { if (v < -1 || v > 1) return fail(MPM_ERR_INVALID, "reduce_mlp must be -1, 0 or 1"); opt_.reduce_mlp = v; return MPM_OK; }
+  if (n == "combine") { if (v < -1 || v > 1) return fail(MPM_ERR_INVALID, "combine must be -1, 0 or 1"); opt_.combine = v; return MPM_OK; }
   if (n == "xcd_map") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "xcd_map must be 0 or 1"); opt_.xcd_map = v; return MPM_OK; }
   if (n == "markers") { if (v != 0 && v != 1) return fail(MPM_ERR_INVALID, "markers must be 0 or 1"); opt_.markers = v; return MPM_OK; }
   if (n == "radix_bits") { if (v != 4 && v != 6 && v != 8) return fail(MPM_ERR_INVALID, "radix_bits must be 4, 6 or 8"); opt_.radix_bits = v; return MPM_OK; }
@@ -1529,6 +1547,7 @@ int Sim::get_option(const char* name, int64_t* v) {
   else if (n == "xcd_map") *v = opt_.xcd_map;
   else if (n == "markers") *v = opt_.markers;
   else if (n == "reduce_mlp") *v = opt_.reduce_mlp;
+  else if (n == "combine") *v = opt_.combine;
   else return fail(MPM_ERR_INVALID, "mpm_get_option: unknown option '%s'", name);
   return MPM_OK;
 }

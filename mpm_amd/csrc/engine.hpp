@@ -33,6 +33,9 @@ struct Options {
   int64_t xcd_map = 1;      // "xcd_map": XCD-contiguous order of the fused kernel's items
   int64_t reduce_mlp = -1;  // "reduce_mlp": tile reduction batching further items of a block
                             // (1), one at a time (0), or chosen from the uploaded density (-1)
+  int64_t combine = -1;     // "combine": sum the tiles of each crowded home block into one before
+                            // the tile reduction (1), not (0), or chosen from the uploaded
+                            // density (-1)
   int64_t markers = 0;      // "markers": roctx ranges around each advance call and its phases
   int64_t timing = 0;       // "timing": k > 0 brackets the kernels of one substep in every k
                             // with HIP events (per-phase time); 0 off
@@ -275,6 +278,10 @@ class Sim {
   bool items_valid_ = false;
   int64_t max_items() const;
   mpmk::TileSums tile_sums() const;
+  // the tiles of the last pass (and the side grid) into `grid`: the grid
+  // update (update = 1) or the raw sums; crowded blocks combined first
+  int reduce_tiles(float4* grid, int update, hipStream_t s);
+  bool use_combine() const;
   int tiled_p2g(hipStream_t s);        // P2G of the current state into the tiles
   // the fused kernel on the current particles; performs a pending reorder
   int run_tiled(const float4* grid_v, int do_g2p, hipStream_t s);

@@ -93,6 +93,7 @@ struct TileSums {
   const uint32_t* block_items;  // 2 entries per home block (Morton space)
   const uint32_t* covered;      // per node block: some tile reaches it
   uint32_t* live;               // per node block: written by the last grid update
+  const uint32_t* crowded;      // count, then the home blocks with several items
 };
 // With `gather`, P is the particle order before the last sort: thread slot i
 // reads particle gather[i] of P and writes it to slot i of Pdst (the sort's
@@ -146,10 +147,6 @@ struct ReduceDesc {
 enum TiledFlags : int { TILED_G2P = 1, TILED_DERIVE_F = 2, TILED_GUARD = 4, TILED_XCD = 8 };
 // launches of the XCD-ordered kernels round their grid up to a multiple of this
 constexpr int kXcds = 8;
-hipError_t g2p2g_block(const ParticlesDev& P, const ParticlesDev& Pdst, const uint32_t* gather,
-                       const MaterialDev* mats, int nmats, const StepConsts& k,
-                       const WorkItems& w, const float4* grid_v, const TileSums& out,
-                       int flags, int kinds, hipStream_t s);
 // The same passes over several scenes in one launch: descs[j] is scene j,
 // prefix[j] its first workgroup (prefix[n] = total; workgroup counts are the
 // scenes' max_items for g2p2g, their node-block counts for the reduction).
@@ -207,9 +204,14 @@ hipError_t liquid_f(const ParticlesDev& P, hipStream_t s);
 // into every node of `grid`; update=1 applies the grid update (velocities),
 // update=0 leaves the raw P2G sums (momentum, mass). mlp=1 suits dense scenes
 // (home blocks of several items: their further tiles are read 8 at a time,
-// at the cost of registers); the result does not depend on it.
+// at the cost of registers); the result does not depend on it. combined: the
+// tiles went through tile_combine (one tile per home block is read).
 hipError_t tile_reduce(const TileSums& in, const StepConsts& k, float4* grid, int update, int mlp,
-                       hipStream_t s);
+                       int combined, hipStream_t s);
+// Sums the tiles of every crowded home block (in.crowded, the sort's list)
+// into its first item's tile, in item order: tile_reduce with `combined` then
+// gives the same bits as without the combine.
+hipError_t tile_combine(const TileSums& in, int64_t nblk, hipStream_t s);
 
 // mesh.hip: surface mesh of the particles. The distance field (side^3
 // voxels of edge vdx, float bits) starts at `cutoff` and takes the distance

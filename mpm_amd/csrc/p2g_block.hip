@@ -30,6 +30,8 @@
 // atomics into a side grid that k_tile_reduce adds and clears.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_math.hpp"
 #include "engine_types.hpp"
 #include "kernels.hpp"
@@ -611,25 +613,34 @@ __device__ __forceinline__ void g2p2g_item(const mpmk::ItemDesc& d, const int it
         }
       }
     }
+    // one step of the segmented scan: cv += (lane shifted by CTRL) * f, f = 1
+    // where COND holds, else 0 (exact as in the pair merge: t * 1 = t, and
+    // t * 0 adds a zero, which leaves every tile sum's bits as they are),
+    // one v_fmac_f32_dpp per value; ROWS masks the 16-lane rows written
+#define MPM_SEG_NODE(o, CTRL, ROWS)                                                          \
+    __asm__("s_nop 1\n\t"                                                                   \
+            "v_fmac_f32_dpp %0, %0, %4 " CTRL " row_mask:" ROWS " bank_mask:0xf bound_ctrl:1\n\t" \
+            "v_fmac_f32_dpp %1, %1, %4 " CTRL " row_mask:" ROWS " bank_mask:0xf bound_ctrl:1\n\t" \
+            "v_fmac_f32_dpp %2, %2, %4 " CTRL " row_mask:" ROWS " bank_mask:0xf bound_ctrl:1\n\t" \
+            "v_fmac_f32_dpp %3, %3, %4 " CTRL " row_mask:" ROWS " bank_mask:0xf bound_ctrl:1"       \
+            : "+v"(cv[o][0]), "+v"(cv[o][1]), "+v"(cv[o][2]), "+v"(cv[o][3])                  \
+            : "v"(f_));
 #define MPM_SEG_STEP(CTRL, ROWS, COND)                                                      \
     {                                                                                      \
-      const bool m_ = (COND);                                                              \
-      _Pragma("unroll") for (int o = 0; o < 9; ++o) _Pragma("unroll") for (int q = 0; q < 4; ++q) { \
-        const float t_ = __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(            \
-            0, __builtin_bit_cast(int, cv[o][q]), CTRL, ROWS, 0xf, true));                 \
-        cv[o][q] = m_ ? cv[o][q] + t_ : cv[o][q];                                          \
-      }                                                                                    \
+      const float f_ = (COND) ? 1.0f : 0.0f;                                               \
+      _Pragma("unroll") for (int o = 0; o < 9; ++o) { MPM_SEG_NODE(o, CTRL, ROWS) }        \
     }
     if (reduce) {
-      MPM_SEG_STEP(0x111 /* row_shr:1 */, 0xf, rank >= 1 && lr >= 1)
-      MPM_SEG_STEP(0x112 /* row_shr:2 */, 0xf, rank >= 2 && lr >= 2)
-      MPM_SEG_STEP(0x114 /* row_shr:4 */, 0xf, rank >= 4 && lr >= 4)
-      if (maxrun > 8) MPM_SEG_STEP(0x118 /* row_shr:8 */, 0xf, rank >= 8 && lr >= 8)
+      MPM_SEG_STEP("row_shr:1", "0xf", rank >= 1 && lr >= 1)
+      MPM_SEG_STEP("row_shr:2", "0xf", rank >= 2 && lr >= 2)
+      MPM_SEG_STEP("row_shr:4", "0xf", rank >= 4 && lr >= 4)
+      if (maxrun > 8) MPM_SEG_STEP("row_shr:8", "0xf", rank >= 8 && lr >= 8)
       // carry across rows: rows 1, 3 take lane 15 of the row below, rows 2, 3 lane 31
-      MPM_SEG_STEP(0x142 /* row_bcast:15 */, 0xa, ((lane >> 4) & 1) && rank > lr)
-      MPM_SEG_STEP(0x143 /* row_bcast:31 */, 0xc, lane >= 32 && rank > lane - 32)
+      MPM_SEG_STEP("row_bcast:15", "0xa", ((lane >> 4) & 1) && rank > lr)
+      MPM_SEG_STEP("row_bcast:31", "0xc", lane >= 32 && rank > lane - 32)
     }
 #undef MPM_SEG_STEP
+#undef MPM_SEG_NODE
 #define MPM_RMW_SLAB                                                                       \
     _Pragma("unroll") for (int b = 0; b < 3; ++b) _Pragma("unroll") for (int c = 0; c < 3; ++c) { \
       const int o = b * 3 + c;                                                             \
@@ -838,34 +849,61 @@ __device__ __forceinline__ void combine_block(const mpmk::ReduceDesc& d, const i
   const uint32_t r0 = d.block_items[2 * (size_t)b], r1 = d.block_items[2 * (size_t)b + 1];
   if (r1 <= r0 + 1) return;  // uniform for the workgroup
   float4* const t0 = const_cast<float4*>(d.tiles) + (size_t)r0 * TN;  // the tiles are the engine's scratch
-  // each thread sums its two nodes over the block's items in item order, 16
-  // items of both nodes in flight at a time (a crowded block holds dozens)
+  // each thread sums its two nodes over the block's items in item order:
+  // rounds of 16 items of both nodes in flight (the densest blocks hold
+  // dozens), then the rest (most crowded blocks hold a few) in rounds of 4
+  // whose loads are clamped to the block's last item, so that all 4 are
+  // issued at once. A load past the block is added times 0 (fma(v, 0, a) =
+  // a + (+-0) = a: a tile node is never -0, its sums start from +0), one of
+  // the block times 1 (fma(v, 1, a) = a + v in one rounding): the bits of
+  // the plain sum. An add under a branch would let the compiler sink each
+  // load into it, one load in flight at a time.
   static_assert(TN == 2 * kCombineThreads, "two tile nodes per thread");
   const int n0 = threadIdx.x, n1 = threadIdx.x + kCombineThreads;
   float4 a0 = t0[n0], a1 = t0[n1];
   const float4* __restrict__ const tiles = d.tiles;
   uint32_t it = r0 + 1;
-  constexpr int B = 16;  // items of both nodes in flight
+  constexpr uint32_t B = 16;
   for (; it + B <= r1; it += B) {
     float4 v[B], u[B];
 #pragma unroll
-    for (int q = 0; q < B; ++q) {
+    for (uint32_t q = 0; q < B; ++q) {
       v[q] = tiles[(size_t)(it + q) * TN + n0];
       u[q] = tiles[(size_t)(it + q) * TN + n1];
     }
 #pragma unroll
-    for (int q = 0; q < B; ++q) {
+    for (uint32_t q = 0; q < B; ++q) {
       a0.x += v[q].x; a0.y += v[q].y; a0.z += v[q].z; a0.w += v[q].w;
       a1.x += u[q].x; a1.y += u[q].y; a1.z += u[q].z; a1.w += u[q].w;
     }
   }
-  for (; it < r1; ++it) {
-    const float4 v = tiles[(size_t)it * TN + n0], u = tiles[(size_t)it * TN + n1];
-    a0.x += v.x; a0.y += v.y; a0.z += v.z; a0.w += v.w;
-    a1.x += u.x; a1.y += u.y; a1.z += u.z; a1.w += u.w;
+  constexpr uint32_t T = 4;
+  for (; it < r1; it += T) {
+    float4 v[T], u[T];
+#pragma unroll
+    for (uint32_t q = 0; q < T; ++q) {
+      const size_t j = (size_t)min(it + q, r1 - 1u) * TN;
+      v[q] = tiles[j + n0];
+      u[q] = tiles[j + n1];
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < T; ++q) {
+      const float m = it + q < r1 ? 1.0f : 0.0f;
+      a0.x = __builtin_fmaf(v[q].x, m, a0.x); a0.y = __builtin_fmaf(v[q].y, m, a0.y);
+      a0.z = __builtin_fmaf(v[q].z, m, a0.z); a0.w = __builtin_fmaf(v[q].w, m, a0.w);
+      a1.x = __builtin_fmaf(u[q].x, m, a1.x); a1.y = __builtin_fmaf(u[q].y, m, a1.y);
+      a1.z = __builtin_fmaf(u[q].z, m, a1.z); a1.w = __builtin_fmaf(u[q].w, m, a1.w);
+    }
   }
   t0[n0] = a0;
   t0[n1] = a1;
+}
+
+// The crowded home blocks of one scene (the list its sort built): workgroup
+// w combines the blocks whose index in the list is w modulo the grid.
+__global__ void __launch_bounds__(kCombineThreads) k_tile_combine(mpmk::ReduceDesc d) {
+  const uint32_t cq = d.crowded[0];
+  for (uint32_t e = blockIdx.x; e < cq; e += gridDim.x) combine_block(d, (int64_t)d.crowded[1 + e]);
 }
 
 // The crowded home blocks (several items) of every scene, from the lists the
@@ -926,30 +964,6 @@ hipError_t g2p2g_desc(const ItemDesc& d, int nmats, int64_t launch, int kinds, h
   else
     hipLaunchKernelGGL((k_g2p2g_tiled<KINDS_ALL>), dim3((unsigned)launch), dim3(64 * kItemWaves), 0, s, d);
   return hipGetLastError();
-}
-
-hipError_t g2p2g_block(const ParticlesDev& P, const ParticlesDev& Pdst, const uint32_t* gather,
-                       const MaterialDev* mats, int nmats, const StepConsts& k,
-                       const WorkItems& w, const float4* grid_v, const TileSums& out,
-                       int flags, int kinds, hipStream_t s) {
-  if (P.n <= 0 || w.max_items <= 0) return hipSuccess;
-  if (w.launch < 1) return hipErrorInvalidValue;
-  ItemDesc d;
-  d.P = P;
-  d.Pdst = Pdst;
-  d.gather = gather;
-  d.mats = mats;
-  d.k = k;
-  d.item_start = w.start;
-  d.item_block = w.block;
-  d.item_waves = w.waves;
-  d.item_count = w.count;
-  d.grid_v = grid_v;
-  d.tiles = out.tiles;
-  d.grid_atomic = (float*)out.atomic;
-  d.dirty = out.dirty;
-  d.flags = flags;
-  return g2p2g_desc(d, nmats, w.launch, kinds, s);
 }
 
 hipError_t g2p2g_multi(const ItemDesc* descs, const uint32_t* prefix, int nscenes, int64_t total,
@@ -1013,8 +1027,20 @@ hipError_t zero_dead_blocks(float4* grid, const uint32_t* live, int N, hipStream
   return hipGetLastError();
 }
 
+hipError_t tile_combine(const TileSums& in, int64_t nblk, hipStream_t s) {
+  if (!in.tiles || !in.block_items || !in.crowded || nblk < 1) return hipErrorInvalidValue;
+  ReduceDesc d{};
+  d.tiles = in.tiles;
+  d.block_items = in.block_items;
+  d.crowded = in.crowded;
+  d.nblk = nblk;
+  const unsigned grid = (unsigned)std::min<int64_t>(kCombineGrid, nblk);
+  hipLaunchKernelGGL(k_tile_combine, dim3(grid), dim3(kCombineThreads), 0, s, d);
+  return hipGetLastError();
+}
+
 hipError_t tile_reduce(const TileSums& in, const StepConsts& k, float4* grid, int update, int mlp,
-                       hipStream_t s) {
+                       int combined, hipStream_t s) {
   if (!in.tiles || !in.atomic || !in.dirty || !in.block_items || !in.covered || !grid)
     return hipErrorInvalidValue;
   ReduceDesc d;
@@ -1027,7 +1053,7 @@ hipError_t tile_reduce(const TileSums& in, const StepConsts& k, float4* grid, in
   d.k = k;
   d.nbx = (k.N + 3) / 4;
   d.update = update;
-  d.combined = 0;
+  d.combined = combined;
   d.nblk = 0;
   d.crowded = nullptr;
   d.live = in.live;

@@ -292,15 +292,16 @@ def test_liquid_derived_f_is_bit_identical(gpu_lib):
 
 
 def test_reduce_strategies_are_bit_identical(gpu_lib):
-    """The two ways the grid update reads the further work items of crowded
-    home blocks (one at a time, or 8 blocks' items per batch) sum in the same
-    order: a dense scene (several items per block) gives the same bits
-    with either, and with the choice left to the engine."""
+    """The ways the grid update reads the further work items of crowded home
+    blocks -- one at a time, 8 blocks' items per batch, or after a combine
+    pass that sums each crowded block's tiles into its first -- all sum in
+    item order: a dense scene (several items per block) gives the same bits
+    with each, and with the choices left to the engine."""
     sc = scenes.batched_scenes()[1]  # 41k particles in a N=32 grid, ~800 per occupied block
     outs = []
-    for mlp in (0, 1, -1):
-        eng = make_engine(sc, reduce_mlp=mlp)
-        assert eng.get_option("reduce_mlp") == mlp
+    for combine, mlp in ((0, 0), (0, 1), (1, 0), (-1, -1)):
+        eng = make_engine(sc, reduce_mlp=mlp, combine=combine)
+        assert eng.get_option("reduce_mlp") == mlp and eng.get_option("combine") == combine
         eng.advance(40)
         outs.append(eng.download())
         eng.close()
