@@ -24,6 +24,11 @@ dam break's length. Prints one JSON line:
                the OpenMP thread budget and on one thread, plus the 10,000-
                particle elastic sphere; `accuracy` holds the engine's error
                against the FP64 oracle over the workload's first 10 substeps
+  snow         the snow scene (500,000 particles in two stacked spheres, N=64,
+               dt=1e-4, snow plasticity and hardening): its 1,000 substeps
+               after tuning, particle-substeps/s and fraction of the HBM
+               roofline (its error against the FP64 oracle sits in
+               cpu_baseline's accuracy)
   batched      64 independent scenes advanced one after the other vs. as one
                batch (HIP graph replay): seconds and speedup
   adjoint      the differentiable elastic cube (32,768 particles, N=32, 200
@@ -173,6 +178,7 @@ def cpu_baseline_leg(scene, options: dict, cpu_steps: int):
     c1 = scenes.elastic_sphere()
     c1_all = cpu_rate(c1, 100, threads, 3)
     c1_one = cpu_rate(c1, 20, 1, 3)
+    snow = scenes.snow_spheres()
     sample = (f"{scene.name}: all {scene.n} particles, N={scene.params.N}, median of 5 runs of "
               f"{main['substeps']} FP32 substeps after 1 warm-up, C++/OpenMP oracle (omp atomic "
               f"scatter), {threads} threads ({main['median_s']:.2f} s per run); CPU "
@@ -182,6 +188,43 @@ def cpu_baseline_leg(scene, options: dict, cpu_steps: int):
         "sample": sample, "host": cpu,
         "lines": [main, one, c1_all, c1_one],
         "accuracy": fp64_check(scene, options, 10, threads),
+        "accuracy_snow": fp64_check(snow, {}, 10, threads),
+    }
+
+
+def snow_leg(steps: int, warm: int = 64):
+    """The snow scene (mpm_amd.scenes.snow_spheres: 500,000 particles, N=64):
+    `steps` substeps after the library's tuner and `warm` warm-up substeps,
+    wall time between device synchronisations."""
+    import numpy as np
+    import torch
+    from mpm_amd import Engine, scenes
+
+    sc = scenes.snow_spheres()
+    st = sc.state()
+    eng = Engine(sc.params)
+    eng.set_materials(sc.materials)
+    eng.add_particles(st["x"], st["material"], st["v"], st["C"], st["F"], st["Jp"])
+    eng.upload()
+    eng.autotune(trial_substeps=64)
+    eng.advance(warm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.advance(steps)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    out = eng.download()
+    opts = {k: eng.get_option(k) for k in ("sort_every", "chunk", "radix_bits", "combine")}
+    eng.close()
+    b_alg = scenes.algorithmic_bytes(sc.n, sc.params.N)
+    return {
+        "workload": f"{sc.name}: {sc.n} particles, N={sc.params.N}, dt={sc.params.dt}",
+        "substeps": steps, "warmup": warm, "seconds": t, "us_per_substep": 1e6 * t / steps,
+        "particle_substeps_per_s": sc.n * steps / t,
+        "algorithmic_bytes_per_substep": b_alg,
+        "fraction_of_hbm_roofline": b_alg * steps / t / (HBM_PEAK_GBS * 1e9),
+        "options": opts,
+        "finite": bool(all(np.isfinite(out[f]).all() for f in ("x", "v", "F"))),
     }
 
 
@@ -330,6 +373,8 @@ def main() -> int:
                     help="substeps of the CPU baseline's main line (5 timed runs of a fifth each)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-snow", action="store_true")
+    ap.add_argument("--snow-steps", type=int, default=1000)
     ap.add_argument("--batched-steps", type=int, default=500)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -442,9 +487,13 @@ def main() -> int:
             "traffic": load_traffic(scene.name, kernel_name),
         },
         "cpu_baseline": None,
+        "snow": None,
         "batched": None,
         "adjoint": None,
     }
+    if not args.no_snow:
+        result["snow"] = snow_leg(args.snow_steps)
+        print(f"snow: {result['snow']}", file=sys.stderr, flush=True)
     if not args.no_batched:
         result["batched"] = batched_leg(48, args.batched_steps)
         print(f"batched: {result['batched']}", file=sys.stderr, flush=True)

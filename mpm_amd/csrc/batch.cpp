@@ -16,17 +16,19 @@
 //
 // Fused cycles (option "fuse", default on): when every scene is in the steady
 // state of the tiled fused mode, a cycle is recorded as one launch per phase
-// for all scenes together instead of per-scene kernel chains: per substep one
-// tile reduction over the node blocks of every scene, the due cell sorts
-// (in one launch per sort kernel), and one fused G2P/P2G pass over the
-// work items of every scene (two when liquid-only and other scenes mix). A
-// workgroup finds its scene in a table of per-scene arguments (the same
+// for many scenes together instead of per-scene kernel chains: per substep a
+// tile combine and reduction over the node blocks of the scenes, the due cell
+// sorts (in one launch per sort kernel), and a fused G2P/P2G pass over their
+// work items (one per kernel: liquid-only scenes and the others apart). The
+// scenes are split into a few chains (option "chains", default 2), balanced
+// by estimated cost, whose launches go to streams of their own and overlap.
+// A workgroup finds its scene in a table of per-scene arguments (the same
 // arguments the scene's own launch would take, so the results are the same
 // bits). The tables are written into pinned host memory while the cycle is
-// recorded and copied to the device by one copy at the head of the graph.
-// An eager fused substep writes its tables into one of a few pinned slots and
-// copies each table on the batch's stream ahead of the launches that read it;
-// a slot is written again once the copies queued from it have run.
+// recorded and copied to the device once, before the graph's first launch;
+// every replay reads them there. An eager fused substep writes its tables
+// into one of a few pinned slots and copies them on the batch's stream ahead
+// of its launches; a slot is written again once the copies from it have run.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -58,6 +60,7 @@ struct mpm_batch_s {
   int64_t use_graph = 1;
   int64_t fuse = 1;
   int64_t multi_sort = 1;  // fused cycles sort all due scenes in one launch per kernel
+  int64_t chains = 2;      // chains of launches per substep, each on a stream of its own
   int64_t launches = 0;        // graph launches so far (for tests and reports)
   int64_t fused_captures = 0;  // cycles recorded as multi-scene launches
   // argument tables of a fused cycle: pinned host copy and device copy
@@ -206,11 +209,12 @@ bool fusable(const mpm_batch_s* b) {
   return true;
 }
 
-// table bytes of one substep at most: one reduce table, two pass tables and
-// a sort table, each launch with its prefix (the sort: about 20 launches)
+// table bytes of one substep at most: per chain (four at most) a reduce
+// table, up to two pass tables and a sort table, each launch with its prefix
+// (the sort: about 20 launches), every table 256-byte aligned
 size_t tables_per_substep(size_t ns) {
   return ns * (sizeof(mpmk::ReduceDesc) + 2 * sizeof(mpmk::ItemDesc) + sizeof(mpmk::SortDesc)) +
-         24 * (4 * (ns + 1) + 256) + 4 * 256;
+         112 * (4 * (ns + 1) + 256) + 4 * 256;
 }
 
 int ensure_eager_tables(mpm_batch_s* b) {
@@ -249,20 +253,176 @@ int ensure_tables(mpm_batch_s* b, int32_t n) {
   return MPM_OK;
 }
 
+// The launches of one substep for a set of scenes, planned before any of
+// them is issued: their argument tables are written (run eagerly: copied)
+// first, so the launches may go to another stream after a fork.
+struct Chain {
+  std::vector<size_t> scenes;
+  const void* rdt = nullptr;  // reduction descriptors and prefix
+  const void* rpt = nullptr;
+  uint32_t rtotal = 0;
+  const void* sdt = nullptr;  // sort descriptors of the scenes whose sort is due
+  int nsort = 0;
+  struct SortLaunch { int phase, pass; const void* prefix; uint32_t total; };
+  std::vector<SortLaunch> sorts;
+  const void* dt[2] = {nullptr, nullptr};  // fused passes: the others, liquid-only scenes
+  const void* pt[2] = {nullptr, nullptr};
+  int nd[2] = {0, 0};
+  uint32_t dtotal[2] = {0, 0};
+};
+
+// Plans one substep of the chain's scenes: (1) the tile reductions with the
+// grid update, after the tiles of home blocks with several items were summed
+// into one (small dense scenes hold dozens of items per block); (2) the cell
+// sorts that are due, one launch per sort kernel for all of them (or, without
+// multi_sort, each scene's own sort, issued here on the forked streams: a sort
+// reads the particles only, so it needs no order against the reductions);
+// (3) the fused passes, liquid-only scenes and the others apart.
+int plan_chain(mpm_batch_s* b, TableWriter& tw, Chain& c, std::vector<mpmk::ItemDesc>& done) {
+  std::vector<mpmk::ReduceDesc> rd;
+  std::vector<uint32_t> rp(1, 0u);
+  for (size_t j : c.scenes) {
+    rd.push_back(b->scenes[j]->reduce_desc());
+    rd.back().combined = 1;
+    rp.push_back(rp.back() + (uint32_t)b->scenes[j]->node_blocks());
+  }
+  c.rdt = tw.put(rd.data(), sizeof(mpmk::ReduceDesc) * rd.size());
+  c.rpt = tw.put(rp.data(), sizeof(uint32_t) * rp.size());
+  if (!c.rdt || !c.rpt) return tw.failed();
+  c.rtotal = rp.back();
+  bool any = false;
+  for (size_t j : c.scenes) any = any || b->scenes[j]->sort_due();
+  if (any && b->multi_sort) {
+    std::vector<mpmk::SortDesc> sd;
+    std::vector<Sim*> due;
+    for (size_t j : c.scenes) {
+      Sim* s = b->scenes[j];
+      if (!s->sort_due()) continue;
+      mpmk::SortDesc d;
+      const int rc = s->sort_desc(d);
+      if (rc) return b->fail(rc, "scene %zu: %s", j, s->last_error().c_str());
+      sd.push_back(d);
+      due.push_back(s);
+    }
+    c.sdt = tw.put(sd.data(), sizeof(mpmk::SortDesc) * sd.size());
+    if (!c.sdt) return tw.failed();
+    c.nsort = (int)sd.size();
+    int maxp = 0;
+    for (const auto& d : sd) maxp = std::max(maxp, d.npasses);
+    std::vector<std::pair<int, int>> seq = {{mpmk::SORT_FILL, 0}, {mpmk::SORT_KEYS, 0}};
+    for (int p = 0; p < maxp; ++p)
+      for (int ph : {(int)mpmk::SORT_HIST, (int)mpmk::SORT_RSCAN, (int)mpmk::SORT_SCATTER}) seq.push_back({ph, p});
+    for (int ph = mpmk::SORT_HEADS; ph < mpmk::SORT_NUM_PHASES; ++ph) seq.push_back({ph, 0});
+    std::vector<uint32_t> pre(sd.size() + 1);
+    for (const auto& pp : seq) {
+      pre[0] = 0;
+      for (size_t j = 0; j < sd.size(); ++j)
+        pre[j + 1] = pre[j] + (uint32_t)mpmk::sort_blocks(sd[j], pp.first, pp.second);
+      if (pre.back() == 0) continue;
+      const void* pt = tw.put(pre.data(), sizeof(uint32_t) * pre.size());
+      if (!pt) return tw.failed();
+      c.sorts.push_back({pp.first, pp.second, pt, pre.back()});
+    }
+    for (Sim* s : due) s->sort_done();
+  } else if (any) {
+    const size_t ns = std::min(b->streams.size(), c.scenes.size());
+    BCHK(hipEventRecord(b->fork, b->origin), "hipEventRecord");
+    for (size_t k = 0; k < ns; ++k) BCHK(hipStreamWaitEvent(b->streams[k], b->fork, 0), "hipStreamWaitEvent");
+    for (size_t q = 0; q < c.scenes.size(); ++q) {
+      Sim* s = b->scenes[c.scenes[q]];
+      if (!s->sort_due()) continue;
+      const int rc = s->sort_now(b->streams[q % ns]);
+      if (rc) return b->fail(rc, "scene %zu: %s", c.scenes[q], s->last_error().c_str());
+    }
+    for (size_t k = 0; k < ns; ++k) {
+      BCHK(hipEventRecord(b->joins[k], b->streams[k]), "hipEventRecord");
+      BCHK(hipStreamWaitEvent(b->origin, b->joins[k], 0), "hipStreamWaitEvent");
+    }
+  }
+  std::vector<mpmk::ItemDesc> gd[2];
+  std::vector<uint32_t> gp[2] = {std::vector<uint32_t>(1, 0u), std::vector<uint32_t>(1, 0u)};
+  for (size_t j : c.scenes) {
+    Sim* s = b->scenes[j];
+    int kinds = 0;
+    int64_t launch = 0;
+    const int rc = s->fused_pass(s->velocity_grid(), 1, done[j], kinds, launch, b->origin);
+    if (rc) return b->fail(rc, "scene %zu: %s", j, s->last_error().c_str());
+    const int g = kinds ? 1 : 0;
+    gd[g].push_back(done[j]);
+    gp[g].push_back(gp[g].back() + (uint32_t)launch);
+  }
+  for (int g = 0; g < 2; ++g) {
+    if (gd[g].empty()) continue;
+    c.dt[g] = tw.put(gd[g].data(), sizeof(mpmk::ItemDesc) * gd[g].size());
+    c.pt[g] = tw.put(gp[g].data(), sizeof(uint32_t) * gp[g].size());
+    if (!c.dt[g] || !c.pt[g]) return tw.failed();
+    c.nd[g] = (int)gd[g].size();
+    c.dtotal[g] = gp[g].back();
+  }
+  return MPM_OK;
+}
+
+int launch_chain(mpm_batch_s* b, const Chain& c, hipStream_t st) {
+  const int n = (int)c.scenes.size();
+  BCHK(mpmk::tile_combine_multi((const mpmk::ReduceDesc*)c.rdt, n, st), "multi-scene tile combine");
+  BCHK(mpmk::tile_reduce_multi((const mpmk::ReduceDesc*)c.rdt, (const uint32_t*)c.rpt, n, c.rtotal, st),
+       "multi-scene grid update");
+  for (const auto& l : c.sorts)
+    BCHK(mpmk::sort_multi(l.phase, l.pass, (const mpmk::SortDesc*)c.sdt, (const uint32_t*)l.prefix, c.nsort,
+                          l.total, st),
+         "multi-scene sort");
+  for (int g = 0; g < 2; ++g) {
+    if (!c.nd[g]) continue;
+    BCHK(mpmk::g2p2g_multi((const mpmk::ItemDesc*)c.dt[g], (const uint32_t*)c.pt[g], c.nd[g], c.dtotal[g], g, st),
+         "multi-scene G2P/P2G");
+  }
+  return MPM_OK;
+}
+
+// Scenes split into `chains` chains (at most 4: the origin and 3 batch
+// streams), balanced by estimated cost: the scenes largest first, each onto
+// the chain of least cost so far, a particle of an SVD material costing 1.7
+// of a liquid-only scene's (the fused passes' measured rates on the batched
+// workload). Without multi_sort: one chain.
+std::vector<std::vector<size_t>> chain_scenes(const mpm_batch_s* b) {
+  const size_t nsc = b->scenes.size();
+  const int64_t k = std::min<int64_t>({b->chains, (int64_t)b->streams.size() + 1, (int64_t)nsc, 4});
+  std::vector<std::vector<size_t>> out;
+  if (k <= 1 || !b->multi_sort) {
+    out.emplace_back();
+    for (size_t j = 0; j < nsc; ++j) out.back().push_back(j);
+    return out;
+  }
+  std::vector<double> cost(nsc);
+  for (size_t j = 0; j < nsc; ++j)
+    cost[j] = (double)b->scenes[j]->count() * (b->scenes[j]->liquid_only() ? 1.0 : 1.7);
+  std::vector<size_t> order(nsc);
+  for (size_t j = 0; j < nsc; ++j) order[j] = j;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t c) { return cost[a] > cost[c]; });
+  out.resize((size_t)k);
+  std::vector<double> load((size_t)k, 0.0);
+  for (size_t j : order) {
+    const size_t q = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+    out[q].push_back(j);
+    load[q] += cost[j];
+  }
+  for (auto& c : out) std::sort(c.begin(), c.end());
+  return out;
+}
+
 // `n` substeps of every scene as multi-scene launches on the origin stream:
 // recorded (the stream is being captured) or, with `eager`, run at once.
+// With several chains (chain_scenes) chain c > 0 runs on batch stream c - 1,
+// forked from the origin and joined back every substep: the chains touch
+// disjoint scenes, and each fills the others' gaps and tails (a launch's
+// last, partly occupied round of workgroups).
 int run_scenes_fused(mpm_batch_s* b, int32_t n, bool eager) {
   const size_t nsc = b->scenes.size();
-  const size_t ns = std::min(b->streams.size(), nsc);
+  // recorded: the tables are copied to the device once, after the capture
+  // (capture_and_launch), and stay there for every replay of the cycle
   TableWriter tw{b, b->h_tables, b->d_tables, b->tables_cap};
-  // recorded: the tables reach the device before any kernel of the cycle runs
-  if (!eager)
-    BCHK(hipMemcpyAsync(b->d_tables, b->h_tables, b->tables_cap, hipMemcpyHostToDevice, b->origin),
-         "hipMemcpyAsync(batch tables)");
-  std::vector<mpmk::ReduceDesc> rd(nsc);
-  std::vector<uint32_t> rp(nsc + 1);
-  std::vector<mpmk::ItemDesc> gd[2];
-  std::vector<uint32_t> gp[2];
+  const std::vector<std::vector<size_t>> groups = chain_scenes(b);
+  const int nchains = (int)groups.size();
   std::vector<mpmk::ItemDesc> done(nsc);
   for (int32_t step = 0; step < n; ++step) {
     int slot = -1;
@@ -273,96 +433,24 @@ int run_scenes_fused(mpm_batch_s* b, int32_t n, bool eager) {
       b->eager_pending[slot] = false;
       tw = TableWriter{b, b->h_eager[slot], b->d_eager, b->eager_cap, b->origin};
     }
-    // 1. the tile reductions with the grid update, after the tiles of home
-    //    blocks with several items were summed into one (small dense scenes
-    //    hold dozens of items per block)
-    rp[0] = 0;
-    for (size_t j = 0; j < nsc; ++j) {
-      rd[j] = b->scenes[j]->reduce_desc();
-      rd[j].combined = 1;
-      rp[j + 1] = rp[j] + (uint32_t)b->scenes[j]->node_blocks();
+    std::vector<Chain> chains((size_t)nchains);
+    for (int c = 0; c < nchains; ++c) {
+      chains[c].scenes = groups[c];
+      const int rc = plan_chain(b, tw, chains[c], done);
+      if (rc) return rc;
     }
-    const void* rdt = tw.put(rd.data(), sizeof(mpmk::ReduceDesc) * nsc);
-    const void* rpt = tw.put(rp.data(), sizeof(uint32_t) * (nsc + 1));
-    if (!rdt || !rpt) return tw.failed();
-    BCHK(mpmk::tile_combine_multi((const mpmk::ReduceDesc*)rdt, (int)nsc, b->origin), "multi-scene tile combine");
-    BCHK(mpmk::tile_reduce_multi((const mpmk::ReduceDesc*)rdt, (const uint32_t*)rpt, (int)nsc, rp[nsc],
-                                 b->origin),
-         "multi-scene grid update");
-    // 2. the cell sorts that are due: one launch per sort kernel for all
-    //    of them (or per scene on the forked streams)
-    bool any = false;
-    for (Sim* s : b->scenes) any = any || s->sort_due();
-    if (any && b->multi_sort) {
-      std::vector<mpmk::SortDesc> sd;
-      std::vector<Sim*> due;
-      for (size_t j = 0; j < nsc; ++j) {
-        Sim* s = b->scenes[j];
-        if (!s->sort_due()) continue;
-        mpmk::SortDesc d;
-        const int rc = s->sort_desc(d);
-        if (rc) return b->fail(rc, "scene %zu: %s", j, s->last_error().c_str());
-        sd.push_back(d);
-        due.push_back(s);
-      }
-      const void* sdt = tw.put(sd.data(), sizeof(mpmk::SortDesc) * sd.size());
-      if (!sdt) return tw.failed();
-      int maxp = 0;
-      for (const auto& d : sd) maxp = std::max(maxp, d.npasses);
-      std::vector<std::pair<int, int>> seq = {{mpmk::SORT_FILL, 0}, {mpmk::SORT_KEYS, 0}};
-      for (int p = 0; p < maxp; ++p)
-        for (int ph : {(int)mpmk::SORT_HIST, (int)mpmk::SORT_RSCAN, (int)mpmk::SORT_SCATTER}) seq.push_back({ph, p});
-      for (int ph = mpmk::SORT_HEADS; ph < mpmk::SORT_NUM_PHASES; ++ph) seq.push_back({ph, 0});
-      std::vector<uint32_t> pre(sd.size() + 1);
-      for (const auto& pp : seq) {
-        pre[0] = 0;
-        for (size_t j = 0; j < sd.size(); ++j)
-          pre[j + 1] = pre[j] + (uint32_t)mpmk::sort_blocks(sd[j], pp.first, pp.second);
-        if (pre.back() == 0) continue;
-        const void* pt = tw.put(pre.data(), sizeof(uint32_t) * pre.size());
-        if (!pt) return tw.failed();
-        BCHK(mpmk::sort_multi(pp.first, pp.second, (const mpmk::SortDesc*)sdt, (const uint32_t*)pt,
-                              (int)sd.size(), pre.back(), b->origin),
-             "multi-scene sort");
-      }
-      for (Sim* s : due) s->sort_done();
-    } else if (any) {
+    if (nchains > 1) {
       BCHK(hipEventRecord(b->fork, b->origin), "hipEventRecord");
-      for (size_t k = 0; k < ns; ++k) BCHK(hipStreamWaitEvent(b->streams[k], b->fork, 0), "hipStreamWaitEvent");
-      for (size_t j = 0; j < nsc; ++j) {
-        Sim* s = b->scenes[j];
-        if (!s->sort_due()) continue;
-        const int rc = s->sort_now(b->streams[j % ns]);
-        if (rc) return b->fail(rc, "scene %zu: %s", j, s->last_error().c_str());
-      }
-      for (size_t k = 0; k < ns; ++k) {
-        BCHK(hipEventRecord(b->joins[k], b->streams[k]), "hipEventRecord");
-        BCHK(hipStreamWaitEvent(b->origin, b->joins[k], 0), "hipStreamWaitEvent");
-      }
+      for (int c = 1; c < nchains; ++c)
+        BCHK(hipStreamWaitEvent(b->streams[c - 1], b->fork, 0), "hipStreamWaitEvent");
     }
-    // 3. the fused passes, liquid-only scenes and the others apart
-    for (int g = 0; g < 2; ++g) {
-      gd[g].clear();
-      gp[g].assign(1, 0u);
+    for (int c = 0; c < nchains; ++c) {
+      const int rc = launch_chain(b, chains[c], c == 0 ? b->origin : b->streams[c - 1]);
+      if (rc) return rc;
     }
-    for (size_t j = 0; j < nsc; ++j) {
-      Sim* s = b->scenes[j];
-      int kinds = 0;
-      int64_t launch = 0;
-      const int rc = s->fused_pass(s->velocity_grid(), 1, done[j], kinds, launch, b->origin);
-      if (rc) return b->fail(rc, "scene %zu: %s", j, s->last_error().c_str());
-      const int g = kinds ? 1 : 0;
-      gd[g].push_back(done[j]);
-      gp[g].push_back(gp[g].back() + (uint32_t)launch);
-    }
-    for (int g = 0; g < 2; ++g) {
-      if (gd[g].empty()) continue;
-      const void* dt = tw.put(gd[g].data(), sizeof(mpmk::ItemDesc) * gd[g].size());
-      const void* pt = tw.put(gp[g].data(), sizeof(uint32_t) * gp[g].size());
-      if (!dt || !pt) return tw.failed();
-      BCHK(mpmk::g2p2g_multi((const mpmk::ItemDesc*)dt, (const uint32_t*)pt, (int)gd[g].size(),
-                             gp[g].back(), g, b->origin),
-           "multi-scene G2P/P2G");
+    for (int c = 1; c < nchains; ++c) {
+      BCHK(hipEventRecord(b->joins[c - 1], b->streams[c - 1]), "hipEventRecord");
+      BCHK(hipStreamWaitEvent(b->origin, b->joins[c - 1], 0), "hipStreamWaitEvent");
     }
     for (size_t j = 0; j < nsc; ++j) {
       b->scenes[j]->fused_pass_done(done[j]);
@@ -442,6 +530,11 @@ int capture_and_launch(mpm_batch_s* b, int32_t cycle) {
   b->graph = g;
   BCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
   b->signature = sig;  // the capture ran the scenes' host logic for one cycle
+  // the cycle's argument tables, written while it was recorded: the same for
+  // every replay, so one copy (ordered before the launches on the origin stream)
+  if (fused)
+    BCHK(hipMemcpyAsync(b->d_tables, b->h_tables, b->tables_cap, hipMemcpyHostToDevice, b->origin),
+         "hipMemcpyAsync(batch tables)");
   BCHK(hipGraphLaunch(b->exec, b->origin), "hipGraphLaunch");
   ++b->launches;
   return MPM_OK;
@@ -508,6 +601,12 @@ extern "C" int mpm_batch_set_option(mpm_batch b, const char* name, int64_t v) {
     b->drop_graph();
     return MPM_OK;
   }
+  if (n == "chains") {
+    if (v < 1 || v > 4) return b->fail(MPM_ERR_INVALID, "chains must be 1..4");
+    b->chains = v;
+    b->drop_graph();
+    return MPM_OK;
+  }
   return b->fail(MPM_ERR_INVALID, "mpm_batch_set_option: unknown option '%s'", name);
 }
 
@@ -520,6 +619,7 @@ extern "C" int mpm_batch_get_option(mpm_batch b, const char* name, int64_t* v) {
   else if (n == "graph_launches") *v = b->launches;
   else if (n == "fuse") *v = b->fuse;
   else if (n == "multi_sort") *v = b->multi_sort;
+  else if (n == "chains") *v = b->chains;
   else if (n == "fused_captures") *v = b->fused_captures;
   else if (n == "fused_eager_substeps") *v = b->eager_fused;
   else return b->fail(MPM_ERR_INVALID, "mpm_batch_get_option: unknown option '%s'", name);
@@ -615,8 +715,8 @@ extern "C" int mpm_batch_autotune(mpm_batch b, int32_t trial_substeps, char* rep
   }
   struct Cand { int64_t sort_every, chunk; bool own; double us; };
   std::vector<Cand> cands = {{0, 0, true, 0.0}};
-  for (int64_t ch : {(int64_t)mpmk::kMaxItemParticles, (int64_t)128})
-    for (int64_t se : {16, 24, 32, 48, 64}) cands.push_back({se, ch, false, 0.0});
+  for (int64_t se : {16, 24, 32, 48, 64, 96, 128}) cands.push_back({se, mpmk::kMaxItemParticles, false, 0.0});
+  for (int64_t se : {32, 64}) cands.push_back({se, 128, false, 0.0});
   hipEvent_t e0 = nullptr, e1 = nullptr;
   BCHK(hipEventCreate(&e0), "hipEventCreate");
   BCHK(hipEventCreate(&e1), "hipEventCreate");

@@ -158,6 +158,12 @@ class Sim {
     t_ += (double)prm_.dt;
   }
   const float4* velocity_grid() const { return d_grid_[0]; }
+  // every material is WC_LIQUID (the fused pass's liquid-only kernel)
+  bool liquid_only() const {
+    for (const auto& m : mats_)
+      if (m.kind != MPM_MATERIAL_WC_LIQUID) return false;
+    return !mats_.empty();
+  }
   int material_count() const { return (int)mats_.size(); }
 
   // device views for other entry points of the library
