@@ -362,15 +362,23 @@ int mpm_batch_advance(mpm_batch b, int32_t substeps, void* stream);
 /* "graph" (1: record and replay, 0: launch kernel by kernel), "streams"
  * (1..16, default 4), "fuse" (1: when every scene is in the steady state of
  * the tiled fused mode, record each phase of all scenes as one multi-scene
- * launch; 0: per-scene kernel chains); read-only "graph_launches" and
- * "fused_captures". */
+ * launch; 0: per-scene kernel chains), "multi_sort" (1: the due sorts of all
+ * scenes in one launch per sort kernel; 0: each scene's own), "chains"
+ * (1..4, default 2: the scenes split into that many chains of launches,
+ * balanced by particles weighted by material kind, each on a stream of its
+ * own), "item_share" (1..100, default 100: percent of each scene's estimated
+ * work items given a workgroup of the fused pass, whose workgroups take
+ * further items of their scene in turn; lower only to exercise that path);
+ * read-only "graph_launches", "fused_captures" and "fused_eager_substeps".
+ * Results do not depend on these settings. */
 int mpm_batch_set_option(mpm_batch b, const char* name, int64_t value);
 int mpm_batch_get_option(mpm_batch b, const char* name, int64_t* value);
 /* Batch tuner: times candidate settings through the batch itself (its graph
  * capture and multi-scene launches included), each from the same snapshot of
  * every scene, and keeps the fastest: every scene's own mpm_autotune choice
  * (sort interval and work-item size; the batch sorts with 8-bit digits), or
- * one sort interval (16..64) and work-item size (192, 128) for all scenes.
+ * one sort interval (16..128) and work-item size (192; 128 at intervals 32
+ * and 64) for all scenes.
  * Scenes with different sort intervals share a cycle of their least common
  * multiple. The scenes' states and times are put back. The report (JSON,
  * truncated to report_len) lists the trials in microseconds per batch

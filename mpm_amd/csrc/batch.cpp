@@ -61,6 +61,7 @@ struct mpm_batch_s {
   int64_t fuse = 1;
   int64_t multi_sort = 1;  // fused cycles sort all due scenes in one launch per kernel
   int64_t chains = 2;      // chains of launches per substep, each on a stream of its own
+  int64_t item_share = 100;  // percent of each scene's estimated items given a workgroup of the pass
   int64_t launches = 0;        // graph launches so far (for tests and reports)
   int64_t fused_captures = 0;  // cycles recorded as multi-scene launches
   // argument tables of a fused cycle: pinned host copy and device copy
@@ -349,7 +350,13 @@ int plan_chain(mpm_batch_s* b, TableWriter& tw, Chain& c, std::vector<mpmk::Item
     if (rc) return b->fail(rc, "scene %zu: %s", j, s->last_error().c_str());
     const int g = kinds ? 1 : 0;
     gd[g].push_back(done[j]);
-    gp[g].push_back(gp[g].back() + (uint32_t)launch);
+    // an estimate of the scene's items, not the bound `launch` (an upload's
+    // worth of occupied home blocks per scene leaves most of the bound's
+    // workgroups empty, and an empty one holds its LDS through three
+    // dependent loads): the pass's workgroups take further items in turn
+    (void)launch;
+    const int64_t wg = std::max<int64_t>(1, s->batch_workgroups() * b->item_share / 100);
+    gp[g].push_back(gp[g].back() + (uint32_t)wg);
   }
   for (int g = 0; g < 2; ++g) {
     if (gd[g].empty()) continue;
@@ -601,6 +608,12 @@ extern "C" int mpm_batch_set_option(mpm_batch b, const char* name, int64_t v) {
     b->drop_graph();
     return MPM_OK;
   }
+  if (n == "item_share") {
+    if (v < 1 || v > 100) return b->fail(MPM_ERR_INVALID, "item_share must be 1..100");
+    b->item_share = v;
+    b->drop_graph();
+    return MPM_OK;
+  }
   if (n == "chains") {
     if (v < 1 || v > 4) return b->fail(MPM_ERR_INVALID, "chains must be 1..4");
     b->chains = v;
@@ -620,6 +633,7 @@ extern "C" int mpm_batch_get_option(mpm_batch b, const char* name, int64_t* v) {
   else if (n == "fuse") *v = b->fuse;
   else if (n == "multi_sort") *v = b->multi_sort;
   else if (n == "chains") *v = b->chains;
+  else if (n == "item_share") *v = b->item_share;
   else if (n == "fused_captures") *v = b->fused_captures;
   else if (n == "fused_eager_substeps") *v = b->eager_fused;
   else return b->fail(MPM_ERR_INVALID, "mpm_batch_get_option: unknown option '%s'", name);

@@ -412,7 +412,7 @@ int Sim::upload(hipStream_t s) {
     for (int a = 0; a < 9; ++a) f_canonical_ = f_canonical_ && hF_[9 * i + a] == ((a % 4 == 0) ? 1.0f : 0.0f);
   }
   f_stale_ = false;
-  dense_ = dense_blocks();
+  dense_ = dense_blocks(&occupied_);
   max_mat_ = 0;
   for (int64_t i = 0; i < n; ++i) max_mat_ = std::max<uint8_t>(max_mat_, hmat_[i]);
   cur_ = 0;
@@ -843,8 +843,9 @@ int Sim::sort_desc(mpmk::SortDesc& d) {
 // occupied blocks needs a further work item (more particles than one item
 // takes). The grid update then reads several tiles per node and block,
 // which the mlp variant of the tile reduction batches.
-bool Sim::dense_blocks() const {
+bool Sim::dense_blocks(int64_t* occupied_out) const {
   const int64_t n = (int64_t)hJp_.size();
+  *occupied_out = 0;
   if (n == 0) return false;
   const int N = (int)k_.N, nb = (N + 3) / 4;
   const int64_t per = opt_.chunk > 0 ? opt_.chunk : mpmk::kMaxItemParticles;
@@ -866,7 +867,18 @@ bool Sim::dense_blocks() const {
     extra += ((int64_t)(j - i) + per - 1) / per - 1;
     i = j;
   }
+  *occupied_out = occupied;
   return extra * 20 > occupied;
+}
+
+// Workgroups of this scene's share of a batch's fused launch: its items (at
+// most one per `chunk` particles plus one per occupied home block) with the
+// upload's occupancy and room for it to grow by half, plus 64; a workgroup
+// takes further items of the scene in turn when there are more. At most
+// max_items(), the bound that needs no loop.
+int64_t Sim::batch_workgroups() const {
+  const int64_t est = (P_ + chunk_ - 1) / chunk_ + occupied_ + occupied_ / 2 + 64;
+  return std::max<int64_t>(1, std::min(max_items(), est));
 }
 
 bool Sim::use_mlp() const { return opt_.reduce_mlp < 0 ? dense_ : opt_.reduce_mlp != 0; }

@@ -150,6 +150,7 @@ class Sim {
   int fused_pass(const float4* grid_v, int do_g2p, mpmk::ItemDesc& d, int& kinds, int64_t& launch,
                  hipStream_t s);
   void fused_pass_done(const mpmk::ItemDesc& d);
+  int64_t batch_workgroups() const;
   // the end of a substep: velocity grid, sort counter, time
   void substep_done() {
     vel_grid_ = d_grid_[0];
@@ -192,9 +193,10 @@ class Sim {
   bool f_canonical_ = false;
   bool f_stale_ = false;
   bool derives_f() const;
-  bool dense_blocks() const;  // from the staged particles
+  bool dense_blocks(int64_t* occupied) const;  // from the staged particles
   bool use_mlp() const;
   bool dense_ = false;
+  int64_t occupied_ = 0;  // home blocks holding particles at the upload
   size_t staged_on_device_ = 0;  // staged particles the last upload / append copied
   // mesh buffers: distance field, per-cube counts and offsets, scan scratch,
   // triangles; the last build's grid side and triangle count

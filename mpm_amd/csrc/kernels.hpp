@@ -148,8 +148,9 @@ enum TiledFlags : int { TILED_G2P = 1, TILED_DERIVE_F = 2, TILED_GUARD = 4, TILE
 // launches of the XCD-ordered kernels round their grid up to a multiple of this
 constexpr int kXcds = 8;
 // The same passes over several scenes in one launch: descs[j] is scene j,
-// prefix[j] its first workgroup (prefix[n] = total; workgroup counts are the
-// scenes' max_items for g2p2g, their node-block counts for the reduction).
+// prefix[j] its first workgroup (prefix[n] = total; workgroup counts are any
+// positive number for g2p2g, whose workgroup takes every count-th item of its
+// scene from its index on, and the node-block counts for the reduction).
 // kinds: KINDS of every scene of the launch.
 hipError_t g2p2g_desc(const ItemDesc& d, int nmats, int64_t launch, int kinds, hipStream_t s);
 hipError_t g2p2g_multi(const ItemDesc* descs, const uint32_t* prefix, int nscenes, int64_t total,

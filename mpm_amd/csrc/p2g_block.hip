@@ -695,6 +695,18 @@ __device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t g, uint32_t n
   return j < per ? (b % X) * per + j : n;
 }
 
+// Items first, first + B, ... of a scene below its item count, in turn (a
+// batch scene that outgrew its workgroups). Out of line: the loop would hold
+// its state in registers through the whole item body, and the first item,
+// the common case, keeps the registers of a single pass.
+template <int KS>
+__device__ __noinline__ void g2p2g_further_items(const mpmk::ItemDesc& d, uint32_t first, uint32_t B) {
+  for (uint32_t item = first; item < *d.item_count; item += B) {
+    __syncthreads();  // the previous item's flush read the tiles this one stages
+    g2p2g_item<KS>(d, (int)item);
+  }
+}
+
 template <int KS>
 __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
     __attribute__((amdgpu_waves_per_eu(KS == KINDS_LIQUID ? 5 : 4)))
@@ -708,9 +720,16 @@ __global__ void __launch_bounds__(64 * mpmk::kItemWaves)
     __attribute__((amdgpu_waves_per_eu(KS == KINDS_LIQUID ? 5 : 4)))
     k_g2p2g_multi(const mpmk::ItemDesc* __restrict__ descs, const uint32_t* __restrict__ prefix,
                   int nscenes) {
+  // workgroup g takes item l = g - prefix[j] of its scene j; the scene's B
+  // workgroups are sized from an estimate of its items, and when the scene
+  // has more, every workgroup takes every B-th further one in turn
   const uint32_t g = blockIdx.x;
   const int j = scene_of_wave(prefix, nscenes, g);
-  g2p2g_item<KS>(descs[j], (int)(g - prefix[j]));
+  const mpmk::ItemDesc& d = descs[j];
+  const uint32_t l = g - prefix[j];
+  g2p2g_item<KS>(d, (int)l);
+  const uint32_t B = prefix[j + 1] - prefix[j];
+  if (l + B < *d.item_count) g2p2g_further_items<KS>(d, l + B, B);  // uniform for the workgroup
 }
 
 // One wavefront per 4^3 node block: each lane sums its node over the tiles of

@@ -151,6 +151,32 @@ def test_batch_of_scenes_with_other_radix_digits(gpu_lib):
         e.close()
 
 
+@pytest.mark.parametrize("share", [1, 20])
+def test_batch_workgroups_taking_further_items(gpu_lib, share):
+    """Each scene's share of the fused pass is sized from an estimate of its
+    work items, and when a scene has more items than workgroups, every
+    workgroup takes further items of it in turn. Shrinking the shares (option
+    item_share: 1% leaves one workgroup per scene, 20% a few) sends the
+    multi-scene passes through that path: the batch still equals every scene's
+    own run bit for bit."""
+    scs = small_batch()
+    k = 100
+    ref = run_alone(scs, k)
+    engines = [make(sc) for sc in scs]
+    b = Batch(engines)
+    b.set_option("item_share", share)
+    assert b.get_option("item_share") == share
+    b.advance(k)
+    outs = [e.download() for e in engines]
+    assert b.get_option("fused_captures") == 1
+    b.close()
+    for o, r in zip(outs, ref):
+        for f in ("x", "v", "C", "F", "Jp"):
+            assert np.array_equal(o[f], r[f]), f
+    for e in engines:
+        e.close()
+
+
 def test_full_batch_equals_individual_runs(gpu_lib):
     """The batched workload at full size (64 scenes, 3.35 M particles, N in
     {32, 48, 64}) for 48 substeps -- 16 per scene, then one recorded cycle of
