@@ -348,7 +348,7 @@ __global__ void __launch_bounds__(256)
   if (c.w > 0.0f) {
     const int zi = (int)(t % N), yi = (int)((t / N) % N), xi = (int)(t / ((size_t)N * N));
     o1.x = c.x / c.w;
-    o1.y = c.y / c.w + k.dt * k.gravity;
+    o1.y = __builtin_fmaf(k.dt, k.gravity, c.y / c.w);  // (as grid_node_update rounds it)
     o1.z = c.z / c.w;
     const float4 u = grid_node_update(c, xi, yi, zi, k);
     bool sticky, floor_;

@@ -93,7 +93,11 @@ __device__ __forceinline__ float4 grid_node_update(float4 c, int xi, int yi, int
   c.y = c.y / c.w;
   c.z = c.z / c.w;
   c.w = 1.0f;
-  c.y += k.dt * k.gravity;
+  // cell[1] += dt * Gravity (mpm.cu:90), which nvcc contracts into one fused
+  // multiply-add: written as one, so that every kernel that inlines this
+  // update rounds it the same way (a compiler left to choose may hoist the
+  // product of the two constants and round it first)
+  c.y = __builtin_fmaf(k.dt, k.gravity, c.y);
   bool sticky, floor_;
   wall_flags(xi, yi, zi, k, sticky, floor_);
   if (sticky) { c.x = 0.f; c.y = 0.f; c.z = 0.f; }
