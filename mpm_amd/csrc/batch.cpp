@@ -27,7 +27,7 @@
 // bits). The tables are written into pinned host memory while the cycle is
 // recorded and copied to the device once, before the graph's first launch;
 // every replay reads them there. An eager fused substep writes its tables
-// into one of a few pinned slots and copies them on the batch's stream ahead
+// into one of a few pinned slots and copies them at once on the batch's stream ahead
 // of its launches; a slot is written again once the copies from it have run.
 #include <hip/hip_runtime.h>
 
@@ -168,14 +168,15 @@ int run_scenes(mpm_batch_s* b, int32_t n) {
 // tables (most substeps between two sorts) share one slot. With `copy` set
 // (eager substeps) each new table is copied to its device slot on that stream
 // at once; otherwise one copy of the whole buffer heads the recorded cycle.
+// Writes argument tables into pinned host memory (the device copy of the
+// buffer, at the same offsets, is the caller's: one copy of [0, used) once
+// every table of a capture or an eager substep is written).
 struct TableWriter {
   mpm_batch_s* b;
   uint8_t* host;
   uint8_t* dev;
   size_t cap;
-  hipStream_t copy = nullptr;
   size_t used = 0;
-  hipError_t err = hipSuccess;
   std::vector<std::pair<size_t, size_t>> slots;  // (offset, bytes)
   // returns the device address of a table holding `bytes` at `src`, or
   // nullptr (see failed())
@@ -185,18 +186,11 @@ struct TableWriter {
     const size_t off = (used + 255) & ~(size_t)255;
     if (off + bytes > cap) return nullptr;
     std::memcpy(host + off, src, bytes);
-    if (copy) {
-      err = hipMemcpyAsync(dev + off, host + off, bytes, hipMemcpyHostToDevice, copy);
-      if (err != hipSuccess) return nullptr;
-    }
     slots.push_back({off, bytes});
     used = off + bytes;
     return dev + off;
   }
-  int failed() const {
-    return err != hipSuccess ? b->hipfail(err, "batch: argument table copy")
-                             : b->fail(MPM_ERR_CAPACITY, "batch: argument tables overflow");
-  }
+  int failed() const { return b->fail(MPM_ERR_CAPACITY, "batch: argument tables overflow"); }
 };
 
 bool fusable(const mpm_batch_s* b) {
@@ -433,12 +427,12 @@ int run_scenes_fused(mpm_batch_s* b, int32_t n, bool eager) {
   std::vector<mpmk::ItemDesc> done(nsc);
   for (int32_t step = 0; step < n; ++step) {
     int slot = -1;
-    if (eager) {  // the next pinned slot, once the copies queued from it ran
+    if (eager) {  // the next pinned slot, once the copy queued from it ran
       slot = b->eager_next;
       b->eager_next = (slot + 1) % mpm_batch_s::kEagerSlots;
       if (b->eager_pending[slot]) BCHK(hipEventSynchronize(b->eager_ev[slot]), "hipEventSynchronize");
       b->eager_pending[slot] = false;
-      tw = TableWriter{b, b->h_eager[slot], b->d_eager, b->eager_cap, b->origin};
+      tw = TableWriter{b, b->h_eager[slot], b->d_eager, b->eager_cap};
     }
     std::vector<Chain> chains((size_t)nchains);
     for (int c = 0; c < nchains; ++c) {
@@ -446,6 +440,12 @@ int run_scenes_fused(mpm_batch_s* b, int32_t n, bool eager) {
       const int rc = plan_chain(b, tw, chains[c], done);
       if (rc) return rc;
     }
+    // every table of the substep in one copy, ahead of its launches on the
+    // origin stream (the device buffer is shared by the slots: the stream
+    // orders each substep's copy after the previous substep's kernels)
+    if (eager && tw.used)
+      BCHK(hipMemcpyAsync(b->d_eager, b->h_eager[slot], tw.used, hipMemcpyHostToDevice, b->origin),
+           "hipMemcpyAsync(batch tables)");
     if (nchains > 1) {
       BCHK(hipEventRecord(b->fork, b->origin), "hipEventRecord");
       for (int c = 1; c < nchains; ++c)
