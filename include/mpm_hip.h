@@ -363,7 +363,8 @@ int mpm_batch_advance(mpm_batch b, int32_t substeps, void* stream);
  * (1..16, default 4), "fuse" (1: when every scene is in the steady state of
  * the tiled fused mode, record each phase of all scenes as one multi-scene
  * launch; 0: per-scene kernel chains), "multi_sort" (1: the due sorts of all
- * scenes in one launch per sort kernel; 0: each scene's own), "chains"
+ * scenes in one launch per sort kernel; 0: per-scene kernel chains, as with
+ * "fuse" 0), "chains"
  * (1..4, default 2: the scenes split into that many chains of launches,
  * balanced by particles weighted by material kind, each on a stream of its
  * own), "item_share" (1..100, default 100: percent of each scene's estimated

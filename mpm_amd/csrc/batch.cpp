@@ -59,7 +59,7 @@ struct mpm_batch_s {
   int64_t nstreams = 4;
   int64_t use_graph = 1;
   int64_t fuse = 1;
-  int64_t multi_sort = 1;  // fused cycles sort all due scenes in one launch per kernel
+  int64_t multi_sort = 1;  // fused substeps sort all due scenes in one launch per kernel (0: no fusion)
   int64_t chains = 2;      // chains of launches per substep, each on a stream of its own
   int64_t item_share = 100;  // percent of each scene's estimated items given a workgroup of the pass
   int64_t launches = 0;        // graph launches so far (for tests and reports)
@@ -197,7 +197,10 @@ bool fusable(const mpm_batch_s* b) {
   if (!b->fuse) return false;
   for (Sim* s : b->scenes) {
     if (!s->fused_ready()) return false;
-    if (b->multi_sort && !s->multi_sort_ok()) return false;
+    // the fused substep's sorts are multi-scene launches, ordered after the
+    // reductions that read the previous items (a scene's own sort, without
+    // multi_sort, runs in its own kernel chain)
+    if (!b->multi_sort || !s->multi_sort_ok()) return false;
     const int nm = s->material_count();
     if (nm < 1 || nm > MPM_MAX_MATERIALS) return false;
   }
@@ -269,10 +272,9 @@ struct Chain {
 // Plans one substep of the chain's scenes: (1) the tile reductions with the
 // grid update, after the tiles of home blocks with several items were summed
 // into one (small dense scenes hold dozens of items per block); (2) the cell
-// sorts that are due, one launch per sort kernel for all of them (or, without
-// multi_sort, each scene's own sort, issued here on the forked streams: a sort
-// reads the particles only, so it needs no order against the reductions);
-// (3) the fused passes, liquid-only scenes and the others apart.
+// sorts that are due, one launch per sort kernel for all of them (launched
+// after the reductions, which read the items the sorts rebuild); (3) the
+// fused passes, liquid-only scenes and the others apart.
 int plan_chain(mpm_batch_s* b, TableWriter& tw, Chain& c, std::vector<mpmk::ItemDesc>& done) {
   std::vector<mpmk::ReduceDesc> rd;
   std::vector<uint32_t> rp(1, 0u);
@@ -287,7 +289,7 @@ int plan_chain(mpm_batch_s* b, TableWriter& tw, Chain& c, std::vector<mpmk::Item
   c.rtotal = rp.back();
   bool any = false;
   for (size_t j : c.scenes) any = any || b->scenes[j]->sort_due();
-  if (any && b->multi_sort) {
+  if (any) {
     std::vector<mpmk::SortDesc> sd;
     std::vector<Sim*> due;
     for (size_t j : c.scenes) {
@@ -319,20 +321,6 @@ int plan_chain(mpm_batch_s* b, TableWriter& tw, Chain& c, std::vector<mpmk::Item
       c.sorts.push_back({pp.first, pp.second, pt, pre.back()});
     }
     for (Sim* s : due) s->sort_done();
-  } else if (any) {
-    const size_t ns = std::min(b->streams.size(), c.scenes.size());
-    BCHK(hipEventRecord(b->fork, b->origin), "hipEventRecord");
-    for (size_t k = 0; k < ns; ++k) BCHK(hipStreamWaitEvent(b->streams[k], b->fork, 0), "hipStreamWaitEvent");
-    for (size_t q = 0; q < c.scenes.size(); ++q) {
-      Sim* s = b->scenes[c.scenes[q]];
-      if (!s->sort_due()) continue;
-      const int rc = s->sort_now(b->streams[q % ns]);
-      if (rc) return b->fail(rc, "scene %zu: %s", c.scenes[q], s->last_error().c_str());
-    }
-    for (size_t k = 0; k < ns; ++k) {
-      BCHK(hipEventRecord(b->joins[k], b->streams[k]), "hipEventRecord");
-      BCHK(hipStreamWaitEvent(b->origin, b->joins[k], 0), "hipStreamWaitEvent");
-    }
   }
   std::vector<mpmk::ItemDesc> gd[2];
   std::vector<uint32_t> gp[2] = {std::vector<uint32_t>(1, 0u), std::vector<uint32_t>(1, 0u)};
@@ -384,12 +372,12 @@ int launch_chain(mpm_batch_s* b, const Chain& c, hipStream_t st) {
 // streams), balanced by estimated cost: the scenes largest first, each onto
 // the chain of least cost so far, a particle of an SVD material costing 1.7
 // of a liquid-only scene's (the fused passes' measured rates on the batched
-// workload). Without multi_sort: one chain.
+// workload).
 std::vector<std::vector<size_t>> chain_scenes(const mpm_batch_s* b) {
   const size_t nsc = b->scenes.size();
   const int64_t k = std::min<int64_t>({b->chains, (int64_t)b->streams.size() + 1, (int64_t)nsc, 4});
   std::vector<std::vector<size_t>> out;
-  if (k <= 1 || !b->multi_sort) {
+  if (k <= 1) {
     out.emplace_back();
     for (size_t j = 0; j < nsc; ++j) out.back().push_back(j);
     return out;

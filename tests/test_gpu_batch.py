@@ -40,24 +40,27 @@ def run_alone(scs, k):
     return out
 
 
-@pytest.mark.parametrize("graph,fuse", [(1, 1), (1, 0), (0, 1)])
-def test_batch_equals_individual_runs(gpu_lib, graph, fuse):
+@pytest.mark.parametrize("graph,fuse,multi_sort", [(1, 1, 1), (1, 0, 1), (0, 1, 1), (1, 1, 0)])
+def test_batch_equals_individual_runs(gpu_lib, graph, fuse, multi_sort):
     scs = small_batch()
     # 16 eager substeps, two recorded cycles of 32, 20 eager; with "fuse" the
     # eager substeps after the first (which primes each scene) run as
-    # multi-scene launches too
+    # multi-scene launches too; without "multi_sort" each scene keeps its own
+    # kernel chain (its sorts cannot join the multi-scene launches)
     k = 100
     ref = run_alone(scs, k)
     engines = [make(sc) for sc in scs]
     b = Batch(engines)
     b.set_option("graph", graph)
     b.set_option("fuse", fuse)
+    b.set_option("multi_sort", multi_sort)
     b.advance(k)
     outs = [e.download() for e in engines]
     launches = b.get_option("graph_launches")
     fused = b.get_option("fused_captures")
     eager = b.get_option("fused_eager_substeps")
     b.close()
+    fuse = fuse and multi_sort
     assert fused == (1 if graph and fuse else 0)
     if not fuse:
         assert eager == 0
