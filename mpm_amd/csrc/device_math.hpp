@@ -426,7 +426,19 @@ __device__ __noinline__ void svd3_f64(const float* Af, float* U, float* s, float
 __device__ __forceinline__ void svd3_robust(const float* A, float* U, float* s, float* V) {
   svd3(A, U, s, V);
   const bool bad = !(fabsf(s[2]) >= 1e-3f * fabsf(s[0])) || !isfinite(s[0]) || !isfinite(s[2]);
-  if (bad) svd3_f64(A, U, s, V);
+  if (bad) {
+    // The fallback is a call: the arrays whose addresses it takes live in
+    // private memory. These copies are those arrays, so that the caller's
+    // (the particle state, U, s, V) stay in registers on the common path.
+    float a[9], u[9], w[3], v[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) a[i] = A[i];
+    svd3_f64(a, u, w, v);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) { U[i] = u[i]; V[i] = v[i]; }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) s[i] = w[i];
+  }
 }
 
 // ---------------------------------------------------------------------------
