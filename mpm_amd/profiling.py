@@ -12,7 +12,9 @@ Subcommands (run as ``python -m mpm_amd.profiling <cmd>``):
              (kernel trace: calls, mean/min/max duration) or counter CSV files
              (mean value per dispatch of every counter, and the ratios of
              the SQ counters: LDS bank conflicts, instructions per wavefront);
-             with --markers also the roctx ranges of a --marker-trace run
+             with --markers also the roctx ranges of a --marker-trace run;
+             with --busy the time the device ran no kernel between the first
+             and the last launch kept (the gaps between launches)
 
 The first two need a GPU; `summarize` runs anywhere.
 """
@@ -88,10 +90,32 @@ def cmd_copy(a) -> int:
     return 0
 
 
-def summarize_db(path: Path, tail_of: str = "") -> str:
+def busy_line(rows) -> str:
+    """Span from the first start to the last end of the launches (name,
+    start, end in ns), the time some kernel ran (the union of their intervals:
+    launches on several streams overlap) and the rest, when none ran."""
+    iv = sorted((s, e) for _, s, e in rows)
+    if not iv:
+        return "busy: no launches"
+    busy, cur_s, cur_e = 0, iv[0][0], iv[0][1]
+    for s, e in iv[1:]:
+        if s > cur_e:
+            busy += cur_e - cur_s
+            cur_s, cur_e = s, e
+        else:
+            cur_e = max(cur_e, e)
+    busy += cur_e - cur_s
+    span = max(e for _, e in iv) - iv[0][0]
+    idle = span - busy
+    return (f"busy: {len(iv)} launches over {span / 1e3:.1f} us, some kernel running {busy / 1e3:.1f} us, "
+            f"none {idle / 1e3:.1f} us ({100.0 * idle / span if span else 0.0:.2f}%)")
+
+
+def summarize_db(path: Path, tail_of: str = "", busy: bool = False) -> str:
     """Per-kernel durations. tail_of = "KERNEL:N" keeps only the launches from
     the N-th last launch of KERNEL on (the timed region of a run whose warm-up
-    launches other configurations, e.g. autotuning trials)."""
+    launches other configurations, e.g. autotuning trials); with it, busy
+    keeps the launches up to the last launch of KERNEL for the busy line."""
     con = sqlite3.connect(str(path))
     rows = con.execute("select name, start, end from kernels").fetchall()
     head = ""
@@ -103,6 +127,11 @@ def summarize_db(path: Path, tail_of: str = "") -> str:
         t0 = starts[-int(n)]
         rows = [r for r in rows if r[1] >= t0]
         head = f"(launches from the {n}-th last {kname} on)\n"
+        if busy:
+            t1 = max(e for nm, _, e in rows if _clean(nm) == kname)
+            head += busy_line([r for r in rows if r[1] < t1]) + "\n"
+    elif busy:
+        head = busy_line(rows) + "\n"
     per = defaultdict(list)
     for n, s, e in rows:
         per[_clean(n)].append((e - s) / 1000.0)
@@ -174,7 +203,7 @@ def cmd_summarize(a) -> int:
     dbs = [p for p in paths if p.suffix == ".db"]
     csvs = [p for p in paths if p.suffix == ".csv"]
     for p in dbs:
-        print(summarize_db(p, a.tail_of))
+        print(summarize_db(p, a.tail_of, a.busy))
         if a.markers:
             print()
             print(summarize_markers(p))
@@ -206,6 +235,7 @@ def main(argv=None) -> int:
     s.add_argument("--tail-of", default="", help="KERNEL:N -- only launches from the N-th last KERNEL on")
     s.add_argument("--kernels", default="", help="regex: counter tables of the matching kernels only")
     s.add_argument("--markers", action="store_true", help="also the roctx ranges of a --marker-trace run")
+    s.add_argument("--busy", action="store_true", help="also the time no kernel ran between the launches kept")
     a = ap.parse_args(argv)
     return {"run": cmd_run, "mesh": cmd_mesh, "copy": cmd_copy, "summarize": cmd_summarize}[a.cmd](a)
 
