@@ -346,7 +346,9 @@ __global__ void __launch_bounds__(256)
   grid[t] = make_float4(0.f, 0.f, 0.f, 0.f);
   float4 o0 = make_float4(0.f, 0.f, 0.f, c.w), o1 = make_float4(0.f, 0.f, 0.f, 0.f);
   if (c.w > 0.0f) {
-    const int zi = (int)(t % N), yi = (int)((t / N) % N), xi = (int)(t / ((size_t)N * N));
+    // (32-bit: N <= 1024, so a node index is below 2^30)
+    const uint32_t ti = (uint32_t)t, n = (uint32_t)N;
+    const int zi = (int)(ti % n), yi = (int)((ti / n) % n), xi = (int)(ti / (n * n));
     o1.x = c.x / c.w;
     o1.y = __builtin_fmaf(k.dt, k.gravity, c.y / c.w);  // (as grid_node_update rounds it)
     o1.z = c.z / c.w;

@@ -164,13 +164,11 @@ int run_scenes(mpm_batch_s* b, int32_t n) {
 }
 
 // ---- fused cycles ----------------------------------------------------------
-// Tables are laid out in a pinned buffer as they are produced; identical
-// tables (most substeps between two sorts) share one slot. With `copy` set
-// (eager substeps) each new table is copied to its device slot on that stream
-// at once; otherwise one copy of the whole buffer heads the recorded cycle.
-// Writes argument tables into pinned host memory (the device copy of the
-// buffer, at the same offsets, is the caller's: one copy of [0, used) once
-// every table of a capture or an eager substep is written).
+// Writes argument tables into pinned host memory as they are produced;
+// identical tables (most substeps between two sorts) share one slot. The
+// device copy of the buffer, at the same offsets, is the caller's: one copy of
+// [0, used) once every table of a recorded cycle or an eager substep is
+// written.
 struct TableWriter {
   mpm_batch_s* b;
   uint8_t* host;
@@ -405,7 +403,7 @@ std::vector<std::vector<size_t>> chain_scenes(const mpm_batch_s* b) {
 // forked from the origin and joined back every substep: the chains touch
 // disjoint scenes, and each fills the others' gaps and tails (a launch's
 // last, partly occupied round of workgroups).
-int run_scenes_fused(mpm_batch_s* b, int32_t n, bool eager) {
+int run_scenes_fused(mpm_batch_s* b, int32_t n, bool eager, size_t* recorded_bytes = nullptr) {
   const size_t nsc = b->scenes.size();
   // recorded: the tables are copied to the device once, after the capture
   // (capture_and_launch), and stay there for every replay of the cycle
@@ -457,6 +455,7 @@ int run_scenes_fused(mpm_batch_s* b, int32_t n, bool eager) {
       ++b->eager_fused;
     }
   }
+  if (recorded_bytes) *recorded_bytes = eager ? 0 : tw.used;
   return MPM_OK;
 }
 
@@ -503,16 +502,18 @@ int capture_and_launch(mpm_batch_s* b, int32_t cycle) {
   for (Sim* s : b->scenes) sig.push_back(s->capture_signature());
   const bool fused = fusable(b);
   if (fused) {
+    // the previous graph's replays and table copy may still be reading the
+    // table buffers that are written (or replaced) next
+    BCHK(hipStreamSynchronize(b->origin), "hipStreamSynchronize");
     int rc0 = ensure_tables(b, cycle);
     if (rc0) return rc0;
-    // the previous graph's table copy may still be reading the pinned buffer
-    BCHK(hipStreamSynchronize(b->origin), "hipStreamSynchronize");
   }
   std::vector<std::unique_lock<std::mutex>> locks;
   if (fused)
     for (Sim* s : b->scenes) locks.emplace_back(s->mutex());
   BCHK(hipStreamBeginCapture(b->origin, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
-  const int rc = fused ? run_scenes_fused(b, cycle, false) : run_scenes(b, cycle);
+  size_t table_bytes = 0;
+  const int rc = fused ? run_scenes_fused(b, cycle, false, &table_bytes) : run_scenes(b, cycle);
   locks.clear();
   if (fused && !rc) ++b->fused_captures;
   hipGraph_t g = nullptr;
@@ -527,8 +528,8 @@ int capture_and_launch(mpm_batch_s* b, int32_t cycle) {
   b->signature = sig;  // the capture ran the scenes' host logic for one cycle
   // the cycle's argument tables, written while it was recorded: the same for
   // every replay, so one copy (ordered before the launches on the origin stream)
-  if (fused)
-    BCHK(hipMemcpyAsync(b->d_tables, b->h_tables, b->tables_cap, hipMemcpyHostToDevice, b->origin),
+  if (fused && table_bytes)
+    BCHK(hipMemcpyAsync(b->d_tables, b->h_tables, table_bytes, hipMemcpyHostToDevice, b->origin),
          "hipMemcpyAsync(batch tables)");
   BCHK(hipGraphLaunch(b->exec, b->origin), "hipGraphLaunch");
   ++b->launches;
@@ -704,24 +705,46 @@ extern "C" int mpm_batch_autotune(mpm_batch b, int32_t trial_substeps, char* rep
                      "p2g_mode 1)", j);
   // each scene's own choice (its tuner restores its state), then snapshots
   std::vector<mpm_engine::Options> own(ns), saved(ns);
+  // on an error before the trials: the snapshots taken so far dropped, the
+  // scenes' own settings back
+  auto undo = [&](size_t upto, int code) {
+    for (size_t j = 0; j < upto; ++j) {
+      Sim* s = b->scenes[j];
+      std::lock_guard<std::mutex> l2(s->mutex());
+      s->snapshot_drop();
+      s->set_options(saved[j]);
+    }
+    return code;
+  };
   for (size_t j = 0; j < ns; ++j) {
     Sim* s = b->scenes[j];
-    std::lock_guard<std::mutex> l2(s->mutex());
+    std::unique_lock<std::mutex> l2(s->mutex());
     saved[j] = s->options();
     const int rc = s->autotune(trial_substeps, nullptr, user);
-    if (rc) return b->fail(rc, "scene %zu: %s", j, s->last_error().c_str());
+    if (rc) {
+      const int code = b->fail(rc, "scene %zu: %s", j, s->last_error().c_str());
+      l2.unlock();
+      return undo(j, code);
+    }
     own[j] = s->options();
     own[j].radix_bits = 8;  // the batch sorts every scene with 8-bit digits (the same order)
     const int rs = s->snapshot_take(user);
-    if (rs) return b->fail(rs, "scene %zu: %s", j, s->last_error().c_str());
+    if (rs) {
+      const int code = b->fail(rs, "scene %zu: %s", j, s->last_error().c_str());
+      s->set_options(saved[j]);
+      l2.unlock();
+      return undo(j, code);
+    }
   }
   struct Cand { int64_t sort_every, chunk; bool own; double us; };
   std::vector<Cand> cands = {{0, 0, true, 0.0}};
   for (int64_t se : {16, 24, 32, 48, 64, 96, 128}) cands.push_back({se, mpmk::kMaxItemParticles, false, 0.0});
   for (int64_t se : {32, 64}) cands.push_back({se, 128, false, 0.0});
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  BCHK(hipEventCreate(&e0), "hipEventCreate");
-  BCHK(hipEventCreate(&e1), "hipEventCreate");
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return undo(ns, b->fail(MPM_ERR_HIP, "mpm_batch_autotune: hipEventCreate"));
+  }
   auto apply = [&](const Cand& c) {
     for (size_t j = 0; j < ns; ++j) {
       Sim* s = b->scenes[j];
@@ -767,23 +790,28 @@ extern "C" int mpm_batch_autotune(mpm_batch b, int32_t trial_substeps, char* rep
     rc = advance_locked(b, (int32_t)(L / 2 + L), user);
     if (rc) break;
     const int32_t n = (int32_t)(L * ((std::max<int64_t>(trial_substeps, 2 * L) + L - 1) / L));
-    BCHK(hipEventRecord(e0, user), "hipEventRecord");
+    // (errors leave the loop: the scenes are restored below)
+    hipError_t e = hipEventRecord(e0, user);
+    if (e != hipSuccess) { rc = b->hipfail(e, "hipEventRecord"); break; }
     rc = advance_locked(b, n, user);
     if (rc) break;
-    BCHK(hipEventRecord(e1, user), "hipEventRecord");
-    BCHK(hipEventSynchronize(e1), "hipEventSynchronize");
     float ms = 0.0f;
-    BCHK(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    e = hipEventRecord(e1, user);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) { rc = b->hipfail(e, "mpm_batch_autotune: timing"); break; }
     c.us = 1e3 * (double)ms / n;
   }
   const int rr = restore_all();
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  for (Sim* s : b->scenes) s->snapshot_drop();
-  if (rc || rr) {
-    for (size_t j = 0; j < ns; ++j) b->scenes[j]->set_options(saved[j]);
-    return rc ? rc : rr;
+  for (size_t j = 0; j < ns; ++j) {
+    Sim* s = b->scenes[j];
+    std::lock_guard<std::mutex> l2(s->mutex());
+    s->snapshot_drop();
+    if (rc || rr) s->set_options(saved[j]);
   }
+  if (rc || rr) return rc ? rc : rr;
   size_t best = 0;
   for (size_t i = 1; i < cands.size(); ++i)
     if (cands[i].us < cands[best].us) best = i;

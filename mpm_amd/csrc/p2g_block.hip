@@ -1035,7 +1035,8 @@ __global__ void __launch_bounds__(256) k_zero_dead(float4* __restrict__ grid, co
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t nodes = (size_t)N * N * N;
   if (t >= nodes) return;
-  const int z = (int)(t % N), y = (int)((t / N) % N), x = (int)(t / ((size_t)N * N));
+  const uint32_t u = (uint32_t)t, n = (uint32_t)N;  // (N <= 1024: a node index is below 2^30)
+  const int z = (int)(u % n), y = (int)((u / n) % n), x = (int)(u / (n * n));
   const int nbx = (N + 3) / 4;
   if (!live[((size_t)(x >> 2) * nbx + (y >> 2)) * nbx + (z >> 2)]) grid[t] = make_float4(0.f, 0.f, 0.f, 0.f);
 }

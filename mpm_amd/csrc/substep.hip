@@ -267,9 +267,10 @@ __global__ void __launch_bounds__(256)
     if (clear) clear[t] = make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 c = g[t];
     if (!(c.w > 0.0f)) continue;
-    const int zi = (int)(t % N);
-    const int yi = (int)((t / N) % N);
-    const int xi = (int)(t / ((size_t)N * N));
+    const uint32_t u = (uint32_t)t, nu = (uint32_t)N;  // (N <= 1024: a node index is below 2^30)
+    const int zi = (int)(u % nu);
+    const int yi = (int)((u / nu) % nu);
+    const int xi = (int)(u / (nu * nu));
     g[t] = grid_node_update(c, xi, yi, zi, k);
   }
 }
