@@ -200,8 +200,12 @@ __device__ __forceinline__ void scatter_global(const float q0[3][3], const float
         const float wt = wij * w[2][kk];
         float* cell = grid + 4 * (size_t)idx;
         if (dirty && !plain) {
-          const uint32_t gx = idx / ((uint32_t)N * (uint32_t)N), gy = (idx / (uint32_t)N) % (uint32_t)N,
-                         gz = idx % (uint32_t)N;
+          // idx's own coordinates: 0 <= base + offset <= N + 1 on every axis
+          // (base < N inside the domain), so a component at or past N carries
+          // once into the next axis; idx < N^3 leaves gx < N
+          uint32_t gz = (uint32_t)(base[2] + kk), gy = (uint32_t)(base[1] + j), gx = (uint32_t)(base[0] + i);
+          if (gz >= (uint32_t)N) { gz -= (uint32_t)N; ++gy; }
+          if (gy >= (uint32_t)N) { gy -= (uint32_t)N; ++gx; }
           dirty[((gx >> 2) * nbx + (gy >> 2)) * nbx + (gz >> 2)] = 1u;
         }
         atomic_add_global(cell + 0, wt * (q0[0][i] + (q1[0][j] + q2[0][kk])));
