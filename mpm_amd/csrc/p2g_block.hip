@@ -556,119 +556,124 @@ __device__ __forceinline__ void g2p2g_item(const mpmk::ItemDesc& d, const int it
   //    one read-modify-write the writing lanes address distinct nodes; the
   //    compiler barrier keeps the read-modify-writes in program order (a
   //    lane's next read may address a node another lane writes this round).
-  // runs of equal cells (DPP lane shifts; lanes without a tile particle are
-  // runs of their own)
-  const int cid1 = (pending ? cell : 1024 + lane) + 1;
-  const int prev = dpp_wave_shr1(cid1), next = dpp_wave_shl1(cid1);  // 0 beyond the ends
-  const int head = wave_max_scan(prev != cid1 ? lane : 0);
-  const int rank = lane - head;  // position within the run
-  const bool tail = next != cid1;
-  const int maxrun = __builtin_amdgcn_readlane(wave_max_scan(rank + 1), 63);
-  const bool reduce = maxrun > 4;
-  // runs of 2 to 4: one wavefront shift merges pairs, the odd ranks taking
-  // their left neighbour's sums; a run then has at most 2 writers
-  const bool pair = !reduce && maxrun > 1;
-  const bool odd = (rank & 1) != 0;
-  const float take = odd ? 1.0f : 0.0f;
-  const int lr = lane & 15;
+  // A wavefront without tile particles (an item needs ceil(n / 64) of the
+  // workgroup's wavefronts; ~20% of them idle on a liquid scene's boundary)
+  // has nothing to add: its tile stays as cleared.
+  if (__ballot(pending) != 0ull) {
+    // runs of equal cells (DPP lane shifts; lanes without a tile particle are
+    // runs of their own)
+    const int cid1 = (pending ? cell : 1024 + lane) + 1;
+    const int prev = dpp_wave_shr1(cid1), next = dpp_wave_shl1(cid1);  // 0 beyond the ends
+    const int head = wave_max_scan(prev != cid1 ? lane : 0);
+    const int rank = lane - head;  // position within the run
+    const bool tail = next != cid1;
+    const int maxrun = __builtin_amdgcn_readlane(wave_max_scan(rank + 1), 63);
+    const bool reduce = maxrun > 4;
+    // runs of 2 to 4: one wavefront shift merges pairs, the odd ranks taking
+    // their left neighbour's sums; a run then has at most 2 writers
+    const bool pair = !reduce && maxrun > 1;
+    const bool odd = (rank & 1) != 0;
+    const float take = odd ? 1.0f : 0.0f;
+    const int lr = lane & 15;
 
-  typedef __attribute__((address_space(3))) volatile uint8_t lds_byte;
-  lds_byte* const claim = (lds_byte*)(&claims[wave][0]);
-  float4* const my = acc[wave];
-  const StencilSlots slot(l0, l1, l2);
-  // Writers (see step 3 above) claim their cell; the lanes whose claim stands add their nodes,
-  // one slab of 9 nodes (offset a along x) at a time so that 36 sums sit in
-  // registers. Writers that lost (a cell with more than one writer in the
-  // wavefront) retry slab by slab in further claim rounds.
-  const bool writer = pending && (reduce ? tail : (!pair || odd || tail));
-  if (writer) claim[cell] = (uint8_t)lane;
-  const bool mine = writer && claim[cell] == (uint8_t)lane;
-  const bool lost = writer && !mine;
-  const bool any_lost = __ballot(lost) != 0ull;
+    typedef __attribute__((address_space(3))) volatile uint8_t lds_byte;
+    lds_byte* const claim = (lds_byte*)(&claims[wave][0]);
+    float4* const my = acc[wave];
+    const StencilSlots slot(l0, l1, l2);
+    // Writers (see step 3 above) claim their cell; the lanes whose claim stands add their nodes,
+    // one slab of 9 nodes (offset a along x) at a time so that 36 sums sit in
+    // registers. Writers that lost (a cell with more than one writer in the
+    // wavefront) retry slab by slab in further claim rounds.
+    const bool writer = pending && (reduce ? tail : (!pair || odd || tail));
+    if (writer) claim[cell] = (uint8_t)lane;
+    const bool mine = writer && claim[cell] == (uint8_t)lane;
+    const bool lost = writer && !mine;
+    const bool any_lost = __ballot(lost) != 0ull;
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    float cv[9][4];
+    for (int a = 0; a < 3; ++a) {
+      float cv[9][4];
 #pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      const float wab = w[0][a] * w[1][b];
-      const float s0 = q0[0][a] + q1[0][b], s1 = q0[1][a] + q1[1][b], s2 = q0[2][a] + q1[2][b];
+      for (int b = 0; b < 3; ++b) {
+        const float wab = w[0][a] * w[1][b];
+        const float s0 = q0[0][a] + q1[0][b], s1 = q0[1][a] + q1[1][b], s2 = q0[2][a] + q1[2][b];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const int o = b * 3 + c;
-        const float wt = pending ? wab * w[2][c] : 0.0f;
-        cv[o][0] = wt * (s0 + q2[0][c]);
-        cv[o][1] = wt * (s1 + q2[1][c]);
-        cv[o][2] = wt * (s2 + q2[2][c]);
-        cv[o][3] = wt * m.mass;
-        if (pair) {  // odd ranks add their left neighbour (whole-wavefront shift)
-          // as cv += shifted cv * (0 or 1): one multiply-add reading its
-          // first operand through the lane shift (exact: t * 1 = t, t * 0 =
-          // 0). Every lane of the wavefront is active here. The s_nop gives
-          // the two wait states a DPP read needs after the VALU writes of
-          // these registers (the compiler does not pad inside inline asm).
-          __asm__(
-              "s_nop 1\n\t"
-              "v_fmac_f32_dpp %0, %0, %4 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
-              "v_fmac_f32_dpp %1, %1, %4 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
-              "v_fmac_f32_dpp %2, %2, %4 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
-              "v_fmac_f32_dpp %3, %3, %4 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1"
-              : "+v"(cv[o][0]), "+v"(cv[o][1]), "+v"(cv[o][2]), "+v"(cv[o][3])
-              : "v"(take));
+        for (int c = 0; c < 3; ++c) {
+          const int o = b * 3 + c;
+          const float wt = pending ? wab * w[2][c] : 0.0f;
+          cv[o][0] = wt * (s0 + q2[0][c]);
+          cv[o][1] = wt * (s1 + q2[1][c]);
+          cv[o][2] = wt * (s2 + q2[2][c]);
+          cv[o][3] = wt * m.mass;
+          if (pair) {  // odd ranks add their left neighbour (whole-wavefront shift)
+            // as cv += shifted cv * (0 or 1): one multiply-add reading its
+            // first operand through the lane shift (exact: t * 1 = t, t * 0 =
+            // 0). Every lane of the wavefront is active here. The s_nop gives
+            // the two wait states a DPP read needs after the VALU writes of
+            // these registers (the compiler does not pad inside inline asm).
+            __asm__(
+                "s_nop 1\n\t"
+                "v_fmac_f32_dpp %0, %0, %4 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+                "v_fmac_f32_dpp %1, %1, %4 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+                "v_fmac_f32_dpp %2, %2, %4 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+                "v_fmac_f32_dpp %3, %3, %4 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+                : "+v"(cv[o][0]), "+v"(cv[o][1]), "+v"(cv[o][2]), "+v"(cv[o][3])
+                : "v"(take));
+          }
         }
       }
-    }
-    // one step of the segmented scan: cv += (lane shifted by CTRL) * f, f = 1
-    // where COND holds, else 0 (exact as in the pair merge: t * 1 = t, and
-    // t * 0 adds a zero, which leaves every tile sum's bits as they are),
-    // one v_fmac_f32_dpp per value; ROWS masks the 16-lane rows written
+      // one step of the segmented scan: cv += (lane shifted by CTRL) * f, f = 1
+      // where COND holds, else 0 (exact as in the pair merge: t * 1 = t, and
+      // t * 0 adds a zero, which leaves every tile sum's bits as they are),
+      // one v_fmac_f32_dpp per value; ROWS masks the 16-lane rows written
 #define MPM_SEG_NODE(o, CTRL, ROWS)                                                          \
-    __asm__("s_nop 1\n\t"                                                                   \
-            "v_fmac_f32_dpp %0, %0, %4 " CTRL " row_mask:" ROWS " bank_mask:0xf bound_ctrl:1\n\t" \
-            "v_fmac_f32_dpp %1, %1, %4 " CTRL " row_mask:" ROWS " bank_mask:0xf bound_ctrl:1\n\t" \
-            "v_fmac_f32_dpp %2, %2, %4 " CTRL " row_mask:" ROWS " bank_mask:0xf bound_ctrl:1\n\t" \
-            "v_fmac_f32_dpp %3, %3, %4 " CTRL " row_mask:" ROWS " bank_mask:0xf bound_ctrl:1"       \
-            : "+v"(cv[o][0]), "+v"(cv[o][1]), "+v"(cv[o][2]), "+v"(cv[o][3])                  \
-            : "v"(f_));
+      __asm__("s_nop 1\n\t"                                                                   \
+              "v_fmac_f32_dpp %0, %0, %4 " CTRL " row_mask:" ROWS " bank_mask:0xf bound_ctrl:1\n\t" \
+              "v_fmac_f32_dpp %1, %1, %4 " CTRL " row_mask:" ROWS " bank_mask:0xf bound_ctrl:1\n\t" \
+              "v_fmac_f32_dpp %2, %2, %4 " CTRL " row_mask:" ROWS " bank_mask:0xf bound_ctrl:1\n\t" \
+              "v_fmac_f32_dpp %3, %3, %4 " CTRL " row_mask:" ROWS " bank_mask:0xf bound_ctrl:1"       \
+              : "+v"(cv[o][0]), "+v"(cv[o][1]), "+v"(cv[o][2]), "+v"(cv[o][3])                  \
+              : "v"(f_));
 #define MPM_SEG_STEP(CTRL, ROWS, COND)                                                      \
-    {                                                                                      \
-      const float f_ = (COND) ? 1.0f : 0.0f;                                               \
-      _Pragma("unroll") for (int o = 0; o < 9; ++o) { MPM_SEG_NODE(o, CTRL, ROWS) }        \
-    }
-    if (reduce) {
-      MPM_SEG_STEP("row_shr:1", "0xf", rank >= 1 && lr >= 1)
-      MPM_SEG_STEP("row_shr:2", "0xf", rank >= 2 && lr >= 2)
-      MPM_SEG_STEP("row_shr:4", "0xf", rank >= 4 && lr >= 4)
-      if (maxrun > 8) MPM_SEG_STEP("row_shr:8", "0xf", rank >= 8 && lr >= 8)
-      // carry across rows: rows 1, 3 take lane 15 of the row below, rows 2, 3 lane 31
-      MPM_SEG_STEP("row_bcast:15", "0xa", ((lane >> 4) & 1) && rank > lr)
-      MPM_SEG_STEP("row_bcast:31", "0xc", lane >= 32 && rank > lane - 32)
-    }
+      {                                                                                      \
+        const float f_ = (COND) ? 1.0f : 0.0f;                                               \
+        _Pragma("unroll") for (int o = 0; o < 9; ++o) { MPM_SEG_NODE(o, CTRL, ROWS) }        \
+      }
+      if (reduce) {
+        MPM_SEG_STEP("row_shr:1", "0xf", rank >= 1 && lr >= 1)
+        MPM_SEG_STEP("row_shr:2", "0xf", rank >= 2 && lr >= 2)
+        MPM_SEG_STEP("row_shr:4", "0xf", rank >= 4 && lr >= 4)
+        if (maxrun > 8) MPM_SEG_STEP("row_shr:8", "0xf", rank >= 8 && lr >= 8)
+        // carry across rows: rows 1, 3 take lane 15 of the row below, rows 2, 3 lane 31
+        MPM_SEG_STEP("row_bcast:15", "0xa", ((lane >> 4) & 1) && rank > lr)
+        MPM_SEG_STEP("row_bcast:31", "0xc", lane >= 32 && rank > lane - 32)
+      }
 #undef MPM_SEG_STEP
 #undef MPM_SEG_NODE
 #define MPM_RMW_SLAB                                                                       \
-    _Pragma("unroll") for (int b = 0; b < 3; ++b) _Pragma("unroll") for (int c = 0; c < 3; ++c) { \
-      const int o = b * 3 + c;                                                             \
-      float4* const p = my + slot(a, b, c);                                                \
-      float4 t = *p;                                                                       \
-      t.x += cv[o][0];                                                                     \
-      t.y += cv[o][1];                                                                     \
-      t.z += cv[o][2];                                                                     \
-      t.w += cv[o][3];                                                                     \
-      *p = t;                                                                              \
-      __asm__ __volatile__("" ::: "memory");                                               \
-    }
-    if (mine) { MPM_RMW_SLAB }
-    if (any_lost) {
-      bool todo = lost;
-      while (__ballot(todo) != 0ull) {
-        if (todo) claim[cell] = (uint8_t)lane;
-        if (todo && claim[cell] == (uint8_t)lane) {
-          MPM_RMW_SLAB
-          todo = false;
+      _Pragma("unroll") for (int b = 0; b < 3; ++b) _Pragma("unroll") for (int c = 0; c < 3; ++c) { \
+        const int o = b * 3 + c;                                                             \
+        float4* const p = my + slot(a, b, c);                                                \
+        float4 t = *p;                                                                       \
+        t.x += cv[o][0];                                                                     \
+        t.y += cv[o][1];                                                                     \
+        t.z += cv[o][2];                                                                     \
+        t.w += cv[o][3];                                                                     \
+        *p = t;                                                                              \
+        __asm__ __volatile__("" ::: "memory");                                               \
+      }
+      if (mine) { MPM_RMW_SLAB }
+      if (any_lost) {
+        bool todo = lost;
+        while (__ballot(todo) != 0ull) {
+          if (todo) claim[cell] = (uint8_t)lane;
+          if (todo && claim[cell] == (uint8_t)lane) {
+            MPM_RMW_SLAB
+            todo = false;
+          }
         }
       }
-    }
 #undef MPM_RMW_SLAB
+    }
   }
   __syncthreads();
 
