@@ -48,6 +48,10 @@ from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
+# The CPU baseline's OpenMP threads stay on the cores they start on. The
+# OpenMP runtime reads this when it loads (with the first import of torch,
+# which every leg does inside its function), unless the caller set it.
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 HBM_PEAK_GBS = 8000.0
 TIMING_EVERY = 16
@@ -88,7 +92,8 @@ def load_traffic(workload: str, kernel: str):
 
 def host_cpu() -> dict:
     """The host CPU as the OS reports it: model name, logical CPUs, the CPUs
-    this process may run on, and the OpenMP thread budget (OMP_NUM_THREADS)."""
+    this process may run on, the OpenMP thread budget (OMP_NUM_THREADS) and
+    thread binding (OMP_PROC_BIND)."""
     model = "unknown"
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
@@ -103,7 +108,8 @@ def host_cpu() -> dict:
         affinity = os.cpu_count() or 1
     omp = os.environ.get("OMP_NUM_THREADS", "")
     return {"model": model, "logical_cpus": os.cpu_count(), "cpus_allowed": affinity,
-            "omp_num_threads": int(omp) if omp.isdigit() else None}
+            "omp_num_threads": int(omp) if omp.isdigit() else None,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}
 
 
 def cpu_rate(scene, steps: int, threads: int, repeats: int) -> dict:
