@@ -48,10 +48,6 @@ from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
-# The CPU baseline's OpenMP threads stay on the cores they start on. The
-# OpenMP runtime reads this when it loads (with the first import of torch,
-# which every leg does inside its function), unless the caller set it.
-os.environ.setdefault("OMP_PROC_BIND", "close")
 
 HBM_PEAK_GBS = 8000.0
 TIMING_EVERY = 16
@@ -92,8 +88,7 @@ def load_traffic(workload: str, kernel: str):
 
 def host_cpu() -> dict:
     """The host CPU as the OS reports it: model name, logical CPUs, the CPUs
-    this process may run on, the OpenMP thread budget (OMP_NUM_THREADS) and
-    thread binding (OMP_PROC_BIND)."""
+    this process may run on, and the OpenMP thread budget (OMP_NUM_THREADS)."""
     model = "unknown"
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
@@ -108,8 +103,7 @@ def host_cpu() -> dict:
         affinity = os.cpu_count() or 1
     omp = os.environ.get("OMP_NUM_THREADS", "")
     return {"model": model, "logical_cpus": os.cpu_count(), "cpus_allowed": affinity,
-            "omp_num_threads": int(omp) if omp.isdigit() else None,
-            "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}
+            "omp_num_threads": int(omp) if omp.isdigit() else None}
 
 
 def cpu_rate(scene, steps: int, threads: int, repeats: int) -> dict:
@@ -168,6 +162,32 @@ def fp64_check(scene, options: dict, steps: int, threads: int) -> dict:
             "fp32_oracle_vs_fp64": errs(ref32, ref64)}
 
 
+def cpu_rate_lines(workload: str, cpu_steps: int, threads: int) -> list:
+    """The CPU baseline's timing lines: the workload on `threads` threads and
+    on one, the 10,000-particle elastic sphere (N=16, 100 substeps) on both."""
+    from mpm_amd import scenes
+    scene = getattr(scenes, workload)()
+    c1 = scenes.elastic_sphere()
+    return [cpu_rate(scene, max(1, cpu_steps // 5), threads, 5), cpu_rate(scene, 1, 1, 3),
+            cpu_rate(c1, 100, threads, 3), cpu_rate(c1, 20, 1, 3)]
+
+
+def cpu_rate_lines_bound(workload: str, cpu_steps: int, threads: int):
+    """cpu_rate_lines in a child process whose OpenMP threads are bound to
+    the cores they start on (OMP_PROC_BIND=close, unless the caller set a
+    binding). The runtime binds when it loads, the calling thread included,
+    so the binding stays out of this process and its HIP threads."""
+    import subprocess
+    env = dict(os.environ)
+    env.setdefault("OMP_PROC_BIND", "close")
+    r = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--cpu-rates", workload,
+                        "--cpu-steps", str(cpu_steps), "--cpu-threads", str(threads)],
+                       env=env, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError(f"CPU baseline child failed ({r.returncode}): {r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1]), env["OMP_PROC_BIND"]
+
+
 def cpu_baseline_leg(scene, options: dict, cpu_steps: int):
     """The reported CPU baseline (the FP32 C++/OpenMP oracle, `omp atomic`
     scatter) on the host of the GPU: the bench workload on every thread of the
@@ -179,11 +199,8 @@ def cpu_baseline_leg(scene, options: dict, cpu_steps: int):
 
     cpu = host_cpu()
     threads = cpu["omp_num_threads"] or cpu["cpus_allowed"] or 1
-    main = cpu_rate(scene, max(1, cpu_steps // 5), threads, 5)
-    one = cpu_rate(scene, 1, 1, 3)
-    c1 = scenes.elastic_sphere()
-    c1_all = cpu_rate(c1, 100, threads, 3)
-    c1_one = cpu_rate(c1, 20, 1, 3)
+    lines, cpu["omp_proc_bind"] = cpu_rate_lines_bound(scene.name, cpu_steps, threads)
+    main = lines[0]
     snow = scenes.snow_spheres()
     sample = (f"{scene.name}: all {scene.n} particles, N={scene.params.N}, median of 5 runs of "
               f"{main['substeps']} FP32 substeps after 1 warm-up, C++/OpenMP oracle (omp atomic "
@@ -192,7 +209,7 @@ def cpu_baseline_leg(scene, options: dict, cpu_steps: int):
     return {
         "value": main["value"], "unit": "particle-substeps/s", "cores": threads, "kind": "port",
         "sample": sample, "host": cpu,
-        "lines": [main, one, c1_all, c1_one],
+        "lines": lines,
         "accuracy": fp64_check(scene, options, 10, threads),
         "accuracy_snow": fp64_check(snow, {}, 10, threads),
     }
@@ -384,7 +401,13 @@ def main() -> int:
     ap.add_argument("--batched-steps", type=int, default=500)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    # (the CPU baseline's child process: its timing lines as one JSON line)
+    ap.add_argument("--cpu-rates", default="", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-threads", type=int, default=1, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_rates:
+        print(json.dumps(cpu_rate_lines(args.cpu_rates, args.cpu_steps, args.cpu_threads)), flush=True)
+        return 0
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py runs on one GPU only: the engine advances one simulation per GPU and this "
