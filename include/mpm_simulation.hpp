@@ -57,7 +57,7 @@ struct Particle {
 static_assert(sizeof(Particle) == 104, "the reference's particle is 104 bytes");
 
 // include/options.h:6-56, plus the parameters the reference hard-codes
-// (gravity mpm.cu:6, walls mpm.cu:92) and this build's options.
+// (gravity mpm.cu:6, walls mpm.cu:92) and the engine's own options.
 struct CLIOptions {
   float dt = 1e-4f;
   uint32_t N = 60;

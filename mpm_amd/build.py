@@ -67,8 +67,8 @@ def build(verbose: bool = False) -> Path:
     with cf.ThreadPoolExecutor(max_workers=min(len(srcs), os.cpu_count() or 4)) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB.with_suffix(".so.tmp")
-    # --no-undefined: a declaration without a definition fails the link here,
-    # not at load time on the GPU box
+    # --no-undefined: a declaration without a definition fails the link,
+    # not the library's first load
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-mcode-object-version=5", "-shared", "-fPIC",
            "-Wl,--no-undefined", "-o", str(tmp), *map(str, objs), "-lrocprofiler-sdk-roctx", "-lz"]
     res = subprocess.run(cmd, capture_output=True, text=True)

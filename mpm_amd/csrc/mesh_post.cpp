@@ -3,7 +3,7 @@
 // (include/mesh_builder.h:196-246: optional Laplacian smoothing, optional
 // decimation, OBJ output) and its ParticleWriter (particle_writer.h:13-44,
 // Partio's .bgeo). The reference uses libigl and Partio, neither of which is
-// in this build; these are restatements of the operations they perform, on
+// a dependency here; these are restatements of the operations they perform, on
 // plain arrays: vertices as double xyz triples, faces as int32 vertex-index
 // triples.
 #include <zlib.h>

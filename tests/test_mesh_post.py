@@ -15,7 +15,7 @@ the C-ABI; no GPU).
   * mpm_bgeo_write writes the Houdini classic layout (gzip, big-endian
     header, attribute table, homogeneous positions, end bytes) and
     mpm_obj_write an OBJ that reads back.
-libigl and Partio are not in this build: the triangles, the smoothed
+libigl and Partio are not available here: the triangles, the smoothed
 vertices and the collapse order are not compared with theirs (parity by the
 properties above).
 """
