@@ -106,13 +106,13 @@ def host_cpu() -> dict:
             "omp_num_threads": int(omp) if omp.isdigit() else None}
 
 
-def cpu_rate(scene, steps: int, threads: int, repeats: int) -> dict:
+def cpu_rate(scene, steps: int, threads: int, repeats: int, warm: int = 1) -> dict:
     """Particle-substeps/s of the FP32 CPU oracle on `scene`: median of
-    `repeats` timed runs of `steps` substeps each, after one warm-up substep;
-    the state carries over from run to run."""
+    `repeats` timed runs of `steps` substeps each, after `warm` warm-up
+    substeps; the state carries over from run to run."""
     import oracle
     st = scene.state()
-    oracle.time_advance_f32(scene.params, scene.materials, st, 1, threads)  # warm-up substep
+    oracle.time_advance_f32(scene.params, scene.materials, st, warm, threads)  # warm-up
     secs = sorted(oracle.time_advance_f32(scene.params, scene.materials, st, steps, threads)
                   for _ in range(repeats))
     med = secs[len(secs) // 2]
@@ -162,30 +162,35 @@ def fp64_check(scene, options: dict, steps: int, threads: int) -> dict:
             "fp32_oracle_vs_fp64": errs(ref32, ref64)}
 
 
-def cpu_rate_lines(workload: str, cpu_steps: int, threads: int) -> list:
-    """The CPU baseline's timing lines: the workload on `threads` threads and
-    on one, the 10,000-particle elastic sphere (N=16, 100 substeps) on both."""
+def cpu_rate_lines(workload: str, cpu_steps: int, threads: int, c1_steps: int = 100) -> list:
+    """The CPU baseline's timing lines: the workload on `threads` threads (5
+    runs of a fifth of `cpu_steps`) and on one; the 10,000-particle elastic
+    sphere (N=16) on both, median of 5 runs of `c1_steps` substeps after 10
+    warm-up substeps."""
     from mpm_amd import scenes
     scene = getattr(scenes, workload)()
     c1 = scenes.elastic_sphere()
     return [cpu_rate(scene, max(1, cpu_steps // 5), threads, 5), cpu_rate(scene, 1, 1, 3),
-            cpu_rate(c1, 100, threads, 3), cpu_rate(c1, 20, 1, 3)]
+            cpu_rate(c1, c1_steps, threads, 5, warm=10), cpu_rate(c1, c1_steps, 1, 5, warm=10)]
 
 
-def cpu_rate_lines_bound(workload: str, cpu_steps: int, threads: int):
+def cpu_rate_lines_bound(workload: str, cpu_steps: int, threads: int, c1_steps: int = 100):
     """cpu_rate_lines in a child process whose OpenMP threads are bound to
-    the cores they start on (OMP_PROC_BIND=close, unless the caller set a
-    binding). The runtime binds when it loads, the calling thread included,
-    so the binding stays out of this process and its HIP threads."""
+    the cores they start on, one per core (OMP_PROC_BIND=close and
+    OMP_PLACES=cores, unless the caller set them). The runtime binds when it
+    loads, the calling thread included, so the binding stays out of this
+    process and its HIP threads."""
     import subprocess
     env = dict(os.environ)
     env.setdefault("OMP_PROC_BIND", "close")
+    env.setdefault("OMP_PLACES", "cores")
     r = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--cpu-rates", workload,
-                        "--cpu-steps", str(cpu_steps), "--cpu-threads", str(threads)],
+                        "--cpu-steps", str(cpu_steps), "--cpu-threads", str(threads),
+                        "--cpu-c1-steps", str(c1_steps)],
                        env=env, capture_output=True, text=True, timeout=900)
     if r.returncode != 0:
         raise RuntimeError(f"CPU baseline child failed ({r.returncode}): {r.stderr[-2000:]}")
-    return json.loads(r.stdout.strip().splitlines()[-1]), env["OMP_PROC_BIND"]
+    return json.loads(r.stdout.strip().splitlines()[-1]), f"{env['OMP_PROC_BIND']}, places {env['OMP_PLACES']}"
 
 
 def cpu_baseline_leg(scene, options: dict, cpu_steps: int):
@@ -404,9 +409,11 @@ def main() -> int:
     # (the CPU baseline's child process: its timing lines as one JSON line)
     ap.add_argument("--cpu-rates", default="", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-threads", type=int, default=1, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-c1-steps", type=int, default=100, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_rates:
-        print(json.dumps(cpu_rate_lines(args.cpu_rates, args.cpu_steps, args.cpu_threads)), flush=True)
+        print(json.dumps(cpu_rate_lines(args.cpu_rates, args.cpu_steps, args.cpu_threads, args.cpu_c1_steps)),
+              flush=True)
         return 0
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

@@ -20,12 +20,13 @@ def test_cpu_rate_lines_in_a_bound_child():
         pytest.skip("the oracle library is not built")
     import bench
     before = os.sched_getaffinity(0)
-    lines, bind = bench.cpu_rate_lines_bound("elastic_sphere", 5, 1)
+    lines, bind = bench.cpu_rate_lines_bound("elastic_sphere", 5, 1, c1_steps=4)
     assert os.sched_getaffinity(0) == before
-    assert bind == os.environ.get("OMP_PROC_BIND", "close")
-    assert [(l["workload"], l["threads"], l["substeps"]) for l in lines] == [
-        ("elastic_sphere", 1, 1), ("elastic_sphere", 1, 1), ("elastic_sphere", 1, 100),
-        ("elastic_sphere", 1, 20)]
+    assert bind == (f"{os.environ.get('OMP_PROC_BIND', 'close')}, places "
+                    f"{os.environ.get('OMP_PLACES', 'cores')}")
+    assert [(l["workload"], l["threads"], l["substeps"], l["repeats"]) for l in lines] == [
+        ("elastic_sphere", 1, 1, 5), ("elastic_sphere", 1, 1, 3), ("elastic_sphere", 1, 4, 5),
+        ("elastic_sphere", 1, 4, 5)]
     for l in lines:
         assert l["particles"] == 10000 and l["value"] > 0 and l["median_s"] > 0
 
