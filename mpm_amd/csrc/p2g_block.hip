@@ -939,7 +939,11 @@ __global__ void __launch_bounds__(kCombineThreads) k_tile_combine(mpmk::ReduceDe
 // workgroup w combines the blocks whose index in it is w modulo the grid.
 // Every wavefront reads the scenes' counts lane by lane (64 scenes per round
 // trip) and their running sum, so all 4 take the same (uniform) loop.
-constexpr int kCombineGrid = 1024;
+// 512 workgroups: in a batch the combine runs beside the other chain's fused
+// pass, and a larger grid takes it more of the chip than it gains (64 scenes:
+// 1,024 workgroups cost ~1% per substep, 2,048 ~3%, 4,096 ~7%; 256 slows the
+// single-scene combine of dense scenes).
+constexpr int kCombineGrid = 512;
 __global__ void __launch_bounds__(kCombineThreads)
     k_tile_combine_multi(const mpmk::ReduceDesc* __restrict__ descs, int nscenes) {
   const int lane = (int)(threadIdx.x & 63u);
