@@ -272,8 +272,10 @@ struct Chain {
 // into one (small dense scenes hold dozens of items per block); (2) the cell
 // sorts that are due, one launch per sort kernel for all of them (launched
 // after the reductions, which read the items the sorts rebuild); (3) the
-// fused passes, liquid-only scenes and the others apart.
-int plan_chain(mpm_batch_s* b, TableWriter& tw, Chain& c, std::vector<mpmk::ItemDesc>& done) {
+// fused passes, liquid-only scenes and the others apart. Work a scene
+// enqueues while it is planned (its F planes written out) goes to `st`, the
+// stream of the chain's launches.
+int plan_chain(mpm_batch_s* b, TableWriter& tw, Chain& c, std::vector<mpmk::ItemDesc>& done, hipStream_t st) {
   std::vector<mpmk::ReduceDesc> rd;
   std::vector<uint32_t> rp(1, 0u);
   for (size_t j : c.scenes) {
@@ -326,7 +328,7 @@ int plan_chain(mpm_batch_s* b, TableWriter& tw, Chain& c, std::vector<mpmk::Item
     Sim* s = b->scenes[j];
     int kinds = 0;
     int64_t launch = 0;
-    const int rc = s->fused_pass(s->velocity_grid(), 1, done[j], kinds, launch, b->origin);
+    const int rc = s->fused_pass(s->velocity_grid(), 1, done[j], kinds, launch, st);
     if (rc) return b->fail(rc, "scene %zu: %s", j, s->last_error().c_str());
     const int g = kinds ? 1 : 0;
     gd[g].push_back(done[j]);
@@ -400,9 +402,12 @@ std::vector<std::vector<size_t>> chain_scenes(const mpm_batch_s* b) {
 // `n` substeps of every scene as multi-scene launches on the origin stream:
 // recorded (the stream is being captured) or, with `eager`, run at once.
 // With several chains (chain_scenes) chain c > 0 runs on batch stream c - 1,
-// forked from the origin and joined back every substep: the chains touch
-// disjoint scenes, and each fills the others' gaps and tails (a launch's
-// last, partly occupied round of workgroups).
+// forked from the origin: the chains touch disjoint scenes, and each fills
+// the others' gaps and tails (a launch's last, partly occupied round of
+// workgroups). A recorded cycle forks once and joins once, so the chains'
+// substeps drift apart and one chain's grid update meets the other's fused
+// pass; an eager substep joins at its end (the next one's table copy
+// overwrites the device tables its launches read).
 int run_scenes_fused(mpm_batch_s* b, int32_t n, bool eager, size_t* recorded_bytes = nullptr) {
   const size_t nsc = b->scenes.size();
   // recorded: the tables are copied to the device once, after the capture
@@ -411,7 +416,23 @@ int run_scenes_fused(mpm_batch_s* b, int32_t n, bool eager, size_t* recorded_byt
   const std::vector<std::vector<size_t>> groups = chain_scenes(b);
   const int nchains = (int)groups.size();
   std::vector<mpmk::ItemDesc> done(nsc);
-  for (int32_t step = 0; step < n; ++step) {
+  std::vector<hipStream_t> st((size_t)nchains);
+  for (int c = 0; c < nchains; ++c) st[c] = c == 0 ? b->origin : b->streams[c - 1];
+  auto fork = [&]() -> int {
+    BCHK(hipEventRecord(b->fork, b->origin), "hipEventRecord");
+    for (int c = 1; c < nchains; ++c) BCHK(hipStreamWaitEvent(st[c], b->fork, 0), "hipStreamWaitEvent");
+    return MPM_OK;
+  };
+  auto join = [&]() -> int {
+    for (int c = 1; c < nchains; ++c) {
+      BCHK(hipEventRecord(b->joins[c - 1], st[c]), "hipEventRecord");
+      BCHK(hipStreamWaitEvent(b->origin, b->joins[c - 1], 0), "hipStreamWaitEvent");
+    }
+    return MPM_OK;
+  };
+  // one substep of every chain (its errors return at once; a recording is
+  // joined below whatever they are, so that its capture can end)
+  auto substep = [&]() -> int {
     int slot = -1;
     if (eager) {  // the next pinned slot, once the copy queued from it ran
       slot = b->eager_next;
@@ -423,7 +444,7 @@ int run_scenes_fused(mpm_batch_s* b, int32_t n, bool eager, size_t* recorded_byt
     std::vector<Chain> chains((size_t)nchains);
     for (int c = 0; c < nchains; ++c) {
       chains[c].scenes = groups[c];
-      const int rc = plan_chain(b, tw, chains[c], done);
+      const int rc = plan_chain(b, tw, chains[c], done, st[c]);
       if (rc) return rc;
     }
     // every table of the substep in one copy, ahead of its launches on the
@@ -432,18 +453,17 @@ int run_scenes_fused(mpm_batch_s* b, int32_t n, bool eager, size_t* recorded_byt
     if (eager && tw.used)
       BCHK(hipMemcpyAsync(b->d_eager, b->h_eager[slot], tw.used, hipMemcpyHostToDevice, b->origin),
            "hipMemcpyAsync(batch tables)");
-    if (nchains > 1) {
-      BCHK(hipEventRecord(b->fork, b->origin), "hipEventRecord");
-      for (int c = 1; c < nchains; ++c)
-        BCHK(hipStreamWaitEvent(b->streams[c - 1], b->fork, 0), "hipStreamWaitEvent");
-    }
-    for (int c = 0; c < nchains; ++c) {
-      const int rc = launch_chain(b, chains[c], c == 0 ? b->origin : b->streams[c - 1]);
+    if (eager && nchains > 1) {
+      const int rc = fork();
       if (rc) return rc;
     }
-    for (int c = 1; c < nchains; ++c) {
-      BCHK(hipEventRecord(b->joins[c - 1], b->streams[c - 1]), "hipEventRecord");
-      BCHK(hipStreamWaitEvent(b->origin, b->joins[c - 1], 0), "hipStreamWaitEvent");
+    for (int c = 0; c < nchains; ++c) {
+      const int rc = launch_chain(b, chains[c], st[c]);
+      if (rc) return rc;
+    }
+    if (eager && nchains > 1) {
+      const int rc = join();
+      if (rc) return rc;
     }
     for (size_t j = 0; j < nsc; ++j) {
       b->scenes[j]->fused_pass_done(done[j]);
@@ -454,7 +474,16 @@ int run_scenes_fused(mpm_batch_s* b, int32_t n, bool eager, size_t* recorded_byt
       b->eager_pending[slot] = true;
       ++b->eager_fused;
     }
+    return MPM_OK;
+  };
+  const bool once = !eager && nchains > 1;
+  int rc = once ? fork() : MPM_OK;
+  for (int32_t step = 0; step < n && !rc; ++step) rc = substep();
+  if (once) {
+    const int rj = join();
+    if (!rc) rc = rj;
   }
+  if (rc) return rc;
   if (recorded_bytes) *recorded_bytes = eager ? 0 : tw.used;
   return MPM_OK;
 }
