@@ -365,7 +365,7 @@ int mpm_batch_advance(mpm_batch b, int32_t substeps, void* stream);
  * launch; 0: per-scene kernel chains), "multi_sort" (1: the due sorts of all
  * scenes in one launch per sort kernel; 0: per-scene kernel chains, as with
  * "fuse" 0), "chains"
- * (1..4, default 2: the scenes split into that many chains of launches,
+ * (1..4, default 4: the scenes split into that many chains of launches,
  * balanced by particles weighted by material kind, each on a stream of its
  * own), "item_share" (1..100, default 100: percent of each scene's estimated
  * work items given a workgroup of the fused pass, whose workgroups take

@@ -20,7 +20,7 @@
 // tile combine and reduction over the node blocks of the scenes, the due cell
 // sorts (in one launch per sort kernel), and a fused G2P/P2G pass over their
 // work items (one per kernel: liquid-only scenes and the others apart). The
-// scenes are split into a few chains (option "chains", default 2), balanced
+// scenes are split into a few chains (option "chains", default 4), balanced
 // by estimated cost, whose launches go to streams of their own and overlap.
 // A workgroup finds its scene in a table of per-scene arguments (the same
 // arguments the scene's own launch would take, so the results are the same
@@ -60,7 +60,7 @@ struct mpm_batch_s {
   int64_t use_graph = 1;
   int64_t fuse = 1;
   int64_t multi_sort = 1;  // fused substeps sort all due scenes in one launch per kernel (0: no fusion)
-  int64_t chains = 2;      // chains of launches per substep, each on a stream of its own
+  int64_t chains = 4;      // chains of launches, each on a stream of its own
   int64_t item_share = 100;  // percent of each scene's estimated items given a workgroup of the pass
   int64_t launches = 0;        // graph launches so far (for tests and reports)
   int64_t fused_captures = 0;  // cycles recorded as multi-scene launches

@@ -81,6 +81,28 @@ def test_batch_equals_individual_runs(gpu_lib, graph, fuse, multi_sort):
         e.close()
 
 
+@pytest.mark.parametrize("chains", [1, 2, 3])
+def test_batch_chains_equal_individual_runs(gpu_lib, chains):
+    """The scenes split into 1-3 chains of launches on streams of their own
+    (the default, 4, is the other tests'): each scene's particles as alone."""
+    scs = small_batch()
+    k = 100
+    ref = run_alone(scs, k)
+    engines = [make(sc) for sc in scs]
+    b = Batch(engines)
+    b.set_option("chains", chains)
+    assert b.get_option("chains") == chains
+    b.advance(k)
+    outs = [e.download() for e in engines]
+    assert b.get_option("fused_captures") == 1
+    b.close()
+    for e, o, r in zip(engines, outs, ref):
+        assert e.time() == r["t"]
+        for f in ("x", "v", "C", "F", "Jp"):
+            assert np.array_equal(o[f], r[f]), f
+        e.close()
+
+
 def test_batch_recaptures_after_a_change(gpu_lib):
     scs = small_batch()[:2]
     engines = [make(sc) for sc in scs]
