@@ -8,7 +8,10 @@ checkpointing: the forward keeps the state every `checkpoint_every`
 substeps; the backward recomputes each segment's states and grid tapes from
 its checkpoint (mpm_diff_forward_steps), then runs the segment's adjoints in
 reverse (mpm_diff_backward_steps). Each segment is one call into the
-library, which enqueues all of its kernels (two per substep each way).
+library, which enqueues all of its kernels (two per substep each way). The
+recompute of segment s - 1 runs on a second stream while segment s's
+adjoints run (two buffers of states and tapes): a cube of a few ten
+thousand particles fills half the SIMDs, and each pass is latency-bound.
 
 Inside the Function the particles are ordered by the grid cell of their
 initial position (a stable sort, undone on the way out, gradients included):
@@ -21,8 +24,10 @@ with the MMFixedCorotated plugin (MaterialModel.cuh:56-61).
 State layout: one float32 tensor of shape (24, n) on the GPU, planes
 x0 x1 x2, v0 v1 v2, C (9, row-major), F (9, row-major). `pack_state` and
 `unpack_state` convert from and to per-field tensors. All work goes on
-torch's current HIP stream. The scatters add with FP32 atomics, so a
-recomputed segment can differ from the first forward in the last bits.
+torch's current HIP stream (the backward's recomputes on a side stream of
+default priority, joined before it returns). The scatters add with FP32
+atomics, so a recomputed segment can differ from the first forward in the
+last bits.
 """
 from __future__ import annotations
 
@@ -91,6 +96,7 @@ class DiffStepper:
         self.grid = torch.empty(self.grid_floats, device=self.device, dtype=torch.float32)
         self._grid2 = None
         self._work = None
+        self._side = None  # the backward's recompute stream
 
     def _stream(self):
         return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
@@ -185,17 +191,50 @@ class _Rollout(torch.autograd.Function):
         adj_mu = torch.zeros_like(mu)
         adj_lam = torch.zeros_like(lam)
         k_max = min(every, steps)
-        states = torch.empty((k_max + 1, PLANES, stepper.n), device=adj.device, dtype=torch.float32)
-        tapes = stepper.new_tapes(k_max)
         nseg = len(ctx.checkpoints)
-        for s in range(nseg - 1, -1, -1):
+        # two buffers: segment s's adjoints read one while segment s - 1 is
+        # recomputed into the other on the side stream
+        nbuf = 2 if nseg > 1 else 1
+        states = [torch.empty((k_max + 1, PLANES, stepper.n), device=adj.device, dtype=torch.float32)
+                  for _ in range(nbuf)]
+        tapes = [stepper.new_tapes(k_max) for _ in range(nbuf)]
+        main = torch.cuda.current_stream(adj.device)
+        if stepper._side is None:
+            stepper._side = torch.cuda.Stream(device=adj.device)
+        side = stepper._side
+        side.wait_stream(main)  # the forward's use of the scatter grid, the inputs
+        ready = [torch.cuda.Event() for _ in range(nbuf)]  # recompute done (side)
+        free = [torch.cuda.Event() for _ in range(nbuf)]   # adjoints done reading (main)
+        freed = [False] * nbuf
+
+        def seg_len(s):
             k0 = s * every
-            k = min(steps, k0 + every) - k0
-            # recompute the segment: states[j] is the state before substep j
-            states[0].copy_(ctx.checkpoints[s])
-            stepper.forward_steps(states[0], states[1:k + 1], mu, lam, tapes)
-            stepper.backward_steps(states[:k], mu, lam, tapes, adj, adj_next, adj_mu, adj_lam)
+            return min(steps, k0 + every) - k0
+
+        def recompute(s, b):
+            # states[b][j] is the state before substep j of segment s
+            k = seg_len(s)
+            with torch.cuda.stream(side):
+                if freed[b]:
+                    side.wait_event(free[b])
+                states[b][0].copy_(ctx.checkpoints[s])
+                stepper.forward_steps(states[b][0], states[b][1:k + 1], mu, lam, tapes[b])
+                ready[b].record(side)
+
+        recompute(nseg - 1, 0)
+        for s in range(nseg - 1, -1, -1):
+            b = (nseg - 1 - s) % nbuf
+            if s > 0:
+                recompute(s - 1, (b + 1) % nbuf)
+            main.wait_event(ready[b])
+            k = seg_len(s)
+            stepper.backward_steps(states[b][:k], mu, lam, tapes[b], adj, adj_next, adj_mu, adj_lam)
+            free[b].record(main)
+            freed[b] = True
             adj, adj_next = adj_next, adj
+        main.wait_stream(side)
+        for t in states + tapes:
+            t.record_stream(side)
         ctx.checkpoints = None
         return None, None, None, adj, adj_mu, adj_lam
 
