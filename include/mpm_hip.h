@@ -346,7 +346,9 @@ int mpm_debug_selftest(char* buf, int32_t len);
  * once into a HIP graph -- every scene's kernels on one of a few streams
  * forked from the batch's own stream, so independent scenes run concurrently
  * -- and replayed with one graph launch per cycle of 2 sort intervals. The
- * results equal advancing each handle on its own, bit for bit. Handles stay
+ * results equal advancing each handle on its own: bit for bit while no
+ * particle takes the direct path (at the grid faces, or drifted past its
+ * tile between sorts), whose global atomics add in hardware order. Handles stay
  * owned by the caller and must outlive the batch; calls on a handle between
  * batch advances are allowed (a changed scene is recorded again). */
 typedef struct mpm_batch_s* mpm_batch;
