@@ -96,7 +96,9 @@ class DiffStepper:
         self.grid = torch.empty(self.grid_floats, device=self.device, dtype=torch.float32)
         self._grid2 = None
         self._work = None
-        self._side = None  # the backward's recompute stream
+        # the backward's recompute stream (default priority), made here rather
+        # than inside a backward that a caller may be recording into a graph
+        self._side = torch.cuda.Stream(device=self.device)
 
     def _stream(self):
         return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
@@ -199,8 +201,6 @@ class _Rollout(torch.autograd.Function):
                   for _ in range(nbuf)]
         tapes = [stepper.new_tapes(k_max) for _ in range(nbuf)]
         main = torch.cuda.current_stream(adj.device)
-        if stepper._side is None:
-            stepper._side = torch.cuda.Stream(device=adj.device)
         side = stepper._side
         side.wait_stream(main)  # the forward's use of the scatter grid, the inputs
         ready = [torch.cuda.Event() for _ in range(nbuf)]  # recompute done (side)
