@@ -174,8 +174,15 @@ class _Rollout(torch.autograd.Function):
         done = 0
         while done < steps:
             k = min(every, steps - done)
-            stepper.forward_steps(cur, seg[:k], mu, lam)
-            cur = seg[k - 1].clone()
+            # the segment's last state goes straight into a tensor of its own
+            # (the next checkpoint, or the output): no copy out of `seg`
+            nxt = torch.empty((PLANES, stepper.n), device=st0.device, dtype=torch.float32)
+            if k > 1:
+                stepper.forward_steps(cur, seg[:k - 1], mu, lam)
+                stepper.forward_steps(seg[k - 2], nxt.unsqueeze(0), mu, lam)
+            else:
+                stepper.forward_steps(cur, nxt.unsqueeze(0), mu, lam)
+            cur = nxt
             done += k
             if done < steps:
                 checkpoints.append(cur)
